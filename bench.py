@@ -1,0 +1,167 @@
+"""HPCG PCG throughput on one MI355X.
+
+A step is one PCG iteration (CG.cpp:100-130 of the reference): one V-cycle
+(4 levels, 1 pre + 1 post symmetric Gauss-Seidel sweep, 8-colour ordering),
+one SpMV, two dot products fused into their producers, the x/r/p updates.
+The timed region runs K iterations as one CG call from x = 0 with tolerance
+0, like the reference's timed sets (main.cpp:334-340), inputs resident in HBM.
+
+Prints one JSON line:
+  value      HPCG GFLOP/s by the reference's flop model (ReportResults.cpp:75-99)
+  roofline   the finest level's colour-step Gauss-Seidel kernel: algorithmic
+             bytes per launch / mean launch time (HIP events on its stream)
+  cpu_baseline  the reference itself (oracle/_ref/ref_time, OpenMP on the host
+             cores), one CG_ref set on a bounded grid
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50, help="timed PCG iterations")
+    ap.add_argument("--warmup", type=int, default=3, help="untimed PCG iterations")
+    ap.add_argument("--nx", type=int, default=512)
+    ap.add_argument("--ny", type=int, default=512)
+    ap.add_argument("--nz", type=int, default=640)
+    ap.add_argument("--ordering", default="multicolor", choices=["multicolor", "lexicographic"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-grid", type=int, default=128, help="grid edge of the CPU sample")
+    ap.add_argument("--cpu-iters", type=int, default=25, help="CG_ref iterations of the CPU sample")
+    return ap.parse_args()
+
+
+def cpu_baseline(grid, iters):
+    """Time the reference (compiled from its own sources into oracle/_ref)
+    on the host cores; fall back to the single-threaded oracle port."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_time")
+    if os.path.exists(exe):
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="spread")
+        out = subprocess.run([exe, "time", str(grid), str(grid), str(grid), str(iters)],
+                             capture_output=True, text=True, env=env, timeout=600, check=True)
+        r = json.loads(out.stdout.strip().splitlines()[-1])
+        return {"value": round(r["gflops"], 4), "unit": "GFLOP/s", "cores": threads,
+                "kind": "reference",
+                "sample": "reference CG_ref, %d^3, %d iterations, 4-level MG, %.1f s "
+                          "(HPCG flop model); serial SYMGS dominates" % (grid, iters, r["seconds"])}
+    sys.path.insert(0, ROOT)
+    from oracle import pyoracle as O
+    from hpcg_amd import model
+    g = min(grid, 64)
+    h = O.Hierarchy(g, g, g)
+    t0 = time.perf_counter()
+    h.cg(max_iter=iters, tol=0.0)
+    dt = time.perf_counter() - t0
+    lv = model.grid_levels(g, g, g)
+    fl = model.hpcg_model([l["n"] for l in lv], [l["nnz"] for l in lv], 1, iters, iters)["total"]
+    return {"value": round(fl / dt * 1e-9, 4), "unit": "GFLOP/s", "cores": 1, "kind": "port",
+            "sample": "oracle port CG, %d^3, %d iterations, %.1f s" % (g, iters, dt)}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import hpcg_amd
+    from hpcg_amd import model
+
+    def log(*m):
+        print("[bench]", *m, file=sys.stderr, flush=True)
+
+    torch.cuda.init()
+    log("generate %dx%dx%d on device" % (a.nx, a.ny, a.nz))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    P = hpcg_amd.Problem.generate(a.nx, a.ny, a.nz, nlevels=4)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    b, x, _ = P.vectors()
+    log("setup %.2f s, device memory %.1f GB" % (setup_s, P.memory_use() / 1e9))
+
+    if a.warmup > 0:
+        P.cg(b, x, max_iter=a.warmup, tolerance=0.0, ordering=a.ordering)
+    x.zero_()
+    P.kernel_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = P.cg(b, x, max_iter=a.steps, tolerance=0.0, ordering=a.ordering)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    P.kernel_timing(False)
+    ks_ms, ks_n = P.kernel_timing_read()
+    log("timed %d iterations in %.3f s; normr/normr0 %.3e" % (r["niters"], dt, r["normr"] / r["normr0"]))
+
+    lv = model.grid_levels(a.nx, a.ny, a.nz)
+    nrow = [l["n"] for l in lv]
+    nnz = [l["nnz"] for l in lv]
+    K = r["niters"]
+    flops = model.hpcg_model(nrow, nnz, 1, K, K)
+    gflops = flops["total"] / dt * 1e-9
+    cb = model.canonical_bytes(lv)
+    pre_bytes = cb["outer_spmv"] + 7 * 8 * lv[0]["n"]  # preamble: copy, SpMV, r = b - Ap, r.r
+    gbs = (K * cb["iteration"] + pre_bytes) / dt * 1e-9
+    # dominant kernel: one colour step of the finest level's Gauss-Seidel
+    info0 = P.info(0)
+    per_launch = cb["symgs_L0"] / (2 * info0["ncolors"])
+    launch_ms = ks_ms / max(ks_n, 1)
+    achieved = per_launch / (launch_ms * 1e-3) * 1e-9 if ks_n else None
+    rating = flops["total"] / (dt + setup_s / 10.0) * 1e-9  # ReportResults.cpp:353, one set
+
+    out = {
+        "metric": "HPCG GFLOP/s (FP64 PCG, 4-level MG, %s SYMGS)" % a.ordering,
+        "value": round(gflops, 3),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": a.warmup,
+        "ms_per_step": round(dt / K * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: HPCG 27-point problem generated on the device, b = A*1, x0 = 0",
+        "config": {"workload": "HPCG PCG %dx%dx%d FP64, 4 MG levels, %s Gauss-Seidel"
+                               % (a.nx, a.ny, a.nz, a.ordering),
+                   "nx": a.nx, "ny": a.ny, "nz": a.nz, "mg_levels": 4,
+                   "ordering": a.ordering, "rows": lv[0]["n"], "nnz": lv[0]["nnz"]},
+        "iteration_gbs_canonical": round(gbs, 1),
+        "iteration_hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+        "rating_with_setup_gflops": round(rating, 3),
+        "setup_s": round(setup_s, 3),
+        "normr_over_normr0": r["normr"] / r["normr0"],
+        "roofline": {
+            "kernel": "k_gs_range<true> (finest-level colour step)",
+            "bound": "hbm",
+            "achieved": round(achieved, 1) if achieved else None,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+            "traffic": None,
+            "bytes_per_launch": per_launch,
+            "mean_launch_ms": round(launch_ms, 5),
+            "launches": ks_n,
+        },
+    }
+    if not a.no_cpu_baseline:
+        log("cpu baseline")
+        try:
+            out["cpu_baseline"] = cpu_baseline(a.cpu_grid, a.cpu_iters)
+        except Exception as e:  # reported, never fatal to the GPU number
+            out["cpu_baseline"] = {"value": None, "error": str(e)[:200]}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

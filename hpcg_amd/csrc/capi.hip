@@ -1,0 +1,414 @@
+// capi.hip -- the extern "C" entry points of libhpcg_amd.so (include/hpcg_amd.h).
+// Exceptions never cross this boundary: each entry point maps them to an
+// HPCGX_E* code and records the message for hpcgx_last_error().
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+struct hpcgx_problem {
+  hpcgx::Problem P;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guard(F &&f) {
+  try {
+    f();
+    return HPCGX_OK;
+  } catch (const hpcgx::Error &e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_err = "host allocation failed";
+    return HPCGX_ENOMEM;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return HPCGX_EHIP;
+  } catch (...) {
+    g_err = "unknown failure";
+    return HPCGX_EHIP;
+  }
+}
+
+inline hipStream_t S(void *s) { return static_cast<hipStream_t>(s); }
+
+using hpcgx::Error;
+using hpcgx::kSlots;
+
+hpcgx::Level &level_of(hpcgx_problem *p, int l) {
+  if (!p) throw Error(HPCGX_EINVAL, "NULL problem");
+  if (l < 0 || l >= p->P.nlevels) throw Error(HPCGX_EINVAL, "level out of range");
+  return p->P.L[l];
+}
+const hpcgx::Level &level_of(const hpcgx_problem *p, int l) {
+  return level_of(const_cast<hpcgx_problem *>(p), l);
+}
+void check_order(int o) {
+  if (o != HPCGX_ORDER_LEXICOGRAPHIC && o != HPCGX_ORDER_MULTICOLOR)
+    throw Error(HPCGX_EINVAL, "unknown ordering");
+}
+
+struct DevBuf {  // device buffer freed on scope exit
+  void *p = nullptr;
+  void alloc(size_t bytes) { HPCGX_HIP(hipMalloc(&p, bytes ? bytes : 1)); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+  }
+  ~DevBuf() { reset(); }
+};
+
+struct NaturalOwner {  // natural-order level plus the buffers behind it
+  hpcgx::NaturalLevel N;
+  DevBuf col, val, nnz, f2c;
+  void alloc(int64_t n, int64_t nc) {
+    col.alloc(sizeof(int32_t) * kSlots * n);
+    val.alloc(sizeof(double) * kSlots * n);
+    nnz.alloc(n);
+    N.col = static_cast<int32_t *>(col.p);
+    N.val = static_cast<double *>(val.p);
+    N.nnz_row = static_cast<uint8_t *>(nnz.p);
+    N.n = n;
+    N.nc = nc;
+    if (nc > 0) {
+      f2c.alloc(sizeof(int32_t) * nc);
+      N.f2c = static_cast<int32_t *>(f2c.p);
+    }
+  }
+  void drop_matrix() {  // keep only f2c (needed to link levels)
+    col.reset();
+    val.reset();
+    nnz.reset();
+    N.col = nullptr;
+    N.val = nullptr;
+    N.nnz_row = nullptr;
+  }
+};
+
+void alloc_solver_state(hpcgx::Problem &P) {
+  const int64_t n = P.L[0].n;
+  P.r = P.arena.alloc<double>(n);
+  P.z = P.arena.alloc<double>(n);
+  P.p = P.arena.alloc<double>(n);
+  P.Ap = P.arena.alloc<double>(n);
+  P.partials = P.arena.alloc<double>(hpcgx::kReduceBlocks);
+  P.ticket = P.arena.alloc<unsigned>(1);
+  P.scalars = P.arena.alloc<double>(hpcgx::kNumScalars);
+  HPCGX_HIP(hipMemset(P.ticket, 0, sizeof(unsigned)));
+  HPCGX_HIP(hipMemset(P.scalars, 0, sizeof(double) * hpcgx::kNumScalars));
+}
+
+// shared tail of both setup paths: schedules, reordering, links
+void build_levels(hpcgx::Problem &P, std::vector<NaturalOwner> &nat, int flags,
+                  hipStream_t s) {
+  for (int l = 0; l < P.nlevels; ++l) {
+    hpcgx::finalize_level(P, l, nat[l].N, flags, s);
+    nat[l].drop_matrix();
+  }
+  for (int l = 0; l + 1 < P.nlevels; ++l) {
+    if (nat[l].N.nc != P.L[l + 1].n)
+      throw Error(HPCGX_EINVAL, "f2c length differs from the next level's rows");
+    hpcgx::link_levels(P, l, nat[l].N, s);
+  }
+  HPCGX_HIP(hipStreamSynchronize(s));
+  alloc_solver_state(P);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *hpcgx_last_error(void) { return g_err.c_str(); }
+int hpcgx_abi_version(void) { return HPCGX_ABI_VERSION; }
+
+int hpcgx_generate_problem(int nx, int ny, int nz, int nlevels, int flags,
+                           hpcgx_problem **out) {
+  return guard([&] {
+    if (!out) throw Error(HPCGX_EINVAL, "NULL out");
+    *out = nullptr;
+    if (nx < 1 || ny < 1 || nz < 1) throw Error(HPCGX_EINVAL, "grid dimensions must be >= 1");
+    if (nlevels < 1 || nlevels > hpcgx::kMaxLevels) throw Error(HPCGX_EINVAL, "nlevels out of range");
+    const int div = 1 << (nlevels - 1);
+    if (nx % div || ny % div || nz % div)  // GenerateCoarseProblem.cpp:50
+      throw Error(HPCGX_EINVAL, "grid dimensions must be divisible by 2^(nlevels-1)");
+    if ((int64_t)nx * ny * nz >= ((int64_t)1 << 31))
+      throw Error(HPCGX_EINVAL, "rows must be < 2^31 (int32 row ids)");
+    auto h = std::make_unique<hpcgx_problem>();
+    hpcgx::Problem &P = h->P;
+    HPCGX_HIP(hipGetDevice(&P.device));
+    P.nlevels = nlevels;
+    P.generated = true;
+    P.L[0].finest = true;
+    std::vector<NaturalOwner> nat(nlevels);
+    int x = nx, y = ny, z = nz;
+    hipStream_t s = nullptr;
+    for (int l = 0; l < nlevels; ++l) {
+      const int64_t n = (int64_t)x * y * z;
+      const int64_t nc = (l + 1 < nlevels) ? n / 8 : 0;
+      nat[l].alloc(n, nc);
+      nat[l].N.nx = x; nat[l].N.ny = y; nat[l].N.nz = z;
+      hpcgx::generate_natural(nat[l].N, s);
+      // schedules and reordering of this level before generating the next
+      // keep the peak footprint at about two copies of one level
+      hpcgx::finalize_level(P, l, nat[l].N, flags, s);
+      nat[l].drop_matrix();
+      x /= 2; y /= 2; z /= 2;
+    }
+    for (int l = 0; l + 1 < nlevels; ++l) hpcgx::link_levels(P, l, nat[l].N, s);
+    HPCGX_HIP(hipStreamSynchronize(s));
+    alloc_solver_state(P);
+    *out = h.release();
+  });
+}
+
+int hpcgx_optimize_problem(const hpcgx_host_level *levels, int nlevels,
+                           int flags, hpcgx_problem **out) {
+  return guard([&] {
+    if (!out || !levels) throw Error(HPCGX_EINVAL, "NULL argument");
+    *out = nullptr;
+    if (nlevels < 1 || nlevels > hpcgx::kMaxLevels) throw Error(HPCGX_EINVAL, "nlevels out of range");
+    auto h = std::make_unique<hpcgx_problem>();
+    hpcgx::Problem &P = h->P;
+    HPCGX_HIP(hipGetDevice(&P.device));
+    P.nlevels = nlevels;
+    P.L[0].finest = true;
+    std::vector<NaturalOwner> nat(nlevels);
+    hipStream_t s = nullptr;
+    for (int l = 0; l < nlevels; ++l) {
+      const hpcgx_host_level &H = levels[l];
+      const int64_t n = H.nrow;
+      if (n < 1 || n >= ((int64_t)1 << 31)) throw Error(HPCGX_EINVAL, "nrow out of range");
+      if (!H.nonzerosInRow || !H.mtxIndL || !H.matrixValues)
+        throw Error(HPCGX_EINVAL, "level arrays missing");
+      const bool has_next = l + 1 < nlevels;
+      if (has_next && !H.f2cOperator) throw Error(HPCGX_EINVAL, "f2cOperator missing");
+      const int64_t nc = has_next ? levels[l + 1].nrow : 0;
+      // column-major ELL built on the host, slots in the reference's row order
+      std::vector<int32_t> col((size_t)kSlots * n);
+      std::vector<double> val((size_t)kSlots * n);
+      std::vector<uint8_t> nnz(n);
+      for (int64_t i = 0; i < n; ++i) {
+        const int m = H.nonzerosInRow[i];
+        if (m < 1 || m > kSlots) throw Error(HPCGX_EUNSUP, "row with 0 or more than 27 entries");
+        nnz[i] = (uint8_t)m;
+        const int *ci = H.mtxIndL[i];
+        const double *vi = H.matrixValues[i];
+        for (int j = 0; j < kSlots; ++j) {
+          col[(size_t)j * n + i] = j < m ? ci[j] : (int32_t)i;
+          val[(size_t)j * n + i] = j < m ? vi[j] : 0.0;
+        }
+      }
+      nat[l].alloc(n, nc);
+      nat[l].N.nx = H.nx; nat[l].N.ny = H.ny; nat[l].N.nz = H.nz;
+      HPCGX_HIP(hipMemcpy(nat[l].N.col, col.data(), sizeof(int32_t) * col.size(), hipMemcpyHostToDevice));
+      HPCGX_HIP(hipMemcpy(nat[l].N.val, val.data(), sizeof(double) * val.size(), hipMemcpyHostToDevice));
+      HPCGX_HIP(hipMemcpy(nat[l].N.nnz_row, nnz.data(), n, hipMemcpyHostToDevice));
+      if (has_next) {
+        for (int64_t c = 0; c < nc; ++c)
+          if (H.f2cOperator[c] < 0 || H.f2cOperator[c] >= n)
+            throw Error(HPCGX_EINVAL, "f2cOperator entry out of range");
+        HPCGX_HIP(hipMemcpy(nat[l].N.f2c, H.f2cOperator, sizeof(int32_t) * nc, hipMemcpyHostToDevice));
+      }
+    }
+    build_levels(P, nat, flags, s);
+    *out = h.release();
+  });
+}
+
+int hpcgx_problem_destroy(hpcgx_problem *p) {
+  return guard([&] { delete p; });
+}
+
+int hpcgx_problem_nlevels(const hpcgx_problem *p) { return p ? p->P.nlevels : 0; }
+
+int hpcgx_problem_info(const hpcgx_problem *p, int level, hpcgx_level_info *info) {
+  return guard([&] {
+    const hpcgx::Level &L = level_of(p, level);
+    if (!info) throw Error(HPCGX_EINVAL, "NULL info");
+    info->nx = L.nx; info->ny = L.ny; info->nz = L.nz;
+    info->nrow = L.n;
+    info->nnz = L.nnz;
+    info->ncolors = L.ncolors;
+    info->nlevels_sched = L.nsched;
+    info->ncoarse = L.nc;
+  });
+}
+
+double hpcgx_optimize_problem_memory_use(const hpcgx_problem *p) {
+  return p ? p->P.arena.bytes : 0.0;
+}
+
+int hpcgx_problem_vectors(hpcgx_problem *p, double *b, double *x, double *xexact,
+                          void *stream) {
+  return guard([&] {
+    if (!p) throw Error(HPCGX_EINVAL, "NULL problem");
+    if (!p->P.generated && b)
+      throw Error(HPCGX_EINVAL, "b is known only for generated problems");
+    hpcgx::problem_vectors(p->P, b, x, xexact, S(stream));
+  });
+}
+
+int hpcgx_get_schedule(const hpcgx_problem *p, int level, int which, int32_t *out_host) {
+  return guard([&] {
+    const hpcgx::Level &L = level_of(p, level);
+    const int32_t *src = which == 0 ? L.level_of : which == 1 ? L.color_of : nullptr;
+    if (which != 0 && which != 1) throw Error(HPCGX_EINVAL, "which must be 0 or 1");
+    if (!src) throw Error(HPCGX_EINVAL, "schedule ids not kept (HPCGX_KEEP_SCHEDULE)");
+    if (!out_host) throw Error(HPCGX_EINVAL, "NULL output");
+    HPCGX_HIP(hipMemcpy(out_host, src, sizeof(int32_t) * L.n, hipMemcpyDeviceToHost));
+  });
+}
+
+int hpcgx_to_device_order(const hpcgx_problem *p, int level, const double *natural,
+                          double *devorder, void *stream) {
+  return guard([&] {
+    const hpcgx::Level &L = level_of(p, level);
+    hpcgx::launch_gather(L.iperm, natural, devorder, L.n, S(stream));
+  });
+}
+
+int hpcgx_to_natural_order(const hpcgx_problem *p, int level, const double *devorder,
+                           double *natural, void *stream) {
+  return guard([&] {
+    const hpcgx::Level &L = level_of(p, level);
+    hpcgx::launch_gather(L.perm, devorder, natural, L.n, S(stream));
+  });
+}
+
+int hpcgx_replace_diagonal(hpcgx_problem *p, int level, const double *diag, void *stream) {
+  return guard([&] {
+    hpcgx::Level &L = level_of(p, level);
+    hpcgx::launch_replace_diag(L, diag, S(stream));
+  });
+}
+
+int hpcgx_copy_diagonal(const hpcgx_problem *p, int level, double *diag, void *stream) {
+  return guard([&] {
+    const hpcgx::Level &L = level_of(p, level);
+    HPCGX_HIP(hipMemcpyAsync(diag, L.diag, sizeof(double) * L.n,
+                             hipMemcpyDeviceToDevice, S(stream)));
+  });
+}
+
+int hpcgx_spmv(const hpcgx_problem *p, int level, const double *x, double *y, void *stream) {
+  return guard([&] { hpcgx::launch_spmv(level_of(p, level), x, y, S(stream)); });
+}
+
+int hpcgx_symgs(const hpcgx_problem *p, int level, int ordering, const double *r,
+                double *x, void *stream) {
+  return guard([&] {
+    level_of(p, level);
+    check_order(ordering);
+    hpcgx::symgs(const_cast<hpcgx_problem *>(p)->P, level, ordering, r, x, S(stream));
+  });
+}
+
+int hpcgx_mg(hpcgx_problem *p, int level, int ordering, const double *r, double *x,
+             void *stream) {
+  return guard([&] {
+    level_of(p, level);
+    check_order(ordering);
+    hpcgx::mg(p->P, level, ordering, r, x, S(stream));
+  });
+}
+
+int hpcgx_restriction(const hpcgx_problem *p, int level, const double *rf,
+                      const double *Axf, double *rc, void *stream) {
+  return guard([&] {
+    const hpcgx::Level &L = level_of(p, level);
+    if (L.nc == 0) throw Error(HPCGX_EINVAL, "no coarser level");
+    hpcgx::launch_restriction(L, rf, Axf, rc, S(stream));
+  });
+}
+
+int hpcgx_residual_restriction(const hpcgx_problem *p, int level, const double *rf,
+                               const double *xf, double *rc, void *stream) {
+  return guard([&] {
+    const hpcgx::Level &L = level_of(p, level);
+    if (L.nc == 0) throw Error(HPCGX_EINVAL, "no coarser level");
+    hpcgx::launch_residual_restriction(L, rf, xf, rc, S(stream));
+  });
+}
+
+int hpcgx_prolongation(const hpcgx_problem *p, int level, const double *xc, double *xf,
+                       void *stream) {
+  return guard([&] {
+    const hpcgx::Level &L = level_of(p, level);
+    if (L.nc == 0) throw Error(HPCGX_EINVAL, "no coarser level");
+    hpcgx::launch_prolongation(L, xc, xf, S(stream));
+  });
+}
+
+int hpcgx_dot(hpcgx_problem *p, int64_t n, const double *x, const double *y,
+              double *result_dev, void *stream) {
+  return guard([&] {
+    if (!p || !result_dev) throw Error(HPCGX_EINVAL, "NULL argument");
+    if (n < 0) throw Error(HPCGX_EINVAL, "negative length");
+    hpcgx::launch_dot(p->P, n, x, y, result_dev, hpcgx::kFinishStore, S(stream));
+  });
+}
+
+int hpcgx_dot_host(hpcgx_problem *p, int64_t n, const double *x, const double *y,
+                   double *result, void *stream) {
+  return guard([&] {
+    if (!p || !result) throw Error(HPCGX_EINVAL, "NULL argument");
+    if (n < 0) throw Error(HPCGX_EINVAL, "negative length");
+    double *slot = p->P.scalars + hpcgx::kDot;
+    hpcgx::launch_dot(p->P, n, x, y, slot, hpcgx::kFinishStore, S(stream));
+    HPCGX_HIP(hipMemcpyAsync(result, slot, sizeof(double), hipMemcpyDeviceToHost, S(stream)));
+    HPCGX_HIP(hipStreamSynchronize(S(stream)));
+  });
+}
+
+int hpcgx_waxpby(int64_t n, double alpha, const double *x, double beta, const double *y,
+                 double *w, void *stream) {
+  return guard([&] {
+    if (n < 0) throw Error(HPCGX_EINVAL, "negative length");
+    hpcgx::launch_waxpby(n, alpha, x, beta, y, w, S(stream));
+  });
+}
+
+int hpcgx_kernel_timing(hpcgx_problem *p, int enable) {
+  return guard([&] {
+    if (!p) throw Error(HPCGX_EINVAL, "NULL problem");
+    if (enable) p->P.klog.used = 0;
+    p->P.klog.on = enable != 0;
+  });
+}
+
+int hpcgx_kernel_timing_read(hpcgx_problem *p, double *total_ms, int64_t *launches) {
+  return guard([&] {
+    if (!p || !total_ms || !launches) throw Error(HPCGX_EINVAL, "NULL argument");
+    auto &K = p->P.klog;
+    double t = 0.0;
+    if (K.used) HPCGX_HIP(hipEventSynchronize(K.ev[K.used - 1]));
+    for (size_t i = 0; i + 1 < K.used; i += 2) {
+      float ms = 0.f;
+      HPCGX_HIP(hipEventElapsedTime(&ms, K.ev[i], K.ev[i + 1]));
+      t += ms;
+    }
+    *total_ms = t;
+    *launches = (int64_t)(K.used / 2);
+  });
+}
+
+int hpcgx_cg(hpcgx_problem *p, int ordering, const double *b, double *x, int max_iter,
+             double tolerance, int precondition, int flags, int *niters, double *normr,
+             double *normr0, double *times, void *stream) {
+  return guard([&] {
+    if (!p || !b || !x || !niters || !normr || !normr0) throw Error(HPCGX_EINVAL, "NULL argument");
+    check_order(ordering);
+    if (max_iter < 0) throw Error(HPCGX_EINVAL, "negative max_iter");
+    hpcgx::cg(p->P, ordering, b, x, max_iter, tolerance, precondition != 0, flags, niters,
+              normr, normr0, times, S(stream));
+  });
+}
+
+}  // extern "C"

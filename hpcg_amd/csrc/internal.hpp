@@ -1,0 +1,186 @@
+// internal.hpp -- device-side state of libhpcg_amd and the kernel launchers
+// shared between translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "hpcg_amd.h"
+
+namespace hpcgx {
+
+constexpr int kSlots = 27;        // ELL width: the 27-point stencil's maximum
+constexpr int kMaxLevels = 8;
+constexpr int kMaxColors = 32;    // first-fit colours of a <=27-entry row: <=27
+constexpr int kBlock = 256;       // threads per workgroup for streaming kernels
+constexpr int kReduceBlocks = 1024;  // partial sums of a grid reduction
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HPCGX_HIP(call)                                                       \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      throw ::hpcgx::Error(e_ == hipErrorOutOfMemory ? HPCGX_ENOMEM           \
+                                                     : HPCGX_EHIP,            \
+                           std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define HPCGX_LAUNCHED() HPCGX_HIP(hipGetLastError())
+
+#define HPCGX_REQUIRE(cond, msg)                                  \
+  do {                                                            \
+    if (!(cond)) throw ::hpcgx::Error(HPCGX_EINVAL, (msg));       \
+  } while (0)
+
+// Device allocation tracked for OptimizeProblemMemoryUse.
+struct DeviceArena {
+  std::vector<void *> ptrs;
+  double bytes = 0.0;
+  template <class T>
+  T *alloc(size_t count) {
+    void *p = nullptr;
+    if (count == 0) count = 1;
+    HPCGX_HIP(hipMalloc(&p, count * sizeof(T)));
+    ptrs.push_back(p);
+    bytes += double(count) * sizeof(T);
+    return static_cast<T *>(p);
+  }
+  void release(void *p, size_t bytes_of) {
+    for (auto &q : ptrs)
+      if (q == p) {
+        (void)hipFree(q);
+        q = nullptr;
+        bytes -= double(bytes_of);
+        return;
+      }
+  }
+  ~DeviceArena() {
+    for (void *p : ptrs)
+      if (p) (void)hipFree(p);
+  }
+};
+
+// One level of the hierarchy.  Rows live in DEVICE ORDER: sorted by colour,
+// natural order within a colour.  ELL storage is column-major: slot j of
+// device row i is at [j * n + i]; unused slots hold (col = i, val = 0) and
+// sit after the row's stored entries, so a full 27-slot traversal adds exact
+// zeros after the reference's terms.
+struct Level {
+  bool finest = false;  // level 0: its matrix is streamed with non-temporal loads
+  int nx = 0, ny = 0, nz = 0;
+  int64_t n = 0, nnz = 0;
+  int32_t *col = nullptr;   // [kSlots * n] device-order column ids
+  double *val = nullptr;    // [kSlots * n]
+  double *diag = nullptr;   // [n]
+  uint8_t *dslot = nullptr; // [n] slot of the diagonal entry
+  uint8_t *nnz_row = nullptr; // [n] stored entries per row
+  int32_t *perm = nullptr;  // natural row -> device row
+  int32_t *iperm = nullptr; // device row -> natural row
+  // multicolour schedule: colour c owns device rows [color_ptr[c], color_ptr[c+1])
+  int ncolors = 0;
+  std::vector<int64_t> color_ptr;
+  // lexicographic schedule: level s owns level_rows[level_ptr[s] .. level_ptr[s+1])
+  int nsched = 0;
+  std::vector<int64_t> level_ptr;
+  int32_t *level_rows = nullptr;  // device-order row ids, level-major
+  // optional per-row ids (natural order) kept for inspection
+  int32_t *level_of = nullptr, *color_of = nullptr;
+  // transfer to the next level: f2c[coarse device row] = fine device row
+  int64_t nc = 0;
+  int32_t *f2c = nullptr;
+  double *rc = nullptr, *xc = nullptr;  // next level's residual and correction
+};
+
+struct Problem {
+  int device = 0;
+  bool generated = false;  // built by hpcgx_generate_problem
+  int nlevels = 0;
+  Level L[kMaxLevels];
+  DeviceArena arena;
+  // CG work vectors (CGData.hpp:27-32 of the reference)
+  double *r = nullptr, *z = nullptr, *p = nullptr, *Ap = nullptr;
+  // reduction scratch: partial sums, completion ticket, scalars
+  double *partials = nullptr;
+  unsigned *ticket = nullptr;
+  double *scalars = nullptr;  // see CgScalar
+  // event log of the finest level's multicolour Gauss-Seidel launches
+  struct KernelLog {
+    bool on = false;
+    std::vector<hipEvent_t> ev;  // pairs: start, stop
+    size_t used = 0;
+    ~KernelLog() {
+      for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+  } klog;
+};
+
+// Slots of Problem::scalars used by CG.
+enum CgScalar { kRtz = 0, kOldRtz, kPAp, kAlpha, kBeta, kNormr2, kDot, kNumScalars };
+
+// ---- launchers (kernels.hip) ----
+void launch_fill(double *x, int64_t n, double v, hipStream_t s);
+void launch_copy(const double *x, double *y, int64_t n, hipStream_t s);
+void launch_gather(const int32_t *idx, const double *src, double *dst,
+                   int64_t n, hipStream_t s);
+void launch_spmv(const Level &L, const double *x, double *y, hipStream_t s);
+void launch_symgs_rows(const Level &L, int64_t begin, int64_t end,
+                       const double *r, double *x, hipStream_t s);
+void launch_symgs_list(const Level &L, const int32_t *rows, int64_t count,
+                       const double *r, double *x, hipStream_t s);
+void launch_restriction(const Level &L, const double *rf, const double *Axf,
+                        double *rc, hipStream_t s);
+void launch_residual_restriction(const Level &L, const double *rf,
+                                 const double *xf, double *rc, hipStream_t s);
+void launch_prolongation(const Level &L, const double *xc, double *xf,
+                         hipStream_t s);
+void launch_waxpby(int64_t n, double alpha, const double *x, double beta,
+                   const double *y, double *w, hipStream_t s);
+// result = sum x*y into scalars[slot]; op selects the finishing step
+enum DotFinish { kFinishStore = 0, kFinishBeta = 1, kFinishAlpha = 2 };
+void launch_dot(Problem &P, int64_t n, const double *x, const double *y,
+                double *out, int finish, hipStream_t s);
+void launch_spmv_dot(Problem &P, const Level &L, const double *p, double *Ap,
+                     hipStream_t s);
+void launch_update_xr(Problem &P, int64_t n, double *x, double *r,
+                      const double *p, const double *Ap, hipStream_t s);
+void launch_update_p(int64_t n, const double *z, double *p,
+                     const double *scalars, hipStream_t s);
+void launch_residual0(Problem &P, int64_t n, const double *b, const double *Ap,
+                      double *r, hipStream_t s);
+void launch_replace_diag(Level &L, const double *d, hipStream_t s);
+
+// ---- setup (setup.hip) ----
+struct NaturalLevel {  // device buffers of one level in natural row order
+  int nx = 0, ny = 0, nz = 0;
+  int64_t n = 0, nnz = 0;
+  int32_t *col = nullptr;     // [kSlots * n] column-major
+  double *val = nullptr;      // [kSlots * n]
+  uint8_t *nnz_row = nullptr; // [n]
+  int32_t *f2c = nullptr;     // natural f2c to next level
+  int64_t nc = 0;
+};
+void generate_natural(NaturalLevel &N, hipStream_t s);
+void finalize_level(Problem &P, int l, NaturalLevel &N, int flags,
+                    hipStream_t s);
+void link_levels(Problem &P, int l, const NaturalLevel &Nf, hipStream_t s);
+void problem_vectors(const Problem &P, double *b, double *x, double *xexact,
+                     hipStream_t s);
+
+// ---- solver (solver.hip) ----
+void symgs(Problem &P, int l, int ordering, const double *r, double *x,
+           hipStream_t s);
+void mg(Problem &P, int l, int ordering, const double *r, double *x,
+        hipStream_t s);
+void cg(Problem &P, int ordering, const double *b, double *x, int max_iter,
+        double tol, bool precond, int flags, int *niters, double *normr,
+        double *normr0, double *times, hipStream_t s);
+
+}  // namespace hpcgx

@@ -1,0 +1,455 @@
+// kernels.hip -- HIP kernels of the FP64 PCG path for gfx950.
+//
+// Every row kernel keeps the reference's per-row arithmetic: terms summed in
+// slot order j = 0..26 with separate multiply and add (the library is built
+// with -ffp-contract=off) and a correctly rounded division, so SpMV, the
+// Gauss-Seidel sweeps, restriction and WAXPBY reproduce the reference bit for
+// bit.  Dot products use a fixed-shape two-level tree, so they are
+// bit-identical from call to call (TestNorms variance 0) and differ from the
+// reference's sequential sum only by reassociation.
+#include "internal.hpp"
+
+namespace hpcgx {
+
+namespace {
+
+inline unsigned grid_for(int64_t n) {
+  return (unsigned)((n + kBlock - 1) / kBlock);
+}
+
+__device__ inline int64_t gtid() {
+  return (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+}
+
+// ---------------------------------------------------------------------------
+// BLAS-1 and data movement
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_fill(double *x, int64_t n, double v) {
+  const int64_t i = gtid();
+  if (i < n) x[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy(const double *__restrict__ x,
+                                                  double *__restrict__ y,
+                                                  int64_t n) {
+  const int64_t i = gtid();
+  if (i < n) y[i] = x[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather(
+    const int32_t *__restrict__ idx, const double *__restrict__ src,
+    double *__restrict__ dst, int64_t n) {
+  const int64_t i = gtid();
+  if (i < n) dst[i] = src[idx[i]];
+}
+
+// ComputeWAXPBY_ref.cpp:52-67: the three forms round differently, so the
+// form is a template parameter chosen on the host exactly as the reference
+// chooses it.  w may alias x or y (each element is read before it is written
+// by the same thread).
+template <int FORM>
+__global__ __launch_bounds__(kBlock) void k_waxpby(int64_t n, double alpha,
+                                                    const double *x,
+                                                    double beta,
+                                                    const double *y,
+                                                    double *w) {
+  const int64_t i = gtid();
+  if (i >= n) return;
+  const double xi = x[i], yi = y[i];
+  if (FORM == 0)
+    w[i] = xi + beta * yi;
+  else if (FORM == 1)
+    w[i] = alpha * xi + yi;
+  else
+    w[i] = alpha * xi + beta * yi;
+}
+
+// ---------------------------------------------------------------------------
+// Sparse row kernels on the column-major 27-slot ELL
+// ---------------------------------------------------------------------------
+// Matrix loads.  STREAM = the level's matrix is far larger than the 256 MB
+// last-level cache, so it is read with non-temporal loads that do not evict
+// the x vector; coarse levels read it through the caches, where it can stay
+// resident between the two sweeps of a smoother step.
+template <bool STREAM, class T>
+__device__ inline T mload(const T *p) {
+  if (STREAM) return __builtin_nontemporal_load(p);
+  return *p;
+}
+
+// y_i = sum_j a_ij x_col(i,j)  (ComputeSPMV_ref.cpp:62-69)
+template <bool STREAM>
+__device__ inline double row_dot(const int32_t *__restrict__ col,
+                                 const double *__restrict__ val,
+                                 const double *__restrict__ x, int64_t n,
+                                 int64_t i) {
+  double sum = 0.0;
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int64_t k = (int64_t)j * n + i;
+    sum += mload<STREAM>(val + k) * x[mload<STREAM>(col + k)];
+  }
+  return sum;
+}
+
+template <bool STREAM>
+__global__ __launch_bounds__(kBlock) void k_spmv(int64_t n,
+                                                  const int32_t *__restrict__ col,
+                                                  const double *__restrict__ val,
+                                                  const double *__restrict__ x,
+                                                  double *__restrict__ y) {
+  const int64_t i = gtid();
+  if (i < n) y[i] = row_dot<STREAM>(col, val, x, n, i);
+}
+
+// One Gauss-Seidel row update in the reference's form
+// (ComputeSYMGS_ref.cpp:72-80): subtract every stored term including the
+// diagonal one, add the diagonal term back, divide by the diagonal.
+template <bool STREAM>
+__device__ inline void gs_row(int64_t i, int64_t n,
+                              const int32_t *__restrict__ col,
+                              const double *__restrict__ val,
+                              const double *__restrict__ diag,
+                              const double *__restrict__ r, double *x) {
+  double sum = r[i];
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int64_t k = (int64_t)j * n + i;
+    sum -= mload<STREAM>(val + k) * x[mload<STREAM>(col + k)];
+  }
+  const double d = diag[i];
+  sum += x[i] * d;
+  x[i] = sum / d;
+}
+
+// All rows of one colour (a contiguous device-order range) at once: rows of
+// one colour share no matrix entry, so their updates are independent.
+template <bool STREAM>
+__global__ __launch_bounds__(kBlock) void k_gs_range(
+    int64_t begin, int64_t end, int64_t n, const int32_t *__restrict__ col,
+    const double *__restrict__ val, const double *__restrict__ diag,
+    const double *__restrict__ r, double *x) {
+  const int64_t i = begin + gtid();
+  if (i < end) gs_row<STREAM>(i, n, col, val, diag, r, x);
+}
+
+// All rows of one level of the lexicographic schedule (a list of rows).
+__global__ __launch_bounds__(kBlock) void k_gs_list(
+    const int32_t *__restrict__ rows, int64_t count, int64_t n,
+    const int32_t *__restrict__ col, const double *__restrict__ val,
+    const double *__restrict__ diag, const double *__restrict__ r, double *x) {
+  const int64_t k = gtid();
+  if (k < count) gs_row<false>(rows[k], n, col, val, diag, r, x);
+}
+
+// ComputeRestriction_ref.cpp:51
+__global__ __launch_bounds__(kBlock) void k_restrict(
+    int64_t nc, const int32_t *__restrict__ f2c, const double *__restrict__ rf,
+    const double *__restrict__ Axf, double *__restrict__ rc) {
+  const int64_t c = gtid();
+  if (c < nc) {
+    const int32_t f = f2c[c];
+    rc[c] = rf[f] - Axf[f];
+  }
+}
+
+// ComputeMG_ref.cpp:49-51 fused: the fine-level product only at injected rows.
+template <bool STREAM>
+__global__ __launch_bounds__(kBlock) void k_resid_restrict(
+    int64_t nc, int64_t n, const int32_t *__restrict__ f2c,
+    const int32_t *__restrict__ col, const double *__restrict__ val,
+    const double *__restrict__ rf, const double *__restrict__ xf,
+    double *__restrict__ rc) {
+  const int64_t c = gtid();
+  if (c < nc) {
+    const int64_t f = f2c[c];
+    rc[c] = rf[f] - row_dot<STREAM>(col, val, xf, n, f);
+  }
+}
+
+// ComputeProlongation_ref.cpp:49 (f2c is injective: no write conflicts)
+__global__ __launch_bounds__(kBlock) void k_prolong(
+    int64_t nc, const int32_t *__restrict__ f2c, const double *__restrict__ xc,
+    double *__restrict__ xf) {
+  const int64_t c = gtid();
+  if (c < nc) xf[f2c[c]] += xc[c];
+}
+
+__global__ __launch_bounds__(kBlock) void k_replace_diag(
+    int64_t n, const uint8_t *__restrict__ dslot, const double *__restrict__ d,
+    double *__restrict__ diag, double *__restrict__ val) {
+  const int64_t i = gtid();
+  if (i < n) {
+    const double v = d[i];
+    diag[i] = v;
+    val[(int64_t)dslot[i] * n + i] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic grid reductions
+// ---------------------------------------------------------------------------
+// Sum over the workgroup in a fixed tree: 64-lane shuffles, then the four
+// wavefront sums in order.  Result valid in thread 0.
+__device__ inline double block_sum(double v) {
+  __shared__ double wsum[kBlock / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) wsum[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) t += wsum[w];
+  }
+  __syncthreads();
+  return t;
+}
+
+// Each workgroup stores its partial; the last one to finish (ticket) sums the
+// partials in index order and applies the finishing step.  The ticket is
+// reset by that workgroup, so the next launch starts from zero.
+__device__ inline void grid_finish(double s, double *partials, unsigned *ticket,
+                                   double *scalars, double *out, int finish) {
+  __shared__ int is_last;
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = s;
+    __threadfence();
+    const unsigned t = atomicAdd(ticket, 1u);
+    is_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  double v = 0.0;
+  for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x)
+    v += __hip_atomic_load(&partials[b], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+  v = block_sum(v);
+  if (threadIdx.x == 0) {
+    *ticket = 0u;
+    if (finish == kFinishStore) {
+      *out = v;
+    } else if (finish == kFinishBeta) {  // CG.cpp:112-114
+      const double old = scalars[kRtz];
+      scalars[kOldRtz] = old;
+      scalars[kRtz] = v;
+      scalars[kBeta] = v / old;
+    } else {  // kFinishAlpha, CG.cpp:119-120
+      scalars[kPAp] = v;
+      scalars[kAlpha] = scalars[kRtz] / v;
+    }
+  }
+}
+
+inline unsigned reduce_grid(int64_t n) {
+  const int64_t g = (n + kBlock - 1) / kBlock;
+  return (unsigned)(g < kReduceBlocks ? (g < 1 ? 1 : g) : kReduceBlocks);
+}
+
+__global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const double *x,
+                                                 const double *y,
+                                                 double *partials,
+                                                 unsigned *ticket,
+                                                 double *scalars, double *out,
+                                                 int finish) {
+  double s = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = gtid(); i < n; i += stride) s += x[i] * y[i];
+  s = block_sum(s);
+  grid_finish(s, partials, ticket, scalars, out, finish);
+}
+
+// Ap = A p and p.Ap (CG.cpp:118-120) in one pass: each workgroup owns a
+// contiguous block of rows so the x-gathers stay local.
+__global__ __launch_bounds__(kBlock) void k_spmv_dot(
+    int64_t n, const int32_t *__restrict__ col, const double *__restrict__ val,
+    const double *__restrict__ p, double *__restrict__ Ap, double *partials,
+    unsigned *ticket, double *scalars, int64_t rows_per_block) {
+  const int64_t lo = (int64_t)blockIdx.x * rows_per_block;
+  int64_t hi = lo + rows_per_block;
+  if (hi > n) hi = n;
+  double s = 0.0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    const double y = row_dot<true>(col, val, p, n, i);
+    Ap[i] = y;
+    s += p[i] * y;
+  }
+  s = block_sum(s);
+  grid_finish(s, partials, ticket, scalars, nullptr, kFinishAlpha);
+}
+
+// x += alpha p; r -= alpha Ap; r.r (CG.cpp:121-124).
+__global__ __launch_bounds__(kBlock) void k_update_xr(
+    int64_t n, double *__restrict__ x, double *__restrict__ r,
+    const double *__restrict__ p, const double *__restrict__ Ap,
+    double *partials, unsigned *ticket, double *scalars) {
+  const double alpha = scalars[kAlpha];
+  const double malpha = -alpha;
+  double s = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = gtid(); i < n; i += stride) {
+    x[i] = x[i] + alpha * p[i];
+    const double ri = r[i] + malpha * Ap[i];
+    r[i] = ri;
+    s += ri * ri;
+  }
+  s = block_sum(s);
+  grid_finish(s, partials, ticket, scalars, &scalars[kNormr2], kFinishStore);
+}
+
+// r = b - Ap and r.r (CG.cpp:88-89)
+__global__ __launch_bounds__(kBlock) void k_residual0(
+    int64_t n, const double *__restrict__ b, const double *__restrict__ Ap,
+    double *__restrict__ r, double *partials, unsigned *ticket,
+    double *scalars) {
+  double s = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = gtid(); i < n; i += stride) {
+    const double ri = b[i] + -1.0 * Ap[i];
+    r[i] = ri;
+    s += ri * ri;
+  }
+  s = block_sum(s);
+  grid_finish(s, partials, ticket, scalars, &scalars[kNormr2], kFinishStore);
+}
+
+// p = z + beta p (CG.cpp:115)
+__global__ __launch_bounds__(kBlock) void k_update_p(
+    int64_t n, const double *__restrict__ z, double *__restrict__ p,
+    const double *__restrict__ scalars) {
+  const int64_t i = gtid();
+  if (i < n) p[i] = z[i] + scalars[kBeta] * p[i];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+void launch_fill(double *x, int64_t n, double v, hipStream_t s) {
+  if (n <= 0) return;
+  k_fill<<<grid_for(n), kBlock, 0, s>>>(x, n, v);
+  HPCGX_LAUNCHED();
+}
+
+void launch_copy(const double *x, double *y, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  k_copy<<<grid_for(n), kBlock, 0, s>>>(x, y, n);
+  HPCGX_LAUNCHED();
+}
+
+void launch_gather(const int32_t *idx, const double *src, double *dst,
+                   int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  k_gather<<<grid_for(n), kBlock, 0, s>>>(idx, src, dst, n);
+  HPCGX_LAUNCHED();
+}
+
+void launch_spmv(const Level &L, const double *x, double *y, hipStream_t s) {
+  if (L.finest)
+    k_spmv<true><<<grid_for(L.n), kBlock, 0, s>>>(L.n, L.col, L.val, x, y);
+  else
+    k_spmv<false><<<grid_for(L.n), kBlock, 0, s>>>(L.n, L.col, L.val, x, y);
+  HPCGX_LAUNCHED();
+}
+
+void launch_symgs_rows(const Level &L, int64_t begin, int64_t end,
+                       const double *r, double *x, hipStream_t s) {
+  if (end <= begin) return;
+  if (L.finest)
+    k_gs_range<true><<<grid_for(end - begin), kBlock, 0, s>>>(
+        begin, end, L.n, L.col, L.val, L.diag, r, x);
+  else
+    k_gs_range<false><<<grid_for(end - begin), kBlock, 0, s>>>(
+        begin, end, L.n, L.col, L.val, L.diag, r, x);
+  HPCGX_LAUNCHED();
+}
+
+void launch_symgs_list(const Level &L, const int32_t *rows, int64_t count,
+                       const double *r, double *x, hipStream_t s) {
+  if (count <= 0) return;
+  k_gs_list<<<grid_for(count), kBlock, 0, s>>>(rows, count, L.n, L.col, L.val,
+                                               L.diag, r, x);
+  HPCGX_LAUNCHED();
+}
+
+void launch_restriction(const Level &L, const double *rf, const double *Axf,
+                        double *rc, hipStream_t s) {
+  k_restrict<<<grid_for(L.nc), kBlock, 0, s>>>(L.nc, L.f2c, rf, Axf, rc);
+  HPCGX_LAUNCHED();
+}
+
+void launch_residual_restriction(const Level &L, const double *rf,
+                                 const double *xf, double *rc, hipStream_t s) {
+  if (L.finest)
+    k_resid_restrict<true><<<grid_for(L.nc), kBlock, 0, s>>>(
+        L.nc, L.n, L.f2c, L.col, L.val, rf, xf, rc);
+  else
+    k_resid_restrict<false><<<grid_for(L.nc), kBlock, 0, s>>>(
+        L.nc, L.n, L.f2c, L.col, L.val, rf, xf, rc);
+  HPCGX_LAUNCHED();
+}
+
+void launch_prolongation(const Level &L, const double *xc, double *xf,
+                         hipStream_t s) {
+  k_prolong<<<grid_for(L.nc), kBlock, 0, s>>>(L.nc, L.f2c, xc, xf);
+  HPCGX_LAUNCHED();
+}
+
+void launch_waxpby(int64_t n, double alpha, const double *x, double beta,
+                   const double *y, double *w, hipStream_t s) {
+  if (n <= 0) return;
+  if (alpha == 1.0)
+    k_waxpby<0><<<grid_for(n), kBlock, 0, s>>>(n, alpha, x, beta, y, w);
+  else if (beta == 1.0)
+    k_waxpby<1><<<grid_for(n), kBlock, 0, s>>>(n, alpha, x, beta, y, w);
+  else
+    k_waxpby<2><<<grid_for(n), kBlock, 0, s>>>(n, alpha, x, beta, y, w);
+  HPCGX_LAUNCHED();
+}
+
+void launch_dot(Problem &P, int64_t n, const double *x, const double *y,
+                double *out, int finish, hipStream_t s) {
+  k_dot<<<reduce_grid(n), kBlock, 0, s>>>(n, x, y, P.partials, P.ticket,
+                                          P.scalars, out, finish);
+  HPCGX_LAUNCHED();
+}
+
+void launch_spmv_dot(Problem &P, const Level &L, const double *p, double *Ap,
+                     hipStream_t s) {
+  const unsigned g = reduce_grid(L.n);
+  const int64_t rpb = (L.n + g - 1) / g;
+  k_spmv_dot<<<g, kBlock, 0, s>>>(L.n, L.col, L.val, p, Ap, P.partials,
+                                  P.ticket, P.scalars, rpb);
+  HPCGX_LAUNCHED();
+}
+
+void launch_update_xr(Problem &P, int64_t n, double *x, double *r,
+                      const double *p, const double *Ap, hipStream_t s) {
+  k_update_xr<<<reduce_grid(n), kBlock, 0, s>>>(n, x, r, p, Ap, P.partials,
+                                                P.ticket, P.scalars);
+  HPCGX_LAUNCHED();
+}
+
+void launch_update_p(int64_t n, const double *z, double *p,
+                     const double *scalars, hipStream_t s) {
+  k_update_p<<<grid_for(n), kBlock, 0, s>>>(n, z, p, scalars);
+  HPCGX_LAUNCHED();
+}
+
+void launch_residual0(Problem &P, int64_t n, const double *b, const double *Ap,
+                      double *r, hipStream_t s) {
+  k_residual0<<<reduce_grid(n), kBlock, 0, s>>>(n, b, Ap, r, P.partials,
+                                                P.ticket, P.scalars);
+  HPCGX_LAUNCHED();
+}
+
+void launch_replace_diag(Level &L, const double *d, hipStream_t s) {
+  k_replace_diag<<<grid_for(L.n), kBlock, 0, s>>>(L.n, L.dslot, d, L.diag,
+                                                  L.val);
+  HPCGX_LAUNCHED();
+}
+
+}  // namespace hpcgx

@@ -1,0 +1,467 @@
+// setup.hip -- device-side problem setup: generation, schedules, reordering.
+//
+// The schedule builder is generic (it reads only the sparsity structure):
+//  * the lexicographic level schedule is the level-set partition of the
+//    dependency DAG "row i needs every column j < i of row i", found with a
+//    level-synchronous Kahn traversal: a row becomes ready when its last
+//    lower neighbour has been placed, and its level is one more than that
+//    neighbour's;
+//  * the colour of a row is the smallest colour not used by its lower
+//    neighbours, assigned the moment the row becomes ready.  That is exactly
+//    the greedy first-fit colouring of the reference's OptimizeProblem.cpp:
+//    50-79 (which visits rows in natural order and looks only at columns
+//    j < i), computed in parallel because a row's colour depends only on
+//    rows of lower levels.
+// Rows are then renumbered colour by colour (natural order inside a colour)
+// and the 27-slot ELL is rebuilt in that order.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <numeric>
+
+#include "internal.hpp"
+
+namespace hpcgx {
+namespace {
+
+constexpr int kStepBlocks = 1024;      // fixed grid of the Kahn steps
+constexpr int kSyncEvery = 128;        // Kahn steps between host checks
+constexpr int64_t kMaxSched = 1 << 22; // bound on schedule levels
+
+inline unsigned grid_for(int64_t n) {
+  return (unsigned)((n + kBlock - 1) / kBlock);
+}
+__device__ inline int64_t gtid() {
+  return (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+}
+
+struct TempBuf {  // a temporary device buffer freed on scope exit
+  void *p = nullptr;
+  explicit TempBuf(size_t bytes) { HPCGX_HIP(hipMalloc(&p, bytes ? bytes : 1)); }
+  ~TempBuf() { (void)hipFree(p); }
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// GenerateProblem_ref.cpp:131-183 (single rank): natural order, neighbours
+// in sz, sy, sx order, 26 on the diagonal and -1 elsewhere.
+__global__ __launch_bounds__(kBlock) void k_generate(int nx, int ny, int nz,
+                                                      int64_t n, int32_t *col,
+                                                      double *val,
+                                                      uint8_t *nnz_row) {
+  const int64_t i = gtid();
+  if (i >= n) return;
+  const int ix = (int)(i % nx);
+  const int64_t t = i / nx;
+  const int iy = (int)(t % ny);
+  const int iz = (int)(t / ny);
+  const int64_t nxy = (int64_t)nx * ny;
+  int cnt = 0;
+  for (int sz = -1; sz <= 1; ++sz) {
+    if (iz + sz < 0 || iz + sz >= nz) continue;
+    for (int sy = -1; sy <= 1; ++sy) {
+      if (iy + sy < 0 || iy + sy >= ny) continue;
+      for (int sx = -1; sx <= 1; ++sx) {
+        if (ix + sx < 0 || ix + sx >= nx) continue;
+        const int64_t c = i + sz * nxy + (int64_t)sy * nx + sx;
+        col[(int64_t)cnt * n + i] = (int32_t)c;
+        val[(int64_t)cnt * n + i] = (c == i) ? 26.0 : -1.0;
+        ++cnt;
+      }
+    }
+  }
+  for (int j = cnt; j < kSlots; ++j) {
+    col[(int64_t)j * n + i] = (int32_t)i;
+    val[(int64_t)j * n + i] = 0.0;
+  }
+  nnz_row[i] = (uint8_t)cnt;
+}
+
+// GenerateCoarseProblem.cpp:71-82
+__global__ __launch_bounds__(kBlock) void k_f2c(int nxf, int nyf, int64_t nc,
+                                                 int nxc, int nyc,
+                                                 int32_t *f2c) {
+  const int64_t c = gtid();
+  if (c >= nc) return;
+  const int ixc = (int)(c % nxc);
+  const int64_t t = c / nxc;
+  const int iyc = (int)(t % nyc);
+  const int izc = (int)(t / nyc);
+  f2c[c] = (int32_t)((int64_t)(2 * izc) * nxf * nyf + (int64_t)(2 * iyc) * nxf +
+                     2 * ixc);
+}
+
+__global__ __launch_bounds__(kBlock) void k_count_nnz(int64_t n,
+                                                       const uint8_t *nnz_row,
+                                                       unsigned long long *out) {
+  const int64_t i = gtid();
+  unsigned long long v = (i < n) ? nnz_row[i] : 0ull;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, v);
+}
+
+// Structural checks: columns in range, a diagonal entry, symmetric pattern
+// (the Kahn traversal walks each row's upper entries as the rows that
+// depend on it).  bad: 1 = column out of range, 2 = no diagonal,
+// 4 = pattern not symmetric.
+__global__ __launch_bounds__(kBlock) void k_check(int64_t n, const int32_t *col,
+                                                   const uint8_t *nnz_row,
+                                                   int32_t *lower_count,
+                                                   int *bad) {
+  const int64_t i = gtid();
+  if (i >= n) return;
+  const int m = nnz_row[i];
+  int lower = 0, has_diag = 0, flags = 0;
+  for (int j = 0; j < m; ++j) {
+    const int64_t c = col[(int64_t)j * n + i];
+    if (c < 0 || c >= n) { flags |= 1; continue; }
+    if (c == i) { has_diag = 1; continue; }
+    if (c < i) ++lower;
+    const int mc = nnz_row[c];
+    int found = 0;
+    for (int k = 0; k < mc; ++k) found |= (col[(int64_t)k * n + c] == i);
+    if (!found) flags |= 4;
+  }
+  if (!has_diag) flags |= 2;
+  lower_count[i] = lower;
+  if (flags) atomicOr(bad, flags);
+}
+
+// Colour of row c: smallest colour absent from its lower neighbours.
+__device__ inline int32_t first_fit(int64_t c, int64_t n, const int32_t *col,
+                                    const uint8_t *nnz_row,
+                                    const int32_t *color) {
+  unsigned mask = 0u;
+  const int m = nnz_row[c];
+  for (int j = 0; j < m; ++j) {
+    const int64_t d = col[(int64_t)j * n + c];
+    if (d < c) mask |= 1u << color[d];
+  }
+  return (int32_t)(__ffs(~mask) - 1);
+}
+
+__global__ __launch_bounds__(kBlock) void k_kahn_init(
+    int64_t n, const int32_t *remaining, int32_t *level, int32_t *color,
+    int32_t *order, unsigned long long *tail) {
+  const int64_t i = gtid();
+  if (i < n && remaining[i] == 0) {
+    level[i] = 0;
+    color[i] = 0;
+    order[atomicAdd(tail, 1ull)] = (int32_t)i;
+  }
+}
+
+// Step t: rows of level t are order[lptr[t] .. lptr[t+1]).  Each one releases
+// its upper neighbours; a neighbour whose last lower neighbour this was gets
+// level t+1 and its first-fit colour and is appended to `order`.
+__global__ __launch_bounds__(kBlock) void k_kahn_step(
+    int t, const int64_t *lptr, int64_t n, const int32_t *col,
+    const uint8_t *nnz_row, int32_t *remaining, int32_t *level, int32_t *color,
+    int32_t *order, unsigned long long *tail) {
+  const int64_t begin = lptr[t], end = lptr[t + 1];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = begin + gtid(); k < end; k += stride) {
+    const int64_t u = order[k];
+    const int m = nnz_row[u];
+    for (int j = 0; j < m; ++j) {
+      const int64_t c = col[(int64_t)j * n + u];
+      if (c <= u) continue;
+      if (atomicSub(&remaining[c], 1) == 1) {
+        level[c] = t + 1;
+        color[c] = first_fit(c, n, col, nnz_row, color);
+        order[atomicAdd(tail, 1ull)] = (int32_t)c;
+      }
+    }
+  }
+}
+
+__global__ void k_kahn_advance(int t, int64_t *lptr,
+                               const unsigned long long *tail) {
+  lptr[t + 2] = (int64_t)*tail;
+}
+
+__global__ __launch_bounds__(kBlock) void k_iota(int64_t n, int32_t *a) {
+  const int64_t i = gtid();
+  if (i < n) a[i] = (int32_t)i;
+}
+
+__global__ __launch_bounds__(kBlock) void k_histogram(int64_t n,
+                                                       const int32_t *key,
+                                                       unsigned *hist) {
+  const int64_t i = gtid();
+  if (i < n) atomicAdd(&hist[key[i]], 1u);
+}
+
+__global__ __launch_bounds__(kBlock) void k_invert(int64_t n,
+                                                    const int32_t *iperm,
+                                                    int32_t *perm) {
+  const int64_t k = gtid();
+  if (k < n) perm[iperm[k]] = (int32_t)k;
+}
+
+__global__ __launch_bounds__(kBlock) void k_map(int64_t n, const int32_t *map,
+                                                 const int32_t *in,
+                                                 int32_t *out) {
+  const int64_t k = gtid();
+  if (k < n) out[k] = map[in[k]];
+}
+
+// Rebuild the ELL in device order: device row k is natural row iperm[k];
+// columns are renumbered through perm; unused slots become (k, 0.0).
+__global__ __launch_bounds__(kBlock) void k_permute_ell(
+    int64_t n, const int32_t *__restrict__ iperm,
+    const int32_t *__restrict__ perm, const int32_t *__restrict__ col_nat,
+    const double *__restrict__ val_nat, const uint8_t *__restrict__ nnz_nat,
+    int32_t *__restrict__ col, double *__restrict__ val,
+    double *__restrict__ diag, uint8_t *__restrict__ dslot,
+    uint8_t *__restrict__ nnz_row) {
+  const int64_t k = gtid();
+  if (k >= n) return;
+  const int64_t i = iperm[k];
+  const int m = nnz_nat[i];
+  int ds = -1;
+  for (int j = 0; j < kSlots; ++j) {
+    const int64_t src = (int64_t)j * n + i, dst = (int64_t)j * n + k;
+    if (j < m) {
+      const int32_t c = col_nat[src];
+      if (c == i && ds < 0) ds = j;
+      col[dst] = perm[c];
+      val[dst] = val_nat[src];
+    } else {
+      col[dst] = (int32_t)k;
+      val[dst] = 0.0;
+    }
+  }
+  dslot[k] = (uint8_t)ds;
+  diag[k] = val_nat[(int64_t)ds * n + i];
+  nnz_row[k] = (uint8_t)m;
+}
+
+// f2c in device order: coarse device row kc -> fine device row.
+__global__ __launch_bounds__(kBlock) void k_link(int64_t nc,
+                                                  const int32_t *iperm_c,
+                                                  const int32_t *f2c_nat,
+                                                  const int32_t *perm_f,
+                                                  int32_t *f2c) {
+  const int64_t kc = gtid();
+  if (kc < nc) f2c[kc] = perm_f[f2c_nat[iperm_c[kc]]];
+}
+
+__global__ __launch_bounds__(kBlock) void k_vectors(int64_t n,
+                                                     const uint8_t *nnz_row,
+                                                     double *b, double *x,
+                                                     double *xexact) {
+  const int64_t k = gtid();
+  if (k >= n) return;
+  if (b) b[k] = 26.0 - ((double)(nnz_row[k] - 1));  // GenerateProblem_ref.cpp:178
+  if (x) x[k] = 0.0;
+  if (xexact) xexact[k] = 1.0;
+}
+
+// Stable sort of row ids by a small integer key: returns rows ordered by
+// (key, row).
+void sort_rows_by_key(int64_t n, const int32_t *key, int end_bit,
+                      int32_t *rows_out, hipStream_t s) {
+  HPCGX_REQUIRE(n < (int64_t)1 << 31, "row count exceeds 2^31");
+  TempBuf ids(sizeof(int32_t) * n), keys_out(sizeof(int32_t) * n);
+  k_iota<<<grid_for(n), kBlock, 0, s>>>(n, ids.as<int32_t>());
+  HPCGX_LAUNCHED();
+  // keys are non-negative: sort them as unsigned
+  const uint32_t *ukey = reinterpret_cast<const uint32_t *>(key);
+  size_t temp_bytes = 0;
+  HPCGX_HIP(hipcub::DeviceRadixSort::SortPairs(
+      nullptr, temp_bytes, ukey, keys_out.as<uint32_t>(), ids.as<int32_t>(),
+      rows_out, (int)n, 0, end_bit, s));
+  TempBuf temp(temp_bytes);
+  HPCGX_HIP(hipcub::DeviceRadixSort::SortPairs(
+      temp.p, temp_bytes, ukey, keys_out.as<uint32_t>(), ids.as<int32_t>(),
+      rows_out, (int)n, 0, end_bit, s));
+  HPCGX_HIP(hipStreamSynchronize(s));
+}
+
+int bits_for(int64_t v) {
+  int b = 1;
+  while (((int64_t)1 << b) <= v) ++b;
+  return b;
+}
+
+std::vector<int64_t> histogram_ptr(int64_t n, const int32_t *key, int nbins,
+                                   hipStream_t s) {
+  TempBuf hist(sizeof(unsigned) * nbins);
+  HPCGX_HIP(hipMemsetAsync(hist.p, 0, sizeof(unsigned) * nbins, s));
+  k_histogram<<<grid_for(n), kBlock, 0, s>>>(n, key, hist.as<unsigned>());
+  HPCGX_LAUNCHED();
+  std::vector<unsigned> h(nbins);
+  HPCGX_HIP(hipMemcpyAsync(h.data(), hist.p, sizeof(unsigned) * nbins,
+                           hipMemcpyDeviceToHost, s));
+  HPCGX_HIP(hipStreamSynchronize(s));
+  std::vector<int64_t> ptr(nbins + 1, 0);
+  for (int b = 0; b < nbins; ++b) ptr[b + 1] = ptr[b] + h[b];
+  return ptr;
+}
+
+}  // namespace
+
+void generate_natural(NaturalLevel &N, hipStream_t s) {
+  const int64_t n = N.n;
+  k_generate<<<grid_for(n), kBlock, 0, s>>>(N.nx, N.ny, N.nz, n, N.col, N.val,
+                                            N.nnz_row);
+  HPCGX_LAUNCHED();
+  if (N.f2c) {
+    k_f2c<<<grid_for(N.nc), kBlock, 0, s>>>(N.nx, N.ny, N.nc, N.nx / 2,
+                                            N.ny / 2, N.f2c);
+    HPCGX_LAUNCHED();
+  }
+}
+
+void finalize_level(Problem &P, int l, NaturalLevel &N, int flags,
+                    hipStream_t s) {
+  Level &L = P.L[l];
+  const int64_t n = N.n;
+  HPCGX_REQUIRE(n > 0 && n < ((int64_t)1 << 31), "rows per level must be in [1, 2^31)");
+  L.nx = N.nx; L.ny = N.ny; L.nz = N.nz;
+  L.n = n;
+
+  // nnz
+  {
+    TempBuf cnt(sizeof(unsigned long long));
+    HPCGX_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), s));
+    k_count_nnz<<<grid_for(n), kBlock, 0, s>>>(n, N.nnz_row, cnt.as<unsigned long long>());
+    HPCGX_LAUNCHED();
+    unsigned long long v = 0;
+    HPCGX_HIP(hipMemcpyAsync(&v, cnt.p, sizeof v, hipMemcpyDeviceToHost, s));
+    HPCGX_HIP(hipStreamSynchronize(s));
+    L.nnz = N.nnz = (int64_t)v;
+  }
+
+  // structure checks + lower-neighbour counts
+  TempBuf remaining(sizeof(int32_t) * n), level(sizeof(int32_t) * n),
+      color(sizeof(int32_t) * n), order(sizeof(int32_t) * n);
+  {
+    TempBuf bad(sizeof(int));
+    HPCGX_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+    k_check<<<grid_for(n), kBlock, 0, s>>>(n, N.col, N.nnz_row,
+                                           remaining.as<int32_t>(), bad.as<int>());
+    HPCGX_LAUNCHED();
+    int b = 0;
+    HPCGX_HIP(hipMemcpyAsync(&b, bad.p, sizeof b, hipMemcpyDeviceToHost, s));
+    HPCGX_HIP(hipStreamSynchronize(s));
+    if (b & 1) throw Error(HPCGX_EINVAL, "column index out of range");
+    if (b & 2) throw Error(HPCGX_EUNSUP, "a row has no diagonal entry");
+    if (b & 4) throw Error(HPCGX_EUNSUP, "sparsity pattern is not symmetric");
+  }
+
+  // Kahn traversal: levels and first-fit colours
+  int nsched = 0;
+  {
+    const int64_t cap = std::min<int64_t>(n, kMaxSched) + 2;
+    TempBuf lptr(sizeof(int64_t) * cap), tail(sizeof(unsigned long long));
+    HPCGX_HIP(hipMemsetAsync(lptr.p, 0, sizeof(int64_t) * cap, s));
+    HPCGX_HIP(hipMemsetAsync(tail.p, 0, sizeof(unsigned long long), s));
+    k_kahn_init<<<grid_for(n), kBlock, 0, s>>>(
+        n, remaining.as<int32_t>(), level.as<int32_t>(), color.as<int32_t>(),
+        order.as<int32_t>(), tail.as<unsigned long long>());
+    HPCGX_LAUNCHED();
+    // lptr[1] = rows of level 0
+    k_kahn_advance<<<1, 1, 0, s>>>(-1, lptr.as<int64_t>(),
+                                   tail.as<unsigned long long>());
+    HPCGX_LAUNCHED();
+    int t = 0;
+    for (;;) {
+      const int stop = (int)std::min<int64_t>(t + kSyncEvery, cap - 2);
+      for (; t < stop; ++t) {
+        k_kahn_step<<<kStepBlocks, kBlock, 0, s>>>(
+            t, lptr.as<int64_t>(), n, N.col, N.nnz_row, remaining.as<int32_t>(),
+            level.as<int32_t>(), color.as<int32_t>(), order.as<int32_t>(),
+            tail.as<unsigned long long>());
+        HPCGX_LAUNCHED();
+        k_kahn_advance<<<1, 1, 0, s>>>(t, lptr.as<int64_t>(),
+                                       tail.as<unsigned long long>());
+        HPCGX_LAUNCHED();
+      }
+      // after step t-1, lptr[t+1] holds the end of level t
+      int64_t last[2] = {0, 0};
+      HPCGX_HIP(hipMemcpyAsync(last, lptr.as<int64_t>() + t, sizeof last,
+                               hipMemcpyDeviceToHost, s));
+      HPCGX_HIP(hipStreamSynchronize(s));
+      if (last[1] == last[0]) break;  // level t is empty: traversal done
+      if (t >= cap - 2) throw Error(HPCGX_EUNSUP, "too many schedule levels");
+    }
+    std::vector<int64_t> h(t + 2);
+    HPCGX_HIP(hipMemcpy(h.data(), lptr.p, sizeof(int64_t) * (t + 2),
+                        hipMemcpyDeviceToHost));
+    int s_ = 0;
+    while (s_ + 1 < (int)h.size() && h[s_ + 1] > h[s_]) ++s_;
+    nsched = s_;
+    if (h[nsched] != n)
+      throw Error(HPCGX_EUNSUP, "dependency graph not traversable (cycle?)");
+  }
+  L.nsched = nsched;
+
+  // colour count
+  int ncolors = 0;
+  {
+    std::vector<int32_t> dummy;
+    // max colour via a histogram over kMaxColors bins
+    L.color_ptr = histogram_ptr(n, color.as<int32_t>(), kMaxColors, s);
+    for (int c = 0; c < kMaxColors; ++c)
+      if (L.color_ptr[c + 1] > L.color_ptr[c]) ncolors = c + 1;
+    L.color_ptr.resize(ncolors + 1);
+  }
+  L.ncolors = ncolors;
+
+  // device order = stable sort of rows by colour
+  L.iperm = P.arena.alloc<int32_t>(n);
+  L.perm = P.arena.alloc<int32_t>(n);
+  sort_rows_by_key(n, color.as<int32_t>(), bits_for(ncolors), L.iperm, s);
+  k_invert<<<grid_for(n), kBlock, 0, s>>>(n, L.iperm, L.perm);
+  HPCGX_LAUNCHED();
+
+  // lexicographic schedule: rows sorted by (level, row), renumbered
+  L.level_ptr = histogram_ptr(n, level.as<int32_t>(), nsched, s);
+  L.level_rows = P.arena.alloc<int32_t>(n);
+  sort_rows_by_key(n, level.as<int32_t>(), bits_for(nsched), order.as<int32_t>(), s);
+  k_map<<<grid_for(n), kBlock, 0, s>>>(n, L.perm, order.as<int32_t>(), L.level_rows);
+  HPCGX_LAUNCHED();
+
+  if (flags & HPCGX_KEEP_SCHEDULE) {
+    L.level_of = P.arena.alloc<int32_t>(n);
+    L.color_of = P.arena.alloc<int32_t>(n);
+    HPCGX_HIP(hipMemcpyAsync(L.level_of, level.p, sizeof(int32_t) * n,
+                             hipMemcpyDeviceToDevice, s));
+    HPCGX_HIP(hipMemcpyAsync(L.color_of, color.p, sizeof(int32_t) * n,
+                             hipMemcpyDeviceToDevice, s));
+  }
+
+  // ELL in device order
+  L.col = P.arena.alloc<int32_t>((size_t)kSlots * n);
+  L.val = P.arena.alloc<double>((size_t)kSlots * n);
+  L.diag = P.arena.alloc<double>(n);
+  L.dslot = P.arena.alloc<uint8_t>(n);
+  L.nnz_row = P.arena.alloc<uint8_t>(n);
+  k_permute_ell<<<grid_for(n), kBlock, 0, s>>>(n, L.iperm, L.perm, N.col, N.val,
+                                               N.nnz_row, L.col, L.val, L.diag,
+                                               L.dslot, L.nnz_row);
+  HPCGX_LAUNCHED();
+  HPCGX_HIP(hipStreamSynchronize(s));
+}
+
+void link_levels(Problem &P, int l, const NaturalLevel &Nf, hipStream_t s) {
+  Level &F = P.L[l];
+  const Level &C = P.L[l + 1];
+  F.nc = C.n;
+  F.f2c = P.arena.alloc<int32_t>(C.n);
+  F.rc = P.arena.alloc<double>(C.n);
+  F.xc = P.arena.alloc<double>(C.n);
+  k_link<<<grid_for(C.n), kBlock, 0, s>>>(C.n, C.iperm, Nf.f2c, F.perm, F.f2c);
+  HPCGX_LAUNCHED();
+}
+
+void problem_vectors(const Problem &P, double *b, double *x, double *xexact,
+                     hipStream_t s) {
+  const Level &L = P.L[0];
+  k_vectors<<<grid_for(L.n), kBlock, 0, s>>>(L.n, L.nnz_row, b, x, xexact);
+  HPCGX_LAUNCHED();
+}
+
+}  // namespace hpcgx

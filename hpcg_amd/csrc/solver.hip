@@ -1,0 +1,177 @@
+// solver.hip -- host orchestration of SYMGS, the MG V-cycle and PCG over the
+// device kernels.  Everything is enqueued on one stream; the host waits only
+// where the reference reads a scalar (initial residual, the convergence test
+// when a tolerance is set, the final residual).
+#include <chrono>
+#include <cmath>
+
+#include "internal.hpp"
+
+namespace hpcgx {
+
+// ComputeSYMGS_ref.cpp:67-100: forward sweep, then backward sweep.
+//  * lexicographic ordering: levels of the row DAG in increasing order, then
+//    in decreasing order; rows of a level are independent, so this is the
+//    reference's sequential sweep, row for row;
+//  * multicolour ordering: colours 0..C-1, then C-1..0.
+namespace {
+// One colour step; on the finest level optionally bracketed by HIP events
+// (hpcgx_kernel_timing) so its launches can be timed on their own stream.
+void colour_step(Problem &P, const Level &L, int c, const double *r, double *x,
+                 hipStream_t s) {
+  const bool log = L.finest && P.klog.on;
+  if (log) {
+    auto &K = P.klog;
+    if (K.used + 2 > K.ev.size()) {
+      for (int k = 0; k < 64; ++k) {
+        hipEvent_t e;
+        HPCGX_HIP(hipEventCreate(&e));
+        K.ev.push_back(e);
+      }
+    }
+    HPCGX_HIP(hipEventRecord(K.ev[K.used], s));
+  }
+  launch_symgs_rows(L, L.color_ptr[c], L.color_ptr[c + 1], r, x, s);
+  if (log) {
+    HPCGX_HIP(hipEventRecord(P.klog.ev[P.klog.used + 1], s));
+    P.klog.used += 2;
+  }
+}
+}  // namespace
+
+void symgs(Problem &P, int l, int ordering, const double *r, double *x,
+           hipStream_t s) {
+  const Level &L = P.L[l];
+  if (ordering == HPCGX_ORDER_MULTICOLOR) {
+    for (int c = 0; c < L.ncolors; ++c) colour_step(P, L, c, r, x, s);
+    for (int c = L.ncolors - 1; c >= 0; --c) colour_step(P, L, c, r, x, s);
+  } else {
+    for (int t = 0; t < L.nsched; ++t)
+      launch_symgs_list(L, L.level_rows + L.level_ptr[t],
+                        L.level_ptr[t + 1] - L.level_ptr[t], r, x, s);
+    for (int t = L.nsched - 1; t >= 0; --t)
+      launch_symgs_list(L, L.level_rows + L.level_ptr[t],
+                        L.level_ptr[t + 1] - L.level_ptr[t], r, x, s);
+  }
+}
+
+// ComputeMG_ref.cpp:39-63 with the fine-level SpMV and the restriction fused
+// (only the injected rows of A*x are computed).
+void mg(Problem &P, int l, int ordering, const double *r, double *x,
+        hipStream_t s) {
+  Level &L = P.L[l];
+  launch_fill(x, L.n, 0.0, s);                          // :42
+  if (l + 1 < P.nlevels) {
+    symgs(P, l, ordering, r, x, s);                     // :47
+    launch_residual_restriction(L, r, x, L.rc, s);      // :49-51
+    mg(P, l + 1, ordering, L.rc, L.xc, s);              // :52
+    launch_prolongation(L, L.xc, x, s);                 // :53
+    symgs(P, l, ordering, r, x, s);                     // :55
+  } else {
+    symgs(P, l, ordering, r, x, s);                     // :59
+  }
+}
+
+namespace {
+double read_scalar(const Problem &P, int slot, hipStream_t s) {
+  double v = 0.0;
+  HPCGX_HIP(hipMemcpyAsync(&v, P.scalars + slot, sizeof v,
+                           hipMemcpyDeviceToHost, s));
+  HPCGX_HIP(hipStreamSynchronize(s));
+  return v;
+}
+
+struct PhaseTimer {  // HIP-event timing of CG phases (times[1..5])
+  bool on;
+  hipStream_t s;
+  hipEvent_t a = nullptr, b = nullptr;
+  PhaseTimer(bool on_, hipStream_t s_) : on(on_), s(s_) {
+    if (on) {
+      HPCGX_HIP(hipEventCreate(&a));
+      HPCGX_HIP(hipEventCreate(&b));
+    }
+  }
+  ~PhaseTimer() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+  void tick() {
+    if (on) HPCGX_HIP(hipEventRecord(a, s));
+  }
+  void tock(double &acc) {
+    if (!on) return;
+    HPCGX_HIP(hipEventRecord(b, s));
+    HPCGX_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    HPCGX_HIP(hipEventElapsedTime(&ms, a, b));
+    acc += ms * 1e-3;
+  }
+};
+}  // namespace
+
+// CG.cpp:59-143.  With the reductions fused into their producers:
+//   r.z            -> dot kernel that also forms beta = rtz/oldrtz (:112-114)
+//   Ap = A p, p.Ap -> one SpMV kernel that also forms alpha (:118-120)
+//   x, r updates, r.r -> one kernel (:121-124)
+void cg(Problem &P, int ordering, const double *b, double *x, int max_iter,
+        double tol, bool precond, int flags, int *niters, double *normr,
+        double *normr0, double *times, hipStream_t s) {
+  const auto t_begin = std::chrono::steady_clock::now();
+  const Level &A = P.L[0];
+  const int64_t n = A.n;
+  double t1 = 0, t2 = 0, t3 = 0, t5 = 0;
+  PhaseTimer T((flags & HPCGX_CG_TIMERS) != 0, s);
+
+  *niters = 0;
+  launch_copy(x, P.p, n, s);                             // :86
+  T.tick();
+  launch_spmv(A, P.p, P.Ap, s);                          // :87
+  T.tock(t3);
+  T.tick();
+  launch_residual0(P, n, b, P.Ap, P.r, s);               // :88-89
+  T.tock(t2);
+  double nr = std::sqrt(read_scalar(P, kNormr2, s));
+  *normr0 = nr;
+  const bool check_each = tol > 0.0;
+  int k = 1;
+  for (; k <= max_iter && nr / *normr0 > tol; ++k) {     // :100
+    T.tick();
+    if (precond)
+      mg(P, 0, ordering, P.r, P.z, s);                   // :103
+    else
+      launch_copy(P.r, P.z, n, s);                       // :105
+    T.tock(t5);
+    T.tick();
+    launch_dot(P, n, P.r, P.z, nullptr, kFinishBeta, s); // :110 / :113-114
+    T.tock(t1);
+    T.tick();
+    if (k == 1)
+      launch_copy(P.z, P.p, n, s);                       // :109
+    else
+      launch_update_p(n, P.z, P.p, P.scalars, s);        // :115
+    T.tock(t2);
+    T.tick();
+    launch_spmv_dot(P, A, P.p, P.Ap, s);                 // :118-120
+    T.tock(t3);
+    T.tick();
+    launch_update_xr(P, n, x, P.r, P.p, P.Ap, s);        // :121-123
+    T.tock(t2);
+    *niters = k;
+    if (check_each || k == max_iter) nr = std::sqrt(read_scalar(P, kNormr2, s));
+  }
+  if (!check_each) nr = std::sqrt(read_scalar(P, kNormr2, s));
+  HPCGX_HIP(hipStreamSynchronize(s));
+  *normr = nr;
+  if (times) {
+    times[0] += std::chrono::duration<double>(std::chrono::steady_clock::now() -
+                                              t_begin).count();
+    times[1] += t1;
+    times[2] += t2;
+    times[3] += t3;
+    times[5] += t5;
+  }
+  if (!std::isfinite(nr))
+    throw Error(HPCGX_EINVAL, "CG residual is not finite");
+}
+
+}  // namespace hpcgx

@@ -1,0 +1,216 @@
+/*
+ * hpcg_amd.h -- C ABI of the MI355X-native HPCG PCG path (libhpcg_amd.so).
+ *
+ * Plain pointers and sizes only.  Every vector argument is a DEVICE pointer
+ * (hipMalloc'd or a torch tensor's storage) in the problem's device row
+ * order unless the name says "natural".  Calls are asynchronous on `stream`
+ * (a hipStream_t; NULL = the default stream) unless documented otherwise.
+ * Every entry point returns 0 on success and a nonzero HPCGX_E* code on
+ * failure (never throws); hpcgx_last_error() describes the last failure of
+ * the calling thread.
+ *
+ * Each declaration names the reference interface (HPCG 3.1, src/...) it
+ * replaces.  Calls on one problem handle are single threaded, like the
+ * reference (every hot-path function is called from one host thread).
+ */
+#ifndef HPCG_AMD_H
+#define HPCG_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HPCGX_ABI_VERSION 1
+
+enum {
+  HPCGX_OK = 0,
+  HPCGX_EINVAL = 1,  /* bad argument (sizes, level, ordering, NULL) */
+  HPCGX_EHIP = 2,    /* a HIP runtime call or kernel launch failed */
+  HPCGX_ENOMEM = 3,  /* device allocation failed */
+  HPCGX_EUNSUP = 4   /* matrix outside what the setup supports */
+};
+
+/* Gauss-Seidel row orderings for SYMGS / MG / CG */
+enum {
+  HPCGX_ORDER_LEXICOGRAPHIC = 0, /* the reference's natural row order,
+                                    run level by level (level-scheduled) */
+  HPCGX_ORDER_MULTICOLOR = 1     /* greedy first-fit colours, one parallel
+                                    step per colour */
+};
+
+/* flags for problem creation */
+enum {
+  HPCGX_KEEP_SCHEDULE = 1 /* keep per-row level and colour ids so that
+                             hpcgx_get_schedule can return them */
+};
+
+typedef struct hpcgx_problem hpcgx_problem; /* device-resident MG hierarchy */
+
+/* Description of one level as the reference stores it (SparseMatrix.hpp:39-77
+ * of the reference: arrays of row pointers, <=27 entries per row, columns
+ * ascending so lower-triangular terms precede the diagonal). */
+typedef struct {
+  int nx, ny, nz;                     /* local grid (Geometry.hpp:54-56) */
+  int64_t nrow;                       /* localNumberOfRows */
+  const char *nonzerosInRow;          /* [nrow] */
+  const int *const *mtxIndL;          /* [nrow] -> row's local column ids */
+  const double *const *matrixValues;  /* [nrow] -> row's values */
+  const double *const *matrixDiagonal;/* [nrow] -> diagonal entry */
+  const int *f2cOperator;             /* MGData.hpp:31, NULL on coarsest */
+} hpcgx_host_level;
+
+typedef struct {
+  int nx, ny, nz;
+  int64_t nrow;          /* rows of the level */
+  int64_t nnz;           /* stored nonzeros (64-bit) */
+  int ncolors;           /* colours of the multicolour schedule */
+  int nlevels_sched;     /* hyperplanes of the lexicographic schedule */
+  int64_t ncoarse;       /* rows of the next level (0 on coarsest) */
+} hpcgx_level_info;
+
+const char *hpcgx_last_error(void);
+int hpcgx_abi_version(void);
+
+/* ---- setup --------------------------------------------------------------- */
+
+/* replaces OptimizeProblem (OptimizeProblem.cpp:37-100 of the reference,
+ * declared OptimizeProblem.hpp:22): converts every level's CSR to a
+ * column-major 27-slot ELL in HBM, builds the lexicographic level schedule
+ * and the greedy colour schedule on the device, and reorders rows by colour.
+ * `levels[0]` is the fine level; levels[l].f2cOperator maps level l+1 rows to
+ * level l rows (GenerateCoarseProblem.cpp:71-82). */
+int hpcgx_optimize_problem(const hpcgx_host_level *levels, int nlevels,
+                           int flags, hpcgx_problem **out);
+
+/* replaces GenerateProblem + SetupHalo + GenerateCoarseProblem
+ * (GenerateProblem_ref.cpp:50-219, SetupHalo_ref.cpp:58-65,
+ * GenerateCoarseProblem.cpp:41-111; main.cpp:133-144 builds 4 levels)
+ * followed by OptimizeProblem: the 27-point problem on an nx*ny*nz grid is
+ * generated directly in device memory, level by level. */
+int hpcgx_generate_problem(int nx, int ny, int nz, int nlevels, int flags,
+                           hpcgx_problem **out);
+
+/* replaces DeleteMatrix / DeleteMGData for device state
+ * (SparseMatrix.hpp:151-183, MGData.hpp:65-75). */
+int hpcgx_problem_destroy(hpcgx_problem *p);
+
+int hpcgx_problem_nlevels(const hpcgx_problem *p);
+int hpcgx_problem_info(const hpcgx_problem *p, int level,
+                       hpcgx_level_info *info);
+
+/* replaces OptimizeProblemMemoryUse (OptimizeProblem.cpp:103-107): bytes of
+ * device memory held for the optimized data structures. */
+double hpcgx_optimize_problem_memory_use(const hpcgx_problem *p);
+
+/* b, x, xexact of GenerateProblem_ref.cpp:178-180 (b_i = 27 - nnz_i, x = 0,
+ * xexact = 1) written in device order on the fine level. Any may be NULL. */
+int hpcgx_problem_vectors(hpcgx_problem *p, double *b, double *x,
+                          double *xexact, void *stream);
+
+/* Per-row schedule ids in NATURAL row order, copied to host memory:
+ * which = 0 -> level id of the lexicographic schedule, 1 -> colour.
+ * Synchronous.  Needs HPCGX_KEEP_SCHEDULE. */
+int hpcgx_get_schedule(const hpcgx_problem *p, int level, int which,
+                       int32_t *out_host);
+
+/* ---- residency (host touch points of Vector.hpp / SparseMatrix.hpp) ------ */
+
+/* natural -> device order and back (device pointers, length nrow of level). */
+int hpcgx_to_device_order(const hpcgx_problem *p, int level,
+                          const double *natural, double *devorder,
+                          void *stream);
+int hpcgx_to_natural_order(const hpcgx_problem *p, int level,
+                           const double *devorder, double *natural,
+                           void *stream);
+
+/* replaces ReplaceMatrixDiagonal / CopyMatrixDiagonal (SparseMatrix.hpp:
+ * 126-145): diagonal in device order (matrix values and the separate
+ * diagonal array are both updated). */
+int hpcgx_replace_diagonal(hpcgx_problem *p, int level, const double *diag,
+                           void *stream);
+int hpcgx_copy_diagonal(const hpcgx_problem *p, int level, double *diag,
+                        void *stream);
+
+/* ---- kernels --------------------------------------------------------------- */
+
+/* replaces ComputeSPMV (ComputeSPMV.cpp:40-45 -> ComputeSPMV_ref.cpp:47-72):
+ * y = A_level * x.  Per-row summation order j = 0..nnz_i-1 as the reference,
+ * no fused multiply-add. */
+int hpcgx_spmv(const hpcgx_problem *p, int level, const double *x, double *y,
+               void *stream);
+
+/* replaces ComputeSYMGS (ComputeSYMGS.cpp:50-55 -> ComputeSYMGS_ref.cpp:
+ * 54-103): one forward and one backward Gauss-Seidel sweep on x with right
+ * hand side r, in the given ordering (HPCGX_ORDER_*). */
+int hpcgx_symgs(const hpcgx_problem *p, int level, int ordering,
+                const double *r, double *x, void *stream);
+
+/* replaces ComputeMG (ComputeMG.cpp:33-38 -> ComputeMG_ref.cpp:39-63): the
+ * V-cycle from `level` down, x := M^-1 r (x is zeroed first). Uses the
+ * problem's own coarse-level vectors. */
+int hpcgx_mg(hpcgx_problem *p, int level, int ordering, const double *r,
+             double *x, void *stream);
+
+/* replaces ComputeRestriction_ref (ComputeRestriction_ref.cpp:40-54):
+ * rc = rf[f2c] - Axf[f2c]; rc has the next level's length. */
+int hpcgx_restriction(const hpcgx_problem *p, int level, const double *rf,
+                      const double *Axf, double *rc, void *stream);
+
+/* ComputeSPMV_ref + ComputeRestriction_ref fused (ComputeMG_ref.cpp:49-51):
+ * rc = rf[f2c] - (A xf)[f2c], computing A xf only at the injected rows. */
+int hpcgx_residual_restriction(const hpcgx_problem *p, int level,
+                               const double *rf, const double *xf, double *rc,
+                               void *stream);
+
+/* replaces ComputeProlongation_ref (ComputeProlongation_ref.cpp:38-52):
+ * xf[f2c] += xc. */
+int hpcgx_prolongation(const hpcgx_problem *p, int level, const double *xc,
+                       double *xf, void *stream);
+
+/* replaces ComputeDotProduct (ComputeDotProduct.cpp:41-47 ->
+ * ComputeDotProduct_ref.cpp:46-80): *result_dev = sum_i x_i y_i, summed in a
+ * fixed order (bit-identical on every call).  result_dev is device memory. */
+int hpcgx_dot(hpcgx_problem *p, int64_t n, const double *x, const double *y,
+              double *result_dev, void *stream);
+
+/* As hpcgx_dot but synchronous, result in host memory. */
+int hpcgx_dot_host(hpcgx_problem *p, int64_t n, const double *x,
+                   const double *y, double *result, void *stream);
+
+/* replaces ComputeWAXPBY (ComputeWAXPBY.cpp:42-48 -> ComputeWAXPBY_ref.cpp:
+ * 42-70): w = alpha x + beta y with the reference's alpha==1 and beta==1
+ * forms; w may alias x or y. */
+int hpcgx_waxpby(int64_t n, double alpha, const double *x, double beta,
+                 const double *y, double *w, void *stream);
+
+/* replaces CG (CG.cpp:59-143 of the reference; CG_ref.cpp:58-144 is the same
+ * algorithm on the *_ref kernels).  b, x in device order on the fine level;
+ * x holds the initial guess and receives the solution.  The work vectors r,
+ * z, p, Ap (CGData.hpp:27-32) belong to the problem.  times[0..5] are
+ * accumulated like the reference (total, dot, waxpby, spmv, allreduce = 0,
+ * MG) when times != NULL; times[1..5] are measured with HIP events only if
+ * HPCGX_CG_TIMERS is set in `flags`.  Synchronous. */
+enum { HPCGX_CG_TIMERS = 1 };
+int hpcgx_cg(hpcgx_problem *p, int ordering, const double *b, double *x,
+             int max_iter, double tolerance, int precondition, int flags,
+             int *niters, double *normr, double *normr0, double *times,
+             void *stream);
+
+/* ---- measurement ----------------------------------------------------------- */
+
+/* No counterpart in the reference (its timers are host clocks around whole
+ * kernels, CG.cpp:36-37): records a HIP event pair on the launch stream
+ * around every launch of the finest level's multicolour Gauss-Seidel step,
+ * the kernel that dominates the V-cycle.  enable = 1 clears the log and
+ * starts recording, 0 stops. */
+int hpcgx_kernel_timing(hpcgx_problem *p, int enable);
+/* Synchronous: summed event time (ms) and number of recorded launches. */
+int hpcgx_kernel_timing_read(hpcgx_problem *p, double *total_ms,
+                             int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HPCG_AMD_H */
