@@ -112,9 +112,12 @@ def main():
     cb = model.canonical_bytes(lv)
     pre_bytes = cb["outer_spmv"] + 7 * 8 * lv[0]["n"]  # preamble: copy, SpMV, r = b - Ap, r.r
     gbs = (K * cb["iteration"] + pre_bytes) / dt * 1e-9
-    # dominant kernel: one colour step of the finest level's Gauss-Seidel
+    # dominant kernel: the finest level's multicolour Gauss-Seidel launches
+    # (two symmetric sweeps per iteration: pre- and post-smoother)
     info0 = P.info(0)
-    per_launch = cb["symgs_L0"] / (2 * info0["ncolors"])
+    kname = ("k_gs_wave<true,true> (finest-level wavefront sweep, one direction)"
+             if info0["wave"] else "k_gs_range<true,true> (finest-level colour step)")
+    per_launch = K * 2 * cb["symgs_L0"] / max(ks_n, 1)
     launch_ms = ks_ms / max(ks_n, 1)
     achieved = per_launch / (launch_ms * 1e-3) * 1e-9 if ks_n else None
     rating = flops["total"] / (dt + setup_s / 10.0) * 1e-9  # ReportResults.cpp:353, one set
@@ -142,7 +145,7 @@ def main():
         "setup_s": round(setup_s, 3),
         "normr_over_normr0": r["normr"] / r["normr0"],
         "roofline": {
-            "kernel": "k_gs_range<true> (finest-level colour step)",
+            "kernel": kname,
             "bound": "hbm",
             "achieved": round(achieved, 1) if achieved else None,
             "peak": HBM_PEAK_GBS,

@@ -38,7 +38,7 @@ class LevelInfo(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
                 ("nrow", ctypes.c_int64), ("nnz", ctypes.c_int64),
                 ("ncolors", ctypes.c_int), ("nlevels_sched", ctypes.c_int),
-                ("ncoarse", ctypes.c_int64)]
+                ("ncoarse", ctypes.c_int64), ("wave", ctypes.c_int), ("nslabs", ctypes.c_int)]
 
 
 # name -> (restype, argtypes)
@@ -69,6 +69,8 @@ SIGNATURES = {
     "hpcgx_dot_host": (I, [P, I64, P, P, P, P]),
     "hpcgx_waxpby": (I, [I64, D, P, D, P, P, P]),
     "hpcgx_cg": (I, [P, I, P, P, I, D, I, I, P, P, P, P, P]),
+    "hpcgx_tune": (I, [P, ctypes.c_char_p, D]),
+    "hpcgx_check_status": (I, [P, P]),
     "hpcgx_kernel_timing": (I, [P, I]),
     "hpcgx_kernel_timing_read": (I, [P, P, P]),
 }

@@ -101,8 +101,11 @@ void alloc_solver_state(hpcgx::Problem &P) {
   P.partials = P.arena.alloc<double>(hpcgx::kReduceBlocks);
   P.ticket = P.arena.alloc<unsigned>(1);
   P.scalars = P.arena.alloc<double>(hpcgx::kNumScalars);
+  P.wave_err = P.arena.alloc<int>(1);
   HPCGX_HIP(hipMemset(P.ticket, 0, sizeof(unsigned)));
   HPCGX_HIP(hipMemset(P.scalars, 0, sizeof(double) * hpcgx::kNumScalars));
+  HPCGX_HIP(hipMemset(P.wave_err, 0, sizeof(int)));
+  P.wave_grid = hpcgx::wave_grid_size();
 }
 
 // shared tail of both setup paths: schedules, reordering, links
@@ -238,6 +241,8 @@ int hpcgx_problem_info(const hpcgx_problem *p, int level, hpcgx_level_info *info
     info->ncolors = L.ncolors;
     info->nlevels_sched = L.nsched;
     info->ncoarse = L.nc;
+    info->wave = (L.wave && p->P.use_wave) ? 1 : 0;
+    info->nslabs = L.nslabs;
   });
 }
 
@@ -372,6 +377,46 @@ int hpcgx_waxpby(int64_t n, double alpha, const double *x, double beta, const do
   return guard([&] {
     if (n < 0) throw Error(HPCGX_EINVAL, "negative length");
     hpcgx::launch_waxpby(n, alpha, x, beta, y, w, S(stream));
+  });
+}
+
+int hpcgx_tune(hpcgx_problem *p, const char *key, double value) {
+  return guard([&] {
+    if (!p || !key) throw Error(HPCGX_EINVAL, "NULL argument");
+    const std::string k(key);
+    hpcgx::Problem &P = p->P;
+    if (k == "wave") {
+      P.use_wave = value != 0.0;
+      return;
+    }
+    for (int l = 0; l < P.nlevels; ++l) {
+      hpcgx::Level &L = P.L[l];
+      if (k == "xcd_remap") {
+        L.xcd = value != 0.0;
+      } else if (k == "stream_min_bytes") {
+        L.stream = (double)L.n * kSlots * 12.0 > value;
+      } else if (k == "wave_min_rows") {
+        if ((double)L.n >= value) {
+          if (!L.wave) hpcgx::build_wave(P, L, hpcgx::kWaveSlabFactor, nullptr);
+        } else {
+          hpcgx::drop_wave(P, L);
+        }
+      } else if (k == "wave_slab_factor") {
+        if (value < 1) throw Error(HPCGX_EINVAL, "wave_slab_factor must be >= 1");
+        if (L.wave) hpcgx::build_wave(P, L, (int)value, nullptr);
+      } else {
+        throw Error(HPCGX_EINVAL, "unknown tuning key " + k);
+      }
+    }
+    HPCGX_HIP(hipDeviceSynchronize());
+  });
+}
+
+int hpcgx_check_status(hpcgx_problem *p, void *stream) {
+  return guard([&] {
+    if (!p) throw Error(HPCGX_EINVAL, "NULL problem");
+    HPCGX_HIP(hipStreamSynchronize(S(stream)));
+    hpcgx::check_status(p->P);
   });
 }
 

@@ -18,6 +18,9 @@ constexpr int kMaxLevels = 8;
 constexpr int kMaxColors = 32;    // first-fit colours of a <=27-entry row: <=27
 constexpr int kBlock = 256;       // threads per workgroup for streaming kernels
 constexpr int kReduceBlocks = 1024;  // partial sums of a grid reduction
+constexpr double kStreamBytes = 512e6;  // matrices above this are streamed
+constexpr int64_t kWaveMinRows = 1 << 23;  // levels swept as a wavefront
+constexpr int kWaveSlabFactor = 2;         // slab width / matrix bandwidth
 
 struct Error : std::runtime_error {
   int code;
@@ -74,7 +77,9 @@ struct DeviceArena {
 // sit after the row's stored entries, so a full 27-slot traversal adds exact
 // zeros after the reference's terms.
 struct Level {
-  bool finest = false;  // level 0: its matrix is streamed with non-temporal loads
+  bool finest = false;  // level 0 (its kernels get their own symbols)
+  bool stream = false;  // matrix >> last-level cache: non-temporal matrix loads
+  bool xcd = false;     // XCD-aware workgroup -> rows mapping (measured slower)
   int nx = 0, ny = 0, nz = 0;
   int64_t n = 0, nnz = 0;
   int32_t *col = nullptr;   // [kSlots * n] device-order column ids
@@ -91,6 +96,19 @@ struct Level {
   int nsched = 0;
   std::vector<int64_t> level_ptr;
   int32_t *level_rows = nullptr;  // device-order row ids, level-major
+  // row chunks (<= kBlock rows, [begin, end) pairs) interleaving the colours
+  int64_t nchunks = 0;
+  int64_t *chunks = nullptr;
+  // wavefront multicolour sweeps: tile (c, s) = device rows of colour c in
+  // natural-row slab s = [tile_ptr[c*(nslabs+1)+s], tile_ptr[c*(nslabs+1)+s+1])
+  int64_t bandwidth = 0;  // max |col - row| in natural order
+  bool wave = false;
+  int64_t wave_slab = 0;
+  int nslabs = 0;
+  int64_t *tile_ptr = nullptr;
+  int64_t ntasks[2] = {0, 0};       // forward, backward
+  int32_t *tasks[2] = {nullptr, nullptr};  // (phase*nslabs + slab, chunk) pairs
+  unsigned *done = nullptr;          // [ncolors*nslabs] finished chunks per tile
   // optional per-row ids (natural order) kept for inspection
   int32_t *level_of = nullptr, *color_of = nullptr;
   // transfer to the next level: f2c[coarse device row] = fine device row
@@ -111,6 +129,9 @@ struct Problem {
   double *partials = nullptr;
   unsigned *ticket = nullptr;
   double *scalars = nullptr;  // see CgScalar
+  int *wave_err = nullptr;    // set when a wavefront wait gives up
+  int wave_grid = 0;          // co-resident workgroups of the wavefront kernel
+  bool use_wave = true;       // multicolour sweeps as wavefronts where built
   // event log of the finest level's multicolour Gauss-Seidel launches
   struct KernelLog {
     bool on = false;
@@ -156,6 +177,9 @@ void launch_update_p(int64_t n, const double *z, double *p,
 void launch_residual0(Problem &P, int64_t n, const double *b, const double *Ap,
                       double *r, hipStream_t s);
 void launch_replace_diag(Level &L, const double *d, hipStream_t s);
+void launch_symgs_wave(Problem &P, const Level &L, int dir, const double *r,
+                       double *x, hipStream_t s);
+int wave_grid_size();
 
 // ---- setup (setup.hip) ----
 struct NaturalLevel {  // device buffers of one level in natural row order
@@ -171,6 +195,8 @@ void generate_natural(NaturalLevel &N, hipStream_t s);
 void finalize_level(Problem &P, int l, NaturalLevel &N, int flags,
                     hipStream_t s);
 void link_levels(Problem &P, int l, const NaturalLevel &Nf, hipStream_t s);
+void build_wave(Problem &P, Level &L, int slab_factor, hipStream_t s);
+void drop_wave(Problem &P, Level &L);
 void problem_vectors(const Problem &P, double *b, double *x, double *xexact,
                      hipStream_t s);
 
@@ -179,6 +205,7 @@ void symgs(Problem &P, int l, int ordering, const double *r, double *x,
            hipStream_t s);
 void mg(Problem &P, int l, int ordering, const double *r, double *x,
         hipStream_t s);
+void check_status(Problem &P);
 void cg(Problem &P, int ordering, const double *b, double *x, int max_iter,
         double tol, bool precond, int flags, int *niters, double *normr,
         double *normr0, double *times, hipStream_t s);

@@ -7,6 +7,8 @@
 // bit.  Dot products use a fixed-shape two-level tree, so they are
 // bit-identical from call to call (TestNorms variance 0) and differ from the
 // reference's sequential sum only by reassociation.
+#include <algorithm>
+
 #include "internal.hpp"
 
 namespace hpcgx {
@@ -69,12 +71,22 @@ __global__ __launch_bounds__(kBlock) void k_waxpby(int64_t n, double alpha,
 // ---------------------------------------------------------------------------
 // Matrix loads.  STREAM = the level's matrix is far larger than the 256 MB
 // last-level cache, so it is read with non-temporal loads that do not evict
-// the x vector; coarse levels read it through the caches, where it can stay
-// resident between the two sweeps of a smoother step.
+// the x vector; small levels read it through the caches, where it can stay
+// resident between the sweeps of a smoother step.
 template <bool STREAM, class T>
 __device__ inline T mload(const T *p) {
   if (STREAM) return __builtin_nontemporal_load(p);
   return *p;
+}
+
+// The hardware hands workgroup b to XCD b % 8.  Remapped, XCD x walks its own
+// contiguous eighth of the logical blocks, so each XCD's L2 works on one
+// narrow window of the gathered vector instead of all eight sharing it.
+__device__ inline int64_t xcd_block(int64_t b, int64_t G, bool on) {
+  if (!on) return b;
+  const int64_t G8 = G - G % 8;
+  if (b >= G8) return b;
+  return (b % 8) * (G8 / 8) + b / 8;
 }
 
 // y_i = sum_j a_ij x_col(i,j)  (ComputeSPMV_ref.cpp:62-69)
@@ -92,54 +104,127 @@ __device__ inline double row_dot(const int32_t *__restrict__ col,
   return sum;
 }
 
+// SpMV over row chunks (<= kBlock rows each).  The chunk list interleaves the
+// colour blocks (chunk k of every colour before chunk k+1 of any), so the x
+// values gathered from all colour blocks stay within a narrow window.
 template <bool STREAM>
-__global__ __launch_bounds__(kBlock) void k_spmv(int64_t n,
-                                                  const int32_t *__restrict__ col,
-                                                  const double *__restrict__ val,
-                                                  const double *__restrict__ x,
-                                                  double *__restrict__ y) {
-  const int64_t i = gtid();
-  if (i < n) y[i] = row_dot<STREAM>(col, val, x, n, i);
+__global__ __launch_bounds__(kBlock) void k_spmv(
+    const int64_t *__restrict__ chunks, int64_t n,
+    const int32_t *__restrict__ col, const double *__restrict__ val,
+    const double *__restrict__ x, double *__restrict__ y, bool xcd) {
+  const int64_t b = xcd_block(blockIdx.x, gridDim.x, xcd);
+  const int64_t i = chunks[2 * b] + threadIdx.x;
+  if (i < chunks[2 * b + 1]) y[i] = row_dot<STREAM>(col, val, x, n, i);
 }
 
 // One Gauss-Seidel row update in the reference's form
-// (ComputeSYMGS_ref.cpp:72-80): subtract every stored term including the
-// diagonal one, add the diagonal term back, divide by the diagonal.
+// (ComputeSYMGS_ref.cpp:71-80): subtract every stored term including the
+// diagonal one, add the diagonal term back, divide by the diagonal.  The
+// diagonal value is taken from the row's own slots (the first one whose
+// column is the row: unused slots, which also point at the row, come after
+// it), so no separate diagonal array is read.
 template <bool STREAM>
 __device__ inline void gs_row(int64_t i, int64_t n,
                               const int32_t *__restrict__ col,
                               const double *__restrict__ val,
-                              const double *__restrict__ diag,
                               const double *__restrict__ r, double *x) {
   double sum = r[i];
+  double d = 0.0;
+  bool have = false;
 #pragma unroll
   for (int j = 0; j < kSlots; ++j) {
     const int64_t k = (int64_t)j * n + i;
-    sum -= mload<STREAM>(val + k) * x[mload<STREAM>(col + k)];
+    const double v = mload<STREAM>(val + k);
+    const int32_t c = mload<STREAM>(col + k);
+    sum -= v * x[c];
+    if (!have && c == i) {
+      d = v;
+      have = true;
+    }
   }
-  const double d = diag[i];
   sum += x[i] * d;
   x[i] = sum / d;
 }
 
 // All rows of one colour (a contiguous device-order range) at once: rows of
 // one colour share no matrix entry, so their updates are independent.
-template <bool STREAM>
+// FINEST only separates the finest level's launches (a distinct symbol for
+// profiling); the code is the same.
+template <bool STREAM, bool FINEST>
 __global__ __launch_bounds__(kBlock) void k_gs_range(
     int64_t begin, int64_t end, int64_t n, const int32_t *__restrict__ col,
-    const double *__restrict__ val, const double *__restrict__ diag,
-    const double *__restrict__ r, double *x) {
-  const int64_t i = begin + gtid();
-  if (i < end) gs_row<STREAM>(i, n, col, val, diag, r, x);
+    const double *__restrict__ val, const double *__restrict__ r, double *x,
+    bool xcd) {
+  const int64_t b = xcd_block(blockIdx.x, gridDim.x, xcd);
+  const int64_t i = begin + b * kBlock + threadIdx.x;
+  if (i < end) gs_row<STREAM>(i, n, col, val, r, x);
+}
+
+// Multicolour sweep in one launch, as a wavefront over natural-row slabs
+// (see build_wave for the task order).  The grid is co-resident (cooperative
+// launch) and workgroup g runs tasks g, g+G, g+2G, ... in list order, so the
+// earliest unfinished task always has its inputs done: the waits end.  A wait
+// that exceeds kSpinLimit polls (seconds) sets *err and gives up, as do all
+// later ones; the sweep then finishes with wrong values instead of hanging.
+// Cross-workgroup visibility of x: every thread fences (agent scope) after
+// its update, thread 0 publishes with a release add; the waiting thread 0
+// acquires, every thread fences before reading.
+constexpr unsigned kSpinLimit = 1u << 22;
+
+template <bool STREAM, bool FINEST>
+__global__ __launch_bounds__(kBlock) void k_gs_wave(
+    const int32_t *__restrict__ tasks, int64_t ntasks,
+    const int64_t *__restrict__ tile_ptr, int S, int C, int dir,
+    unsigned *done, int *err, int64_t n, const int32_t *__restrict__ col,
+    const double *__restrict__ val, const double *__restrict__ r, double *x) {
+  for (int64_t k = blockIdx.x; k < ntasks; k += gridDim.x) {
+    const int tile = tasks[2 * k];
+    const int chunk = tasks[2 * k + 1];
+    const int p = tile / S, sl = tile % S;
+    if (threadIdx.x == 0 && p > 0) {
+      const int cq = dir == 0 ? p - 1 : C - p;  // colour of phase p-1
+      const int64_t *tq = tile_ptr + (int64_t)cq * (S + 1);
+      const int lo = sl > 0 ? sl - 1 : 0, hi = sl + 1 < S ? sl + 1 : S - 1;
+      for (int s2 = lo; s2 <= hi; ++s2) {
+        const unsigned need = (unsigned)((tq[s2 + 1] - tq[s2] + kBlock - 1) / kBlock);
+        unsigned spins = 0;
+        while (__hip_atomic_load(&done[(p - 1) * S + s2], __ATOMIC_ACQUIRE,
+                                 __HIP_MEMORY_SCOPE_AGENT) < need) {
+          __builtin_amdgcn_s_sleep(2);
+          if (++spins > kSpinLimit ||
+              __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            atomicOr(err, 1);
+            break;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const int c = dir == 0 ? p : C - 1 - p;
+    const int64_t *tp = tile_ptr + (int64_t)c * (S + 1);
+    const int64_t i = tp[sl] + (int64_t)chunk * kBlock + threadIdx.x;
+    if (i < tp[sl + 1]) gs_row<STREAM>(i, n, col, val, r, x);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0)
+      __hip_atomic_fetch_add(&done[p * S + sl], 1u, __ATOMIC_RELEASE,
+                             __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_zero_u32(unsigned *a, int64_t n) {
+  const int64_t i = gtid();
+  if (i < n) a[i] = 0u;
 }
 
 // All rows of one level of the lexicographic schedule (a list of rows).
 __global__ __launch_bounds__(kBlock) void k_gs_list(
     const int32_t *__restrict__ rows, int64_t count, int64_t n,
     const int32_t *__restrict__ col, const double *__restrict__ val,
-    const double *__restrict__ diag, const double *__restrict__ r, double *x) {
+    const double *__restrict__ r, double *x) {
   const int64_t k = gtid();
-  if (k < count) gs_row<false>(rows[k], n, col, val, diag, r, x);
+  if (k < count) gs_row<false>(rows[k], n, col, val, r, x);
 }
 
 // ComputeRestriction_ref.cpp:51
@@ -159,8 +244,8 @@ __global__ __launch_bounds__(kBlock) void k_resid_restrict(
     int64_t nc, int64_t n, const int32_t *__restrict__ f2c,
     const int32_t *__restrict__ col, const double *__restrict__ val,
     const double *__restrict__ rf, const double *__restrict__ xf,
-    double *__restrict__ rc) {
-  const int64_t c = gtid();
+    double *__restrict__ rc, bool xcd) {
+  const int64_t c = xcd_block(blockIdx.x, gridDim.x, xcd) * kBlock + threadIdx.x;
   if (c < nc) {
     const int64_t f = f2c[c];
     rc[c] = rf[f] - row_dot<STREAM>(col, val, xf, n, f);
@@ -261,20 +346,26 @@ __global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const double *x,
   grid_finish(s, partials, ticket, scalars, out, finish);
 }
 
-// Ap = A p and p.Ap (CG.cpp:118-120) in one pass: each workgroup owns a
-// contiguous block of rows so the x-gathers stay local.
+// Ap = A p and p.Ap (CG.cpp:118-120) in one pass.  A persistent grid (a
+// multiple of 8 workgroups): XCD x (= blockIdx % 8) owns the x-th eighth of
+// the colour-interleaved chunk list and its workgroups stride through it, so
+// both the partial sums and each XCD's gather window are fixed.
+template <bool STREAM>
 __global__ __launch_bounds__(kBlock) void k_spmv_dot(
-    int64_t n, const int32_t *__restrict__ col, const double *__restrict__ val,
+    int64_t nchunks, const int64_t *__restrict__ chunks, int64_t n,
+    const int32_t *__restrict__ col, const double *__restrict__ val,
     const double *__restrict__ p, double *__restrict__ Ap, double *partials,
-    unsigned *ticket, double *scalars, int64_t rows_per_block) {
-  const int64_t lo = (int64_t)blockIdx.x * rows_per_block;
-  int64_t hi = lo + rows_per_block;
-  if (hi > n) hi = n;
+    unsigned *ticket, double *scalars) {
+  const int64_t x = blockIdx.x % 8, slot = blockIdx.x / 8, per = gridDim.x / 8;
+  const int64_t lo = x * nchunks / 8, hi = (x + 1) * nchunks / 8;
   double s = 0.0;
-  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-    const double y = row_dot<true>(col, val, p, n, i);
-    Ap[i] = y;
-    s += p[i] * y;
+  for (int64_t w = lo + slot; w < hi; w += per) {
+    const int64_t i = chunks[2 * w] + threadIdx.x;
+    if (i < chunks[2 * w + 1]) {
+      const double y = row_dot<STREAM>(col, val, p, n, i);
+      Ap[i] = y;
+      s += p[i] * y;
+    }
   }
   s = block_sum(s);
   grid_finish(s, partials, ticket, scalars, nullptr, kFinishAlpha);
@@ -348,22 +439,27 @@ void launch_gather(const int32_t *idx, const double *src, double *dst,
 }
 
 void launch_spmv(const Level &L, const double *x, double *y, hipStream_t s) {
-  if (L.finest)
-    k_spmv<true><<<grid_for(L.n), kBlock, 0, s>>>(L.n, L.col, L.val, x, y);
+  const unsigned g = (unsigned)L.nchunks;
+  if (L.stream)
+    k_spmv<true><<<g, kBlock, 0, s>>>(L.chunks, L.n, L.col, L.val, x, y, L.xcd);
   else
-    k_spmv<false><<<grid_for(L.n), kBlock, 0, s>>>(L.n, L.col, L.val, x, y);
+    k_spmv<false><<<g, kBlock, 0, s>>>(L.chunks, L.n, L.col, L.val, x, y, L.xcd);
   HPCGX_LAUNCHED();
 }
 
 void launch_symgs_rows(const Level &L, int64_t begin, int64_t end,
                        const double *r, double *x, hipStream_t s) {
   if (end <= begin) return;
+  const unsigned g = grid_for(end - begin);
   if (L.finest)
-    k_gs_range<true><<<grid_for(end - begin), kBlock, 0, s>>>(
-        begin, end, L.n, L.col, L.val, L.diag, r, x);
+    k_gs_range<true, true><<<g, kBlock, 0, s>>>(begin, end, L.n, L.col, L.val,
+                                                r, x, L.xcd);
+  else if (L.stream)
+    k_gs_range<true, false><<<g, kBlock, 0, s>>>(begin, end, L.n, L.col, L.val,
+                                                 r, x, L.xcd);
   else
-    k_gs_range<false><<<grid_for(end - begin), kBlock, 0, s>>>(
-        begin, end, L.n, L.col, L.val, L.diag, r, x);
+    k_gs_range<false, false><<<g, kBlock, 0, s>>>(begin, end, L.n, L.col,
+                                                  L.val, r, x, L.xcd);
   HPCGX_LAUNCHED();
 }
 
@@ -371,7 +467,7 @@ void launch_symgs_list(const Level &L, const int32_t *rows, int64_t count,
                        const double *r, double *x, hipStream_t s) {
   if (count <= 0) return;
   k_gs_list<<<grid_for(count), kBlock, 0, s>>>(rows, count, L.n, L.col, L.val,
-                                               L.diag, r, x);
+                                               r, x);
   HPCGX_LAUNCHED();
 }
 
@@ -383,12 +479,12 @@ void launch_restriction(const Level &L, const double *rf, const double *Axf,
 
 void launch_residual_restriction(const Level &L, const double *rf,
                                  const double *xf, double *rc, hipStream_t s) {
-  if (L.finest)
+  if (L.stream)
     k_resid_restrict<true><<<grid_for(L.nc), kBlock, 0, s>>>(
-        L.nc, L.n, L.f2c, L.col, L.val, rf, xf, rc);
+        L.nc, L.n, L.f2c, L.col, L.val, rf, xf, rc, L.xcd);
   else
     k_resid_restrict<false><<<grid_for(L.nc), kBlock, 0, s>>>(
-        L.nc, L.n, L.f2c, L.col, L.val, rf, xf, rc);
+        L.nc, L.n, L.f2c, L.col, L.val, rf, xf, rc, L.xcd);
   HPCGX_LAUNCHED();
 }
 
@@ -419,10 +515,18 @@ void launch_dot(Problem &P, int64_t n, const double *x, const double *y,
 
 void launch_spmv_dot(Problem &P, const Level &L, const double *p, double *Ap,
                      hipStream_t s) {
-  const unsigned g = reduce_grid(L.n);
-  const int64_t rpb = (L.n + g - 1) / g;
-  k_spmv_dot<<<g, kBlock, 0, s>>>(L.n, L.col, L.val, p, Ap, P.partials,
-                                  P.ticket, P.scalars, rpb);
+  int64_t g = std::min<int64_t>(L.nchunks, kReduceBlocks);
+  g = std::max<int64_t>(8, (g + 7) / 8 * 8);
+  if (L.stream)
+    k_spmv_dot<true><<<(unsigned)g, kBlock, 0, s>>>(L.nchunks, L.chunks, L.n,
+                                                   L.col, L.val, p, Ap,
+                                                   P.partials, P.ticket,
+                                                   P.scalars);
+  else
+    k_spmv_dot<false><<<(unsigned)g, kBlock, 0, s>>>(L.nchunks, L.chunks, L.n,
+                                                    L.col, L.val, p, Ap,
+                                                    P.partials, P.ticket,
+                                                    P.scalars);
   HPCGX_LAUNCHED();
 }
 
@@ -444,6 +548,65 @@ void launch_residual0(Problem &P, int64_t n, const double *b, const double *Ap,
   k_residual0<<<reduce_grid(n), kBlock, 0, s>>>(n, b, Ap, r, P.partials,
                                                 P.ticket, P.scalars);
   HPCGX_LAUNCHED();
+}
+
+namespace {
+template <bool STREAM, bool FINEST>
+void wave_launch(Problem &P, const Level &L, int dir, const double *r,
+                 double *x, hipStream_t s) {
+  const int32_t *tasks = L.tasks[dir];
+  int64_t ntasks = L.ntasks[dir];
+  const int64_t *tile_ptr = L.tile_ptr;
+  int S = L.nslabs, C = L.ncolors;
+  unsigned *done = L.done;
+  int *err = P.wave_err;
+  int64_t n = L.n;
+  const int32_t *col = L.col;
+  const double *val = L.val;
+  void *args[] = {(void *)&tasks, (void *)&ntasks, (void *)&tile_ptr,
+                  (void *)&S,     (void *)&C,      (void *)&dir,
+                  (void *)&done,  (void *)&err,    (void *)&n,
+                  (void *)&col,   (void *)&val,    (void *)&r,
+                  (void *)&x};
+  const unsigned g = (unsigned)std::min<int64_t>(P.wave_grid, ntasks);
+  HPCGX_HIP(hipLaunchCooperativeKernel(
+      reinterpret_cast<const void *>(&k_gs_wave<STREAM, FINEST>), dim3(g),
+      dim3(kBlock), args, 0, s));
+}
+
+template <bool STREAM, bool FINEST>
+int wave_blocks_per_cu() {
+  int nb = 0;
+  HPCGX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &nb, reinterpret_cast<const void *>(&k_gs_wave<STREAM, FINEST>), kBlock,
+      0));
+  return nb;
+}
+}  // namespace
+
+int wave_grid_size() {
+  int dev = 0;
+  HPCGX_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  HPCGX_HIP(hipGetDeviceProperties(&prop, dev));
+  const int nb = std::min({wave_blocks_per_cu<true, true>(),
+                           wave_blocks_per_cu<true, false>(),
+                           wave_blocks_per_cu<false, false>()});
+  if (nb < 1) throw Error(HPCGX_EHIP, "wavefront kernel cannot be resident");
+  return nb * prop.multiProcessorCount;
+}
+
+void launch_symgs_wave(Problem &P, const Level &L, int dir, const double *r,
+                       double *x, hipStream_t s) {
+  const int64_t nt = (int64_t)L.ncolors * L.nslabs;
+  k_zero_u32<<<grid_for(nt), kBlock, 0, s>>>(L.done, nt);
+  HPCGX_LAUNCHED();
+  if (L.finest)
+    wave_launch<true, true>(P, L, dir, r, x, s);
+  else if (L.stream)
+    wave_launch<true, false>(P, L, dir, r, x, s);
+  else
+    wave_launch<false, false>(P, L, dir, r, x, s);
 }
 
 void launch_replace_diag(Level &L, const double *d, hipStream_t s) {

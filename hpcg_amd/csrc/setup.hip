@@ -24,7 +24,7 @@
 namespace hpcgx {
 namespace {
 
-constexpr int kStepBlocks = 1024;      // fixed grid of the Kahn steps
+constexpr int kStepBlocks = 2048;      // fixed grid of the Kahn steps
 constexpr int kSyncEvery = 128;        // Kahn steps between host checks
 constexpr int64_t kMaxSched = 1 << 22; // bound on schedule levels
 
@@ -107,15 +107,18 @@ __global__ __launch_bounds__(kBlock) void k_count_nnz(int64_t n,
 __global__ __launch_bounds__(kBlock) void k_check(int64_t n, const int32_t *col,
                                                    const uint8_t *nnz_row,
                                                    int32_t *lower_count,
-                                                   int *bad) {
+                                                   int *bad,
+                                                   unsigned long long *bandwidth) {
   const int64_t i = gtid();
   if (i >= n) return;
   const int m = nnz_row[i];
   int lower = 0, has_diag = 0, flags = 0;
+  int64_t bw = 0;
   for (int j = 0; j < m; ++j) {
     const int64_t c = col[(int64_t)j * n + i];
     if (c < 0 || c >= n) { flags |= 1; continue; }
     if (c == i) { has_diag = 1; continue; }
+    bw = max(bw, c > i ? c - i : i - c);
     if (c < i) ++lower;
     const int mc = nnz_row[c];
     int found = 0;
@@ -125,71 +128,94 @@ __global__ __launch_bounds__(kBlock) void k_check(int64_t n, const int32_t *col,
   if (!has_diag) flags |= 2;
   lower_count[i] = lower;
   if (flags) atomicOr(bad, flags);
-}
-
-// Colour of row c: smallest colour absent from its lower neighbours.
-__device__ inline int32_t first_fit(int64_t c, int64_t n, const int32_t *col,
-                                    const uint8_t *nnz_row,
-                                    const int32_t *color) {
-  unsigned mask = 0u;
-  const int m = nnz_row[c];
-  for (int j = 0; j < m; ++j) {
-    const int64_t d = col[(int64_t)j * n + c];
-    if (d < c) mask |= 1u << color[d];
+  unsigned long long w = (unsigned long long)bw;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_down(w, off, 64);
+    w = o > w ? o : w;
   }
-  return (int32_t)(__ffs(~mask) - 1);
+  if ((threadIdx.x & 63) == 0 && w) atomicMax(bandwidth, w);
 }
 
-__global__ __launch_bounds__(kBlock) void k_kahn_init(
-    int64_t n, const int32_t *remaining, int32_t *level, int32_t *color,
-    int32_t *order, unsigned long long *tail) {
+// Append `want` lanes of the calling wavefront to a list with one atomic per
+// wavefront (all active lanes must call it together).  Returns the slot of
+// the lane, or -1.
+__device__ inline int64_t wave_append(bool want, unsigned *tail) {
+  const unsigned long long mask = __ballot(want);
+  if (mask == 0ull) return -1;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)mask) - 1;
+  unsigned base = 0;
+  if (lane == leader) base = atomicAdd(tail, (unsigned)__popcll(mask));
+  base = __shfl(base, leader, 64);
+  const unsigned rank = (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+  return want ? (int64_t)(base + rank) : -1;
+}
+
+// Level 0 of the traversal: rows without lower neighbours.
+__global__ __launch_bounds__(kBlock) void k_kahn_init(int64_t n,
+                                                       const int32_t *remaining,
+                                                       int32_t *order,
+                                                       unsigned *tail) {
   const int64_t i = gtid();
-  if (i < n && remaining[i] == 0) {
-    level[i] = 0;
-    color[i] = 0;
-    order[atomicAdd(tail, 1ull)] = (int32_t)i;
-  }
+  const bool ready = i < n && remaining[i] == 0;
+  const int64_t slot = wave_append(ready, tail);
+  if (ready) order[slot] = (int32_t)i;
 }
 
-// Step t: rows of level t are order[lptr[t] .. lptr[t+1]).  Each one releases
-// its upper neighbours; a neighbour whose last lower neighbour this was gets
-// level t+1 and its first-fit colour and is appended to `order`.
+// Step t: the rows of level t are order[lptr[t] .. lptr[t+1]).  Each row is
+// handled by 32 lanes, one per slot:
+//  * its colour is the smallest one absent from its lower neighbours (all on
+//    lower levels, coloured in earlier steps): the lanes OR their
+//    neighbour's colour bit together;
+//  * it releases its upper neighbours: a neighbour whose last lower
+//    neighbour this was is appended to `order` as a row of level t+1.
+// Unused slots hold the row itself and do neither.
 __global__ __launch_bounds__(kBlock) void k_kahn_step(
     int t, const int64_t *lptr, int64_t n, const int32_t *col,
-    const uint8_t *nnz_row, int32_t *remaining, int32_t *level, int32_t *color,
-    int32_t *order, unsigned long long *tail) {
+    int32_t *remaining, int32_t *level, int32_t *color, int32_t *order,
+    unsigned *tail) {
   const int64_t begin = lptr[t], end = lptr[t + 1];
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t k = begin + gtid(); k < end; k += stride) {
+  const int64_t stride = ((int64_t)gridDim.x * blockDim.x) >> 5;
+  const int j = threadIdx.x & 31;
+  for (int64_t k = begin + (gtid() >> 5); k < end; k += stride) {
     const int64_t u = order[k];
-    const int m = nnz_row[u];
-    for (int j = 0; j < m; ++j) {
-      const int64_t c = col[(int64_t)j * n + u];
-      if (c <= u) continue;
-      if (atomicSub(&remaining[c], 1) == 1) {
-        level[c] = t + 1;
-        color[c] = first_fit(c, n, col, nnz_row, color);
-        order[atomicAdd(tail, 1ull)] = (int32_t)c;
-      }
+    const int64_t c = (j < kSlots) ? col[(int64_t)j * n + u] : u;
+    unsigned bit = (c < u) ? (1u << color[c]) : 0u;
+#pragma unroll
+    for (int off = 16; off > 0; off >>= 1) bit |= __shfl_xor(bit, off, 32);
+    if (j == 0) {
+      level[u] = t;
+      color[u] = __ffs(~bit) - 1;
     }
+    bool ready = false;
+    if (c > u) ready = atomicSub(&remaining[c], 1) == 1;
+    const int64_t slot = wave_append(ready, tail);
+    if (ready) order[slot] = (int32_t)c;
   }
 }
 
-__global__ void k_kahn_advance(int t, int64_t *lptr,
-                               const unsigned long long *tail) {
+__global__ void k_kahn_advance(int t, int64_t *lptr, const unsigned *tail) {
   lptr[t + 2] = (int64_t)*tail;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_i64(int64_t *a, int64_t n,
+                                                      int64_t v) {
+  const int64_t i = gtid();
+  if (i < n) a[i] = v;
+}
+
+// first[key] = first position of key in a sorted key array.
+__global__ __launch_bounds__(kBlock) void k_bounds(int64_t n,
+                                                    const uint32_t *sorted,
+                                                    int64_t *first) {
+  const int64_t i = gtid();
+  if (i < n && (i == 0 || sorted[i - 1] != sorted[i])) first[sorted[i]] = i;
 }
 
 __global__ __launch_bounds__(kBlock) void k_iota(int64_t n, int32_t *a) {
   const int64_t i = gtid();
   if (i < n) a[i] = (int32_t)i;
-}
-
-__global__ __launch_bounds__(kBlock) void k_histogram(int64_t n,
-                                                       const int32_t *key,
-                                                       unsigned *hist) {
-  const int64_t i = gtid();
-  if (i < n) atomicAdd(&hist[key[i]], 1u);
 }
 
 __global__ __launch_bounds__(kBlock) void k_invert(int64_t n,
@@ -258,49 +284,105 @@ __global__ __launch_bounds__(kBlock) void k_vectors(int64_t n,
   if (xexact) xexact[k] = 1.0;
 }
 
-// Stable sort of row ids by a small integer key: returns rows ordered by
-// (key, row).
-void sort_rows_by_key(int64_t n, const int32_t *key, int end_bit,
-                      int32_t *rows_out, hipStream_t s) {
-  HPCGX_REQUIRE(n < (int64_t)1 << 31, "row count exceeds 2^31");
-  TempBuf ids(sizeof(int32_t) * n), keys_out(sizeof(int32_t) * n);
-  k_iota<<<grid_for(n), kBlock, 0, s>>>(n, ids.as<int32_t>());
-  HPCGX_LAUNCHED();
-  // keys are non-negative: sort them as unsigned
-  const uint32_t *ukey = reinterpret_cast<const uint32_t *>(key);
-  size_t temp_bytes = 0;
-  HPCGX_HIP(hipcub::DeviceRadixSort::SortPairs(
-      nullptr, temp_bytes, ukey, keys_out.as<uint32_t>(), ids.as<int32_t>(),
-      rows_out, (int)n, 0, end_bit, s));
-  TempBuf temp(temp_bytes);
-  HPCGX_HIP(hipcub::DeviceRadixSort::SortPairs(
-      temp.p, temp_bytes, ukey, keys_out.as<uint32_t>(), ids.as<int32_t>(),
-      rows_out, (int)n, 0, end_bit, s));
-  HPCGX_HIP(hipStreamSynchronize(s));
-}
-
 int bits_for(int64_t v) {
   int b = 1;
   while (((int64_t)1 << b) <= v) ++b;
   return b;
 }
 
-std::vector<int64_t> histogram_ptr(int64_t n, const int32_t *key, int nbins,
-                                   hipStream_t s) {
-  TempBuf hist(sizeof(unsigned) * nbins);
-  HPCGX_HIP(hipMemsetAsync(hist.p, 0, sizeof(unsigned) * nbins, s));
-  k_histogram<<<grid_for(n), kBlock, 0, s>>>(n, key, hist.as<unsigned>());
+// Stable sort of row ids by a small non-negative integer key (< nbins):
+// rows_out = rows ordered by (key, row); returns the bin pointers
+// ptr[0..nbins], bin k owning rows_out[ptr[k] .. ptr[k+1]).
+std::vector<int64_t> sort_rows_by_key(int64_t n, const int32_t *key,
+                                      int nbins, int32_t *rows_out,
+                                      hipStream_t s) {
+  HPCGX_REQUIRE(n < (int64_t)1 << 31, "row count exceeds 2^31");
+  TempBuf ids(sizeof(int32_t) * n), keys_out(sizeof(uint32_t) * n);
+  k_iota<<<grid_for(n), kBlock, 0, s>>>(n, ids.as<int32_t>());
   HPCGX_LAUNCHED();
-  std::vector<unsigned> h(nbins);
-  HPCGX_HIP(hipMemcpyAsync(h.data(), hist.p, sizeof(unsigned) * nbins,
+  const uint32_t *ukey = reinterpret_cast<const uint32_t *>(key);
+  const int end_bit = bits_for(nbins);
+  size_t temp_bytes = 0;
+  HPCGX_HIP(hipcub::DeviceRadixSort::SortPairs(
+      nullptr, temp_bytes, ukey, keys_out.as<uint32_t>(), ids.as<int32_t>(),
+      rows_out, (int)n, 0, end_bit, s));
+  {
+    TempBuf temp(temp_bytes);
+    HPCGX_HIP(hipcub::DeviceRadixSort::SortPairs(
+        temp.p, temp_bytes, ukey, keys_out.as<uint32_t>(), ids.as<int32_t>(),
+        rows_out, (int)n, 0, end_bit, s));
+    HPCGX_HIP(hipStreamSynchronize(s));
+  }
+  // bin starts from the sorted keys; empty bins start where the next begins
+  TempBuf first(sizeof(int64_t) * (nbins + 1));
+  k_fill_i64<<<grid_for(nbins + 1), kBlock, 0, s>>>(first.as<int64_t>(),
+                                                    nbins + 1, n);
+  HPCGX_LAUNCHED();
+  k_bounds<<<grid_for(n), kBlock, 0, s>>>(n, keys_out.as<uint32_t>(),
+                                          first.as<int64_t>());
+  HPCGX_LAUNCHED();
+  std::vector<int64_t> ptr(nbins + 1);
+  HPCGX_HIP(hipMemcpyAsync(ptr.data(), first.p, sizeof(int64_t) * (nbins + 1),
                            hipMemcpyDeviceToHost, s));
   HPCGX_HIP(hipStreamSynchronize(s));
-  std::vector<int64_t> ptr(nbins + 1, 0);
-  for (int b = 0; b < nbins; ++b) ptr[b + 1] = ptr[b] + h[b];
+  ptr[nbins] = n;
+  for (int b = nbins - 1; b >= 0; --b) ptr[b] = std::min(ptr[b], ptr[b + 1]);
   return ptr;
 }
 
+// tile_ptr[c*(S+1) + s] = first device row of colour c whose natural row is
+// >= s*W (binary search: a colour block lists its natural rows in order).
+__global__ __launch_bounds__(kBlock) void k_tiles(int C, int S, int64_t W,
+                                                   const int64_t *color_ptr,
+                                                   const int32_t *iperm,
+                                                   int64_t *tile_ptr) {
+  const int64_t k = gtid();
+  if (k >= (int64_t)C * (S + 1)) return;
+  const int c = (int)(k / (S + 1)), sl = (int)(k % (S + 1));
+  int64_t lo = color_ptr[c], hi = color_ptr[c + 1];
+  const int64_t key = (int64_t)sl * W;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) / 2;
+    if ((int64_t)iperm[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  tile_ptr[k] = lo;
+}
+
+// Row chunks of at most kBlock rows, ordered by their relative position in
+// their colour block and then by colour: chunk k of every colour comes
+// before chunk k+1 of any colour.  A sweep over this list reads each colour
+// block's x values at about the same relative position at the same time.
+void build_chunks(Problem &P, Level &L) {
+  struct Chunk {
+    double key;
+    int colour;
+    int64_t begin, end;
+  };
+  std::vector<Chunk> v;
+  for (int c = 0; c < L.ncolors; ++c) {
+    const int64_t b0 = L.color_ptr[c], e0 = L.color_ptr[c + 1];
+    const int64_t nch = (e0 - b0 + kBlock - 1) / kBlock;
+    for (int64_t k = 0; k < nch; ++k)
+      v.push_back({(k + 0.5) / (double)nch, c, b0 + k * kBlock,
+                   std::min<int64_t>(b0 + (k + 1) * kBlock, e0)});
+  }
+  std::stable_sort(v.begin(), v.end(), [](const Chunk &a, const Chunk &b) {
+    return a.key < b.key || (a.key == b.key && a.colour < b.colour);
+  });
+  std::vector<int64_t> h(2 * v.size());
+  for (size_t k = 0; k < v.size(); ++k) {
+    h[2 * k] = v[k].begin;
+    h[2 * k + 1] = v[k].end;
+  }
+  L.nchunks = (int64_t)v.size();
+  L.chunks = P.arena.alloc<int64_t>(h.size());
+  HPCGX_HIP(hipMemcpy(L.chunks, h.data(), sizeof(int64_t) * h.size(),
+                      hipMemcpyHostToDevice));
+}
+
 }  // namespace
+
+void build_wave(Problem &P, Level &L, int slab_factor, hipStream_t s);
 
 void generate_natural(NaturalLevel &N, hipStream_t s) {
   const int64_t n = N.n;
@@ -338,14 +420,19 @@ void finalize_level(Problem &P, int l, NaturalLevel &N, int flags,
   TempBuf remaining(sizeof(int32_t) * n), level(sizeof(int32_t) * n),
       color(sizeof(int32_t) * n), order(sizeof(int32_t) * n);
   {
-    TempBuf bad(sizeof(int));
+    TempBuf bad(sizeof(int)), bw(sizeof(unsigned long long));
     HPCGX_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+    HPCGX_HIP(hipMemsetAsync(bw.p, 0, sizeof(unsigned long long), s));
     k_check<<<grid_for(n), kBlock, 0, s>>>(n, N.col, N.nnz_row,
-                                           remaining.as<int32_t>(), bad.as<int>());
+                                           remaining.as<int32_t>(), bad.as<int>(),
+                                           bw.as<unsigned long long>());
     HPCGX_LAUNCHED();
     int b = 0;
+    unsigned long long bwv = 0;
     HPCGX_HIP(hipMemcpyAsync(&b, bad.p, sizeof b, hipMemcpyDeviceToHost, s));
+    HPCGX_HIP(hipMemcpyAsync(&bwv, bw.p, sizeof bwv, hipMemcpyDeviceToHost, s));
     HPCGX_HIP(hipStreamSynchronize(s));
+    L.bandwidth = (int64_t)bwv;
     if (b & 1) throw Error(HPCGX_EINVAL, "column index out of range");
     if (b & 2) throw Error(HPCGX_EUNSUP, "a row has no diagonal entry");
     if (b & 4) throw Error(HPCGX_EUNSUP, "sparsity pattern is not symmetric");
@@ -355,28 +442,25 @@ void finalize_level(Problem &P, int l, NaturalLevel &N, int flags,
   int nsched = 0;
   {
     const int64_t cap = std::min<int64_t>(n, kMaxSched) + 2;
-    TempBuf lptr(sizeof(int64_t) * cap), tail(sizeof(unsigned long long));
+    TempBuf lptr(sizeof(int64_t) * cap), tail(sizeof(unsigned));
     HPCGX_HIP(hipMemsetAsync(lptr.p, 0, sizeof(int64_t) * cap, s));
-    HPCGX_HIP(hipMemsetAsync(tail.p, 0, sizeof(unsigned long long), s));
-    k_kahn_init<<<grid_for(n), kBlock, 0, s>>>(
-        n, remaining.as<int32_t>(), level.as<int32_t>(), color.as<int32_t>(),
-        order.as<int32_t>(), tail.as<unsigned long long>());
+    HPCGX_HIP(hipMemsetAsync(tail.p, 0, sizeof(unsigned), s));
+    k_kahn_init<<<grid_for(n), kBlock, 0, s>>>(n, remaining.as<int32_t>(),
+                                               order.as<int32_t>(), tail.as<unsigned>());
     HPCGX_LAUNCHED();
     // lptr[1] = rows of level 0
-    k_kahn_advance<<<1, 1, 0, s>>>(-1, lptr.as<int64_t>(),
-                                   tail.as<unsigned long long>());
+    k_kahn_advance<<<1, 1, 0, s>>>(-1, lptr.as<int64_t>(), tail.as<unsigned>());
     HPCGX_LAUNCHED();
     int t = 0;
     for (;;) {
       const int stop = (int)std::min<int64_t>(t + kSyncEvery, cap - 2);
       for (; t < stop; ++t) {
         k_kahn_step<<<kStepBlocks, kBlock, 0, s>>>(
-            t, lptr.as<int64_t>(), n, N.col, N.nnz_row, remaining.as<int32_t>(),
+            t, lptr.as<int64_t>(), n, N.col, remaining.as<int32_t>(),
             level.as<int32_t>(), color.as<int32_t>(), order.as<int32_t>(),
-            tail.as<unsigned long long>());
+            tail.as<unsigned>());
         HPCGX_LAUNCHED();
-        k_kahn_advance<<<1, 1, 0, s>>>(t, lptr.as<int64_t>(),
-                                       tail.as<unsigned long long>());
+        k_kahn_advance<<<1, 1, 0, s>>>(t, lptr.as<int64_t>(), tail.as<unsigned>());
         HPCGX_LAUNCHED();
       }
       // after step t-1, lptr[t+1] holds the end of level t
@@ -398,29 +482,27 @@ void finalize_level(Problem &P, int l, NaturalLevel &N, int flags,
   }
   L.nsched = nsched;
 
-  // colour count
-  int ncolors = 0;
-  {
-    std::vector<int32_t> dummy;
-    // max colour via a histogram over kMaxColors bins
-    L.color_ptr = histogram_ptr(n, color.as<int32_t>(), kMaxColors, s);
-    for (int c = 0; c < kMaxColors; ++c)
-      if (L.color_ptr[c + 1] > L.color_ptr[c]) ncolors = c + 1;
-    L.color_ptr.resize(ncolors + 1);
-  }
-  L.ncolors = ncolors;
-
   // device order = stable sort of rows by colour
   L.iperm = P.arena.alloc<int32_t>(n);
   L.perm = P.arena.alloc<int32_t>(n);
-  sort_rows_by_key(n, color.as<int32_t>(), bits_for(ncolors), L.iperm, s);
+  {
+    std::vector<int64_t> cp = sort_rows_by_key(n, color.as<int32_t>(), kMaxColors, L.iperm, s);
+    int ncolors = 0;
+    for (int c = 0; c < kMaxColors; ++c)
+      if (cp[c + 1] > cp[c]) ncolors = c + 1;
+    cp.resize(ncolors + 1);
+    L.color_ptr = cp;
+    L.ncolors = ncolors;
+  }
   k_invert<<<grid_for(n), kBlock, 0, s>>>(n, L.iperm, L.perm);
+  HPCGX_LAUNCHED();
+  build_chunks(P, L);
+  L.stream = (double)n * kSlots * (sizeof(double) + sizeof(int32_t)) > kStreamBytes;
   HPCGX_LAUNCHED();
 
   // lexicographic schedule: rows sorted by (level, row), renumbered
-  L.level_ptr = histogram_ptr(n, level.as<int32_t>(), nsched, s);
   L.level_rows = P.arena.alloc<int32_t>(n);
-  sort_rows_by_key(n, level.as<int32_t>(), bits_for(nsched), order.as<int32_t>(), s);
+  L.level_ptr = sort_rows_by_key(n, level.as<int32_t>(), nsched, order.as<int32_t>(), s);
   k_map<<<grid_for(n), kBlock, 0, s>>>(n, L.perm, order.as<int32_t>(), L.level_rows);
   HPCGX_LAUNCHED();
 
@@ -444,6 +526,73 @@ void finalize_level(Problem &P, int l, NaturalLevel &N, int flags,
                                                L.dslot, L.nnz_row);
   HPCGX_LAUNCHED();
   HPCGX_HIP(hipStreamSynchronize(s));
+  if (n >= kWaveMinRows) build_wave(P, L, kWaveSlabFactor, s);
+}
+
+// Wavefront schedule of the multicolour sweeps.  Natural rows are cut into
+// slabs of W >= bandwidth rows, so a row of slab s only couples to slabs
+// s-1..s+1.  Tile (c, s) = device rows of colour c in slab s: contiguous,
+// because a colour block keeps natural order.  Tasks (chunks of kBlock rows
+// of one tile) are listed by t = s + 2*phase, then phase: a task of phase p
+// at slab s needs only the phase p-1 tiles at slabs s-1..s+1 (everything it
+// must see updated, or must read before it is updated, follows from those),
+// and each of them comes earlier in the list.
+void drop_wave(Problem &P, Level &L) {
+  const int64_t C = L.ncolors, S = L.nslabs;
+  if (L.tile_ptr) P.arena.release(L.tile_ptr, sizeof(int64_t) * C * (S + 1));
+  for (int d = 0; d < 2; ++d)
+    if (L.tasks[d]) P.arena.release(L.tasks[d], sizeof(int32_t) * 2 * L.ntasks[d]);
+  if (L.done) P.arena.release(L.done, sizeof(unsigned) * C * S);
+  L.tile_ptr = nullptr;
+  L.tasks[0] = L.tasks[1] = nullptr;
+  L.ntasks[0] = L.ntasks[1] = 0;
+  L.done = nullptr;
+  L.nslabs = 0;
+  L.wave = false;
+}
+
+void build_wave(Problem &P, Level &L, int slab_factor, hipStream_t s) {
+  drop_wave(P, L);
+  const int C = L.ncolors;
+  const int64_t n = L.n;
+  const int64_t W = std::max<int64_t>(1, L.bandwidth) * std::max(1, slab_factor);
+  const int S = (int)((n + W - 1) / W);
+  L.wave_slab = W;
+  L.nslabs = S;
+  const int64_t ntp = (int64_t)C * (S + 1);
+  L.tile_ptr = P.arena.alloc<int64_t>(ntp);
+  TempBuf cp(sizeof(int64_t) * (C + 1));
+  HPCGX_HIP(hipMemcpyAsync(cp.p, L.color_ptr.data(), sizeof(int64_t) * (C + 1),
+                           hipMemcpyHostToDevice, s));
+  k_tiles<<<grid_for(ntp), kBlock, 0, s>>>(C, S, W, cp.as<int64_t>(), L.iperm,
+                                           L.tile_ptr);
+  HPCGX_LAUNCHED();
+  std::vector<int64_t> tp(ntp);
+  HPCGX_HIP(hipMemcpyAsync(tp.data(), L.tile_ptr, sizeof(int64_t) * ntp,
+                           hipMemcpyDeviceToHost, s));
+  HPCGX_HIP(hipStreamSynchronize(s));
+  for (int dir = 0; dir < 2; ++dir) {
+    std::vector<int32_t> tasks;
+    for (int t = 0; t < S + 2 * (C - 1); ++t)
+      for (int p = 0; p < C; ++p) {
+        const int sl = t - 2 * p;
+        if (sl < 0 || sl >= S) continue;
+        const int c = dir == 0 ? p : C - 1 - p;
+        const int64_t rows = tp[(int64_t)c * (S + 1) + sl + 1] - tp[(int64_t)c * (S + 1) + sl];
+        const int64_t nch = (rows + kBlock - 1) / kBlock;
+        for (int64_t k = 0; k < nch; ++k) {
+          tasks.push_back(p * S + sl);
+          tasks.push_back((int32_t)k);
+        }
+      }
+    L.ntasks[dir] = (int64_t)tasks.size() / 2;
+    L.tasks[dir] = P.arena.alloc<int32_t>(tasks.size());
+    HPCGX_HIP(hipMemcpy(L.tasks[dir], tasks.data(), sizeof(int32_t) * tasks.size(),
+                        hipMemcpyHostToDevice));
+  }
+  L.done = P.arena.alloc<unsigned>((size_t)C * S);
+  HPCGX_HIP(hipMemset(L.done, 0, sizeof(unsigned) * C * S));
+  L.wave = true;
 }
 
 void link_levels(Problem &P, int l, const NaturalLevel &Nf, hipStream_t s) {

@@ -15,12 +15,15 @@ namespace hpcgx {
 //    reference's sequential sweep, row for row;
 //  * multicolour ordering: colours 0..C-1, then C-1..0.
 namespace {
-// One colour step; on the finest level optionally bracketed by HIP events
-// (hpcgx_kernel_timing) so its launches can be timed on their own stream.
-void colour_step(Problem &P, const Level &L, int c, const double *r, double *x,
-                 hipStream_t s) {
-  const bool log = L.finest && P.klog.on;
-  if (log) {
+// Event pair around a launch on the finest level (hpcgx_kernel_timing), so
+// the dominant kernel's launches can be timed on their own stream.
+struct LaunchLog {
+  Problem &P;
+  hipStream_t s;
+  bool on;
+  LaunchLog(Problem &P_, const Level &L, hipStream_t s_)
+      : P(P_), s(s_), on(L.finest && P_.klog.on) {
+    if (!on) return;
     auto &K = P.klog;
     if (K.used + 2 > K.ev.size()) {
       for (int k = 0; k < 64; ++k) {
@@ -31,18 +34,37 @@ void colour_step(Problem &P, const Level &L, int c, const double *r, double *x,
     }
     HPCGX_HIP(hipEventRecord(K.ev[K.used], s));
   }
-  launch_symgs_rows(L, L.color_ptr[c], L.color_ptr[c + 1], r, x, s);
-  if (log) {
+  void done() {
+    if (!on) return;
     HPCGX_HIP(hipEventRecord(P.klog.ev[P.klog.used + 1], s));
     P.klog.used += 2;
   }
+};
+
+// One colour step (one launch per colour).
+void colour_step(Problem &P, const Level &L, int c, const double *r, double *x,
+                 hipStream_t s) {
+  LaunchLog log(P, L, s);
+  launch_symgs_rows(L, L.color_ptr[c], L.color_ptr[c + 1], r, x, s);
+  log.done();
+}
+
+// A whole sweep direction as one wavefront launch.
+void wave_sweep(Problem &P, const Level &L, int dir, const double *r, double *x,
+                hipStream_t s) {
+  LaunchLog log(P, L, s);
+  launch_symgs_wave(P, L, dir, r, x, s);
+  log.done();
 }
 }  // namespace
 
 void symgs(Problem &P, int l, int ordering, const double *r, double *x,
            hipStream_t s) {
   const Level &L = P.L[l];
-  if (ordering == HPCGX_ORDER_MULTICOLOR) {
+  if (ordering == HPCGX_ORDER_MULTICOLOR && L.wave && P.use_wave) {
+    wave_sweep(P, L, 0, r, x, s);
+    wave_sweep(P, L, 1, r, x, s);
+  } else if (ordering == HPCGX_ORDER_MULTICOLOR) {
     for (int c = 0; c < L.ncolors; ++c) colour_step(P, L, c, r, x, s);
     for (int c = L.ncolors - 1; c >= 0; --c) colour_step(P, L, c, r, x, s);
   } else {
@@ -109,6 +131,16 @@ struct PhaseTimer {  // HIP-event timing of CG phases (times[1..5])
 };
 }  // namespace
 
+void check_status(Problem &P) {
+  if (!P.wave_err) return;
+  int e = 0;
+  HPCGX_HIP(hipMemcpy(&e, P.wave_err, sizeof e, hipMemcpyDeviceToHost));
+  if (e) {
+    HPCGX_HIP(hipMemset(P.wave_err, 0, sizeof(int)));
+    throw Error(HPCGX_EHIP, "wavefront sweep: a dependency wait timed out");
+  }
+}
+
 // CG.cpp:59-143.  With the reductions fused into their producers:
 //   r.z            -> dot kernel that also forms beta = rtz/oldrtz (:112-114)
 //   Ap = A p, p.Ap -> one SpMV kernel that also forms alpha (:118-120)
@@ -170,6 +202,7 @@ void cg(Problem &P, int ordering, const double *b, double *x, int max_iter,
     times[3] += t3;
     times[5] += t5;
   }
+  check_status(P);
   if (!std::isfinite(nr))
     throw Error(HPCGX_EINVAL, "CG residual is not finite");
 }

@@ -124,7 +124,8 @@ class Problem:
         i = LevelInfo()
         call("hpcgx_problem_info", self._h, level, ctypes.byref(i))
         return dict(nx=i.nx, ny=i.ny, nz=i.nz, nrow=i.nrow, nnz=i.nnz, ncolors=i.ncolors,
-                    nlevels_sched=i.nlevels_sched, ncoarse=i.ncoarse)
+                    nlevels_sched=i.nlevels_sched, ncoarse=i.ncoarse, wave=bool(i.wave),
+                    nslabs=i.nslabs)
 
     def nrow(self, level=0):
         return self.levels[level]["nrow"]
@@ -239,6 +240,15 @@ class Problem:
         out = self.empty(level)
         call("hpcgx_copy_diagonal", self._h, level, _ptr(out), _stream())
         return out
+
+    def tune(self, key, value):
+        """Set a tuning knob ("wave", "wave_min_rows", "wave_slab_factor",
+        "xcd_remap", "stream_min_bytes"); see include/hpcg_amd.h."""
+        return call("hpcgx_tune", self._h, key.encode(), float(value))
+
+    def check(self):
+        """Wait for the stream; raise if a wavefront sweep gave up waiting."""
+        return call("hpcgx_check_status", self._h, _stream())
 
     def kernel_timing(self, enable=True):
         """Start (clearing the log) or stop the event log of the finest

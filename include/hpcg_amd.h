@@ -68,6 +68,8 @@ typedef struct {
   int ncolors;           /* colours of the multicolour schedule */
   int nlevels_sched;     /* hyperplanes of the lexicographic schedule */
   int64_t ncoarse;       /* rows of the next level (0 on coarsest) */
+  int wave;              /* 1: multicolour sweeps run as a wavefront */
+  int nslabs;            /* slabs of the wavefront schedule */
 } hpcgx_level_info;
 
 const char *hpcgx_last_error(void);
@@ -197,6 +199,25 @@ int hpcgx_cg(hpcgx_problem *p, int ordering, const double *b, double *x,
              int max_iter, double tolerance, int precondition, int flags,
              int *niters, double *normr, double *normr0, double *times,
              void *stream);
+
+/* ---- tuning ----------------------------------------------------------------- */
+
+/* No counterpart in the reference (tuning knobs of this implementation):
+ *   "wave"             1/0: multicolour sweeps of large levels as one
+ *                      wavefront launch per direction (default 1)
+ *   "wave_min_rows"    levels with at least this many rows get a wavefront
+ *                      schedule (default 2^23)
+ *   "wave_slab_factor" slab width in units of the matrix bandwidth (>= 1)
+ *   "xcd_remap"        1/0: map workgroups to XCD-contiguous row ranges
+ *   "stream_min_bytes" matrices larger than this are read with non-temporal
+ *                      loads (default 512e6)
+ * Synchronous.  Unknown keys return HPCGX_EINVAL. */
+int hpcgx_tune(hpcgx_problem *p, const char *key, double value);
+
+/* No counterpart in the reference: waits for `stream` and reports a
+ * wavefront sweep whose dependency wait gave up (HPCGX_EHIP).  hpcgx_cg
+ * performs the same check before returning. */
+int hpcgx_check_status(hpcgx_problem *p, void *stream);
 
 /* ---- measurement ----------------------------------------------------------- */
 

@@ -80,7 +80,9 @@ def test_schedules_match_oracle(hp, dims):
         assert info["nrow"] == h.nrow(l) and info["nnz"] == h.nnz(l)
         lev, nl = O.levels(m)
         col, nc = O.colors(m)
-        assert info["nlevels_sched"] == nl and info["ncolors"] == nc == 8
+        assert info["nlevels_sched"] == nl and info["ncolors"] == nc
+        if min(h.dims(l)) >= 2:
+            assert nc == 8  # the parity classes
         assert np.array_equal(P.schedule(l, "level"), lev)
         assert np.array_equal(P.schedule(l, "color"), col)
 
@@ -223,6 +225,35 @@ def test_mg_golden(hp, g16, g81624):
         assert np.array_equal(P.to_numpy(z), g["mg_u1"])
         P.mg(P.to_device_order(g["b"]), z, ordering="lexicographic")
         assert np.array_equal(P.to_numpy(z), g["mg_b"])
+
+
+@pytest.mark.parametrize("dims", SMALL + [(64, 64, 64)])
+@pytest.mark.parametrize("slab_factor", [1, 3])
+def test_wavefront_bitexact(hp, dims, slab_factor):
+    """Multicolour sweeps run as one dependency-tracked wavefront launch per
+    direction (forced onto every level) give the colour-ordered Gauss-Seidel
+    result bit for bit."""
+    say("wavefront", dims, slab_factor)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    P.tune("wave_min_rows", 1)
+    P.tune("wave_slab_factor", slab_factor)
+    assert all(P.info(l)["wave"] for l in range(4))
+    h = hier(dims, color=True)
+    for l in (0, 1, 3):
+        m = h.level(l)
+        n = h.nrow(l)
+        r, x0 = rnd(110 + l, n), rnd(120 + l, n)
+        want = O.symgs(m, r, x0, O.color_order(O.colors(m)[0]))
+        x = P.to_device_order(x0, l)
+        P.symgs(P.to_device_order(r, l), x, level=l, ordering="multicolor")
+        P.check()
+        assert np.array_equal(P.to_numpy(x, l), want), l
+    r = rnd(130, h.nrow(0))
+    z = P.zeros()
+    P.mg(P.to_device_order(r), z, ordering="multicolor")
+    P.check()
+    assert np.array_equal(P.to_numpy(z), h.mg(r))
+    del P
 
 
 def test_waxpby_bitexact(hp, torch):
@@ -391,7 +422,24 @@ def test_256_properties(hp, torch):
         xx = xe.clone()
         P.symgs(b, xx, ordering=ordering)
         assert float((xx - 1.0).abs().max()) <= 1e-14
+    # the 8-colour smoother converges more slowly than the lexicographic one
+    # (oracle, 64^3, 50 iterations: 3.9e-10 vs 1.1e-11); at 256^3 the
+    # residual after 50 iterations is about 7e-3
+    assert P.info(0)["wave"]
+    say("wavefront vs colour-step kernels", dims)
+    rr = P.to_device_order(rnd(140, P.nrow(0)))
+    xw = P.to_device_order(rnd(141, P.nrow(0)))
+    xk = xw.clone()
+    P.symgs(rr, xw, ordering="multicolor")
+    P.check()
+    P.tune("wave", 0)
+    P.symgs(rr, xk, ordering="multicolor")
+    P.tune("wave", 1)
+    assert torch.equal(xw, xk)
     say("cg 256^3")
+    r25 = P.cg(b, x, max_iter=25, tolerance=0.0, ordering="multicolor")
+    x.zero_()
     r = P.cg(b, x, max_iter=50, tolerance=0.0, ordering="multicolor")
-    assert r["niters"] == 50 and r["normr"] / r["normr0"] < 1e-4
+    assert r["niters"] == 50 and r["normr"] / r["normr0"] < 2e-2
+    assert r["normr"] < 0.5 * r25["normr"]
     del P
