@@ -387,6 +387,11 @@ int hpcgx_tune(hpcgx_problem *p, const char *key, double value) {
     hpcgx::Problem &P = p->P;
     if (k == "wave") {
       P.use_wave = value != 0.0;
+      if (P.use_wave)
+        for (int l = 0; l < P.nlevels; ++l)
+          if (P.L[l].n >= hpcgx::kWaveMinRows && !P.L[l].wave)
+            hpcgx::build_wave(P, P.L[l], hpcgx::kWaveSlabFactor, hpcgx::kWaveRows, nullptr);
+      HPCGX_HIP(hipDeviceSynchronize());
       return;
     }
     for (int l = 0; l < P.nlevels; ++l) {
@@ -397,13 +402,19 @@ int hpcgx_tune(hpcgx_problem *p, const char *key, double value) {
         L.stream = (double)L.n * kSlots * 12.0 > value;
       } else if (k == "wave_min_rows") {
         if ((double)L.n >= value) {
-          if (!L.wave) hpcgx::build_wave(P, L, hpcgx::kWaveSlabFactor, nullptr);
+          if (!L.wave) hpcgx::build_wave(P, L, hpcgx::kWaveSlabFactor, hpcgx::kWaveRows, nullptr);
         } else {
           hpcgx::drop_wave(P, L);
         }
       } else if (k == "wave_slab_factor") {
         if (value < 1) throw Error(HPCGX_EINVAL, "wave_slab_factor must be >= 1");
-        if (L.wave) hpcgx::build_wave(P, L, (int)value, nullptr);
+        if (L.wave)
+          hpcgx::build_wave(P, L, (int)value, L.wave_chunk / hpcgx::kBlock, nullptr);
+      } else if (k == "wave_rows") {
+        if (value < 1 || value > 16) throw Error(HPCGX_EINVAL, "wave_rows must be in [1, 16]");
+        if (L.wave)
+          hpcgx::build_wave(P, L, (int)(L.wave_slab / std::max<int64_t>(1, L.bandwidth)),
+                            (int)value, nullptr);
       } else {
         throw Error(HPCGX_EINVAL, "unknown tuning key " + k);
       }

@@ -21,6 +21,7 @@ constexpr int kReduceBlocks = 1024;  // partial sums of a grid reduction
 constexpr double kStreamBytes = 512e6;  // matrices above this are streamed
 constexpr int64_t kWaveMinRows = 1 << 23;  // levels swept as a wavefront
 constexpr int kWaveSlabFactor = 2;         // slab width / matrix bandwidth
+constexpr int kWaveRows = 2;               // rows per thread of a wavefront task
 
 struct Error : std::runtime_error {
   int code;
@@ -104,6 +105,7 @@ struct Level {
   int64_t bandwidth = 0;  // max |col - row| in natural order
   bool wave = false;
   int64_t wave_slab = 0;
+  int wave_chunk = kBlock;  // rows per task
   int nslabs = 0;
   int64_t *tile_ptr = nullptr;
   int64_t ntasks[2] = {0, 0};       // forward, backward
@@ -131,7 +133,7 @@ struct Problem {
   double *scalars = nullptr;  // see CgScalar
   int *wave_err = nullptr;    // set when a wavefront wait gives up
   int wave_grid = 0;          // co-resident workgroups of the wavefront kernel
-  bool use_wave = true;       // multicolour sweeps as wavefronts where built
+  bool use_wave = false;      // multicolour sweeps as wavefronts (measured slower)
   // event log of the finest level's multicolour Gauss-Seidel launches
   struct KernelLog {
     bool on = false;
@@ -195,7 +197,7 @@ void generate_natural(NaturalLevel &N, hipStream_t s);
 void finalize_level(Problem &P, int l, NaturalLevel &N, int flags,
                     hipStream_t s);
 void link_levels(Problem &P, int l, const NaturalLevel &Nf, hipStream_t s);
-void build_wave(Problem &P, Level &L, int slab_factor, hipStream_t s);
+void build_wave(Problem &P, Level &L, int slab_factor, int rows, hipStream_t s);
 void drop_wave(Problem &P, Level &L);
 void problem_vectors(const Problem &P, double *b, double *x, double *xexact,
                      hipStream_t s);

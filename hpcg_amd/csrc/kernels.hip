@@ -166,15 +166,59 @@ __global__ __launch_bounds__(kBlock) void k_gs_range(
 // earliest unfinished task always has its inputs done: the waits end.  A wait
 // that exceeds kSpinLimit polls (seconds) sets *err and gives up, as do all
 // later ones; the sweep then finishes with wrong values instead of hanging.
-// Cross-workgroup visibility of x: every thread fences (agent scope) after
-// its update, thread 0 publishes with a release add; the waiting thread 0
-// acquires, every thread fences before reading.
+//
+// Visibility of x across workgroups (and XCDs) without cache maintenance:
+// inside this kernel x is only accessed with device-scope (agent) atomic
+// loads and stores, which are served at the device-coherent level (the
+// Infinity Cache / HBM) rather than by the per-XCD L2.  Every wave waits for
+// its stores to complete (s_waitcnt 0) before the barrier after which thread
+// 0 bumps the tile's counter, and readers issue their x loads only after
+// thread 0 has seen the counter.  Agent-scope fences are not used: each one
+// writes back / invalidates the XCD's whole L2.
 constexpr unsigned kSpinLimit = 1u << 22;
 
-template <bool STREAM, bool FINEST>
+__device__ inline double xload(const double *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void xstore(double *p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// gs_row inside the wavefront: x of rows already updated in this launch
+// (colours of earlier phases: device rows below `lo` in the forward sweep,
+// at or above `hi` in the backward sweep, [lo, hi) being this row's colour
+// block) is read device-coherently; every other x value cannot have changed
+// since the launch began and is read through the caches.  The new x is
+// stored device-coherently.
+template <bool STREAM, int DIR>
+__device__ inline void gs_row_wave(int64_t i, int64_t n, int64_t lo, int64_t hi,
+                                   const int32_t *__restrict__ col,
+                                   const double *__restrict__ val,
+                                   const double *__restrict__ r, double *x) {
+  double sum = r[i];
+  double d = 0.0;
+  bool have = false;
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int64_t k = (int64_t)j * n + i;
+    const double v = mload<STREAM>(val + k);
+    const int32_t c = mload<STREAM>(col + k);
+    const bool fresh = DIR == 0 ? c < lo : c >= hi;
+    const double xc = fresh ? xload(x + c) : x[c];
+    sum -= v * xc;
+    if (!have && c == i) {
+      d = v;
+      have = true;
+    }
+  }
+  sum += x[i] * d;
+  xstore(x + i, sum / d);
+}
+
+template <bool STREAM, bool FINEST, int DIR>
 __global__ __launch_bounds__(kBlock) void k_gs_wave(
     const int32_t *__restrict__ tasks, int64_t ntasks,
-    const int64_t *__restrict__ tile_ptr, int S, int C, int dir,
+    const int64_t *__restrict__ tile_ptr, int S, int C, int chunk_rows,
     unsigned *done, int *err, int64_t n, const int32_t *__restrict__ col,
     const double *__restrict__ val, const double *__restrict__ r, double *x) {
   for (int64_t k = blockIdx.x; k < ntasks; k += gridDim.x) {
@@ -182,15 +226,16 @@ __global__ __launch_bounds__(kBlock) void k_gs_wave(
     const int chunk = tasks[2 * k + 1];
     const int p = tile / S, sl = tile % S;
     if (threadIdx.x == 0 && p > 0) {
-      const int cq = dir == 0 ? p - 1 : C - p;  // colour of phase p-1
+      const int cq = DIR == 0 ? p - 1 : C - p;  // colour of phase p-1
       const int64_t *tq = tile_ptr + (int64_t)cq * (S + 1);
       const int lo = sl > 0 ? sl - 1 : 0, hi = sl + 1 < S ? sl + 1 : S - 1;
       for (int s2 = lo; s2 <= hi; ++s2) {
-        const unsigned need = (unsigned)((tq[s2 + 1] - tq[s2] + kBlock - 1) / kBlock);
+        const unsigned need =
+            (unsigned)((tq[s2 + 1] - tq[s2] + chunk_rows - 1) / chunk_rows);
         unsigned spins = 0;
-        while (__hip_atomic_load(&done[(p - 1) * S + s2], __ATOMIC_ACQUIRE,
+        while (__hip_atomic_load(&done[(p - 1) * S + s2], __ATOMIC_RELAXED,
                                  __HIP_MEMORY_SCOPE_AGENT) < need) {
-          __builtin_amdgcn_s_sleep(2);
+          __builtin_amdgcn_s_sleep(1);
           if (++spins > kSpinLimit ||
               __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
             atomicOr(err, 1);
@@ -200,15 +245,17 @@ __global__ __launch_bounds__(kBlock) void k_gs_wave(
       }
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    const int c = dir == 0 ? p : C - 1 - p;
+    const int c = DIR == 0 ? p : C - 1 - p;
     const int64_t *tp = tile_ptr + (int64_t)c * (S + 1);
-    const int64_t i = tp[sl] + (int64_t)chunk * kBlock + threadIdx.x;
-    if (i < tp[sl + 1]) gs_row<STREAM>(i, n, col, val, r, x);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const int64_t cb = tp[0], ce = tp[S];  // this colour's device rows
+    const int64_t b0 = tp[sl] + (int64_t)chunk * chunk_rows;
+    const int64_t e0 = min(b0 + chunk_rows, tp[sl + 1]);
+    for (int64_t i = b0 + threadIdx.x; i < e0; i += kBlock)
+      gs_row_wave<STREAM, DIR>(i, n, cb, ce, col, val, r, x);
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's x stores are complete
     __syncthreads();
     if (threadIdx.x == 0)
-      __hip_atomic_fetch_add(&done[p * S + sl], 1u, __ATOMIC_RELEASE,
+      __hip_atomic_fetch_add(&done[p * S + sl], 1u, __ATOMIC_RELAXED,
                              __HIP_MEMORY_SCOPE_AGENT);
   }
 }
@@ -551,35 +598,44 @@ void launch_residual0(Problem &P, int64_t n, const double *b, const double *Ap,
 }
 
 namespace {
-template <bool STREAM, bool FINEST>
-void wave_launch(Problem &P, const Level &L, int dir, const double *r,
-                 double *x, hipStream_t s) {
-  const int32_t *tasks = L.tasks[dir];
-  int64_t ntasks = L.ntasks[dir];
+template <bool STREAM, bool FINEST, int DIR>
+void wave_launch(Problem &P, const Level &L, const double *r, double *x,
+                 hipStream_t s) {
+  const int32_t *tasks = L.tasks[DIR];
+  int64_t ntasks = L.ntasks[DIR];
   const int64_t *tile_ptr = L.tile_ptr;
-  int S = L.nslabs, C = L.ncolors;
+  int S = L.nslabs, C = L.ncolors, chunk_rows = L.wave_chunk;
   unsigned *done = L.done;
   int *err = P.wave_err;
   int64_t n = L.n;
   const int32_t *col = L.col;
   const double *val = L.val;
-  void *args[] = {(void *)&tasks, (void *)&ntasks, (void *)&tile_ptr,
-                  (void *)&S,     (void *)&C,      (void *)&dir,
-                  (void *)&done,  (void *)&err,    (void *)&n,
-                  (void *)&col,   (void *)&val,    (void *)&r,
+  void *args[] = {(void *)&tasks, (void *)&ntasks,     (void *)&tile_ptr,
+                  (void *)&S,     (void *)&C,          (void *)&chunk_rows,
+                  (void *)&done,  (void *)&err,        (void *)&n,
+                  (void *)&col,   (void *)&val,        (void *)&r,
                   (void *)&x};
   const unsigned g = (unsigned)std::min<int64_t>(P.wave_grid, ntasks);
   HPCGX_HIP(hipLaunchCooperativeKernel(
-      reinterpret_cast<const void *>(&k_gs_wave<STREAM, FINEST>), dim3(g),
+      reinterpret_cast<const void *>(&k_gs_wave<STREAM, FINEST, DIR>), dim3(g),
       dim3(kBlock), args, 0, s));
 }
 
 template <bool STREAM, bool FINEST>
+void wave_launch(Problem &P, const Level &L, int dir, const double *r,
+                 double *x, hipStream_t s) {
+  if (dir == 0)
+    wave_launch<STREAM, FINEST, 0>(P, L, r, x, s);
+  else
+    wave_launch<STREAM, FINEST, 1>(P, L, r, x, s);
+}
+
+template <bool STREAM, bool FINEST, int DIR>
 int wave_blocks_per_cu() {
   int nb = 0;
   HPCGX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &nb, reinterpret_cast<const void *>(&k_gs_wave<STREAM, FINEST>), kBlock,
-      0));
+      &nb, reinterpret_cast<const void *>(&k_gs_wave<STREAM, FINEST, DIR>),
+      kBlock, 0));
   return nb;
 }
 }  // namespace
@@ -589,9 +645,12 @@ int wave_grid_size() {
   HPCGX_HIP(hipGetDevice(&dev));
   hipDeviceProp_t prop;
   HPCGX_HIP(hipGetDeviceProperties(&prop, dev));
-  const int nb = std::min({wave_blocks_per_cu<true, true>(),
-                           wave_blocks_per_cu<true, false>(),
-                           wave_blocks_per_cu<false, false>()});
+  const int nb = std::min({wave_blocks_per_cu<true, true, 0>(),
+                           wave_blocks_per_cu<true, true, 1>(),
+                           wave_blocks_per_cu<true, false, 0>(),
+                           wave_blocks_per_cu<true, false, 1>(),
+                           wave_blocks_per_cu<false, false, 0>(),
+                           wave_blocks_per_cu<false, false, 1>()});
   if (nb < 1) throw Error(HPCGX_EHIP, "wavefront kernel cannot be resident");
   return nb * prop.multiProcessorCount;
 }

@@ -382,8 +382,6 @@ void build_chunks(Problem &P, Level &L) {
 
 }  // namespace
 
-void build_wave(Problem &P, Level &L, int slab_factor, hipStream_t s);
-
 void generate_natural(NaturalLevel &N, hipStream_t s) {
   const int64_t n = N.n;
   k_generate<<<grid_for(n), kBlock, 0, s>>>(N.nx, N.ny, N.nz, n, N.col, N.val,
@@ -526,7 +524,8 @@ void finalize_level(Problem &P, int l, NaturalLevel &N, int flags,
                                                L.dslot, L.nnz_row);
   HPCGX_LAUNCHED();
   HPCGX_HIP(hipStreamSynchronize(s));
-  if (n >= kWaveMinRows) build_wave(P, L, kWaveSlabFactor, s);
+  if (P.use_wave && n >= kWaveMinRows)
+    build_wave(P, L, kWaveSlabFactor, kWaveRows, s);
 }
 
 // Wavefront schedule of the multicolour sweeps.  Natural rows are cut into
@@ -551,9 +550,11 @@ void drop_wave(Problem &P, Level &L) {
   L.wave = false;
 }
 
-void build_wave(Problem &P, Level &L, int slab_factor, hipStream_t s) {
+void build_wave(Problem &P, Level &L, int slab_factor, int rows, hipStream_t s) {
   drop_wave(P, L);
   const int C = L.ncolors;
+  const int CH = kBlock * std::max(1, rows);
+  L.wave_chunk = CH;
   const int64_t n = L.n;
   const int64_t W = std::max<int64_t>(1, L.bandwidth) * std::max(1, slab_factor);
   const int S = (int)((n + W - 1) / W);
@@ -578,8 +579,8 @@ void build_wave(Problem &P, Level &L, int slab_factor, hipStream_t s) {
         const int sl = t - 2 * p;
         if (sl < 0 || sl >= S) continue;
         const int c = dir == 0 ? p : C - 1 - p;
-        const int64_t rows = tp[(int64_t)c * (S + 1) + sl + 1] - tp[(int64_t)c * (S + 1) + sl];
-        const int64_t nch = (rows + kBlock - 1) / kBlock;
+        const int64_t nrows = tp[(int64_t)c * (S + 1) + sl + 1] - tp[(int64_t)c * (S + 1) + sl];
+        const int64_t nch = (nrows + CH - 1) / CH;
         for (int64_t k = 0; k < nch; ++k) {
           tasks.push_back(p * S + sl);
           tasks.push_back((int32_t)k);

@@ -204,7 +204,10 @@ int hpcgx_cg(hpcgx_problem *p, int ordering, const double *b, double *x,
 
 /* No counterpart in the reference (tuning knobs of this implementation):
  *   "wave"             1/0: multicolour sweeps of large levels as one
- *                      wavefront launch per direction (default 1)
+ *                      wavefront launch per direction (default 0: on
+ *                      MI355X it measured slower than one launch per
+ *                      colour step)
+ *   "wave_rows"        rows per thread of a wavefront task (1..16)
  *   "wave_min_rows"    levels with at least this many rows get a wavefront
  *                      schedule (default 2^23)
  *   "wave_slab_factor" slab width in units of the matrix bandwidth (>= 1)
