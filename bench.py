@@ -36,6 +36,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-grid", type=int, default=128, help="grid edge of the CPU sample")
     ap.add_argument("--cpu-iters", type=int, default=25, help="CG_ref iterations of the CPU sample")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device to device (the known-size kernel that scales "
+                         "the FETCH_SIZE / WRITE_SIZE counters) and exit")
     return ap.parse_args()
 
 
@@ -74,6 +77,13 @@ def main():
               "(--gpus %d requested)" % a.gpus, file=sys.stderr)
         sys.exit(2)
     import torch
+    if a.copy_calibration:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": src.numel() * 8}))
+        return
     import hpcg_amd
     from hpcg_amd import model
 
@@ -121,6 +131,17 @@ def main():
     launch_ms = ks_ms / max(ks_n, 1)
     achieved = per_launch / (launch_ms * 1e-3) * 1e-9 if ks_n else None
     rating = flops["total"] / (dt + setup_s / 10.0) * 1e-9  # ReportResults.cpp:353, one set
+    # HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # (profiles/), when this run is the profiled workload and kernel
+    traffic = None
+    try:
+        pm = json.load(open(os.path.join(ROOT, "profiles", "pmc_traffic.json")))
+        kk = "k_gs_range<true,true>"
+        if (not info0["wave"] and pm["workload"] == "%dx%dx%d fp64 %s" % (a.nx, a.ny, a.nz, a.ordering)
+                and kk in pm["kernels"]):
+            traffic = pm["kernels"][kk]["traffic_bytes"]
+    except (OSError, ValueError, KeyError):
+        traffic = None
 
     out = {
         "metric": "HPCG GFLOP/s (FP64 PCG, 4-level MG, %s SYMGS)" % a.ordering,
@@ -151,7 +172,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-            "traffic": None,
+            "traffic": round(traffic) if traffic else None,
             "bytes_per_launch": per_launch,
             "mean_launch_ms": round(launch_ms, 5),
             "launches": ks_n,

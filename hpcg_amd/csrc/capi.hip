@@ -398,6 +398,8 @@ int hpcgx_tune(hpcgx_problem *p, const char *key, double value) {
       hpcgx::Level &L = P.L[l];
       if (k == "xcd_remap") {
         L.xcd = value != 0.0;
+      } else if (k == "pair_rows") {
+        L.pair_rows = value != 0.0;
       } else if (k == "stream_min_bytes") {
         L.stream = (double)L.n * kSlots * 12.0 > value;
       } else if (k == "wave_min_rows") {

@@ -81,6 +81,7 @@ struct Level {
   bool finest = false;  // level 0 (its kernels get their own symbols)
   bool stream = false;  // matrix >> last-level cache: non-temporal matrix loads
   bool xcd = false;     // XCD-aware workgroup -> rows mapping (measured slower)
+  bool pair_rows = false;  // colour steps with two adjacent rows per thread
   int nx = 0, ny = 0, nz = 0;
   int64_t n = 0, nnz = 0;
   int32_t *col = nullptr;   // [kSlots * n] device-order column ids

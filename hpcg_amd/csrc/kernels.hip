@@ -160,6 +160,53 @@ __global__ __launch_bounds__(kBlock) void k_gs_range(
   if (i < end) gs_row<STREAM>(i, n, col, val, r, x);
 }
 
+// Two adjacent rows per thread: the matrix is read with 16-byte (values) and
+// 8-byte (column ids) loads per lane.  Needs an even level size and an even
+// range start, so that slot j of the pair is 16-byte aligned.
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef int i32x2 __attribute__((ext_vector_type(2)));
+
+template <bool STREAM>
+__device__ inline void gs_row_pair(int64_t i, int64_t n,
+                                   const int32_t *__restrict__ col,
+                                   const double *__restrict__ val,
+                                   const double *__restrict__ r, double *x) {
+  const f64x2 rr = *reinterpret_cast<const f64x2 *>(r + i);
+  double s0 = rr.x, s1 = rr.y, d0 = 0.0, d1 = 0.0;
+  bool h0 = false, h1 = false;
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int64_t k = (int64_t)j * n + i;
+    const f64x2 v = mload<STREAM>(reinterpret_cast<const f64x2 *>(val + k));
+    const i32x2 c = mload<STREAM>(reinterpret_cast<const i32x2 *>(col + k));
+    s0 -= v.x * x[c.x];
+    s1 -= v.y * x[c.y];
+    if (!h0 && c.x == i) {
+      d0 = v.x;
+      h0 = true;
+    }
+    if (!h1 && c.y == i + 1) {
+      d1 = v.y;
+      h1 = true;
+    }
+  }
+  s0 += x[i] * d0;
+  s1 += x[i + 1] * d1;
+  x[i] = s0 / d0;
+  x[i + 1] = s1 / d1;
+}
+
+template <bool STREAM, bool FINEST>
+__global__ __launch_bounds__(kBlock) void k_gs_range2(
+    int64_t begin, int64_t end, int64_t n, const int32_t *__restrict__ col,
+    const double *__restrict__ val, const double *__restrict__ r, double *x) {
+  const int64_t i = begin + 2 * gtid();
+  if (i + 1 < end)
+    gs_row_pair<STREAM>(i, n, col, val, r, x);
+  else if (i < end)
+    gs_row<STREAM>(i, n, col, val, r, x);
+}
+
 // Multicolour sweep in one launch, as a wavefront over natural-row slabs
 // (see build_wave for the task order).  The grid is co-resident (cooperative
 // launch) and workgroup g runs tasks g, g+G, g+2G, ... in list order, so the
@@ -497,6 +544,17 @@ void launch_spmv(const Level &L, const double *x, double *y, hipStream_t s) {
 void launch_symgs_rows(const Level &L, int64_t begin, int64_t end,
                        const double *r, double *x, hipStream_t s) {
   if (end <= begin) return;
+  if (L.pair_rows && L.n % 2 == 0 && begin % 2 == 0) {
+    const unsigned g2 = grid_for((end - begin + 1) / 2);
+    if (L.finest)
+      k_gs_range2<true, true><<<g2, kBlock, 0, s>>>(begin, end, L.n, L.col, L.val, r, x);
+    else if (L.stream)
+      k_gs_range2<true, false><<<g2, kBlock, 0, s>>>(begin, end, L.n, L.col, L.val, r, x);
+    else
+      k_gs_range2<false, false><<<g2, kBlock, 0, s>>>(begin, end, L.n, L.col, L.val, r, x);
+    HPCGX_LAUNCHED();
+    return;
+  }
   const unsigned g = grid_for(end - begin);
   if (L.finest)
     k_gs_range<true, true><<<g, kBlock, 0, s>>>(begin, end, L.n, L.col, L.val,

@@ -212,6 +212,8 @@ int hpcgx_cg(hpcgx_problem *p, int ordering, const double *b, double *x,
  *                      schedule (default 2^23)
  *   "wave_slab_factor" slab width in units of the matrix bandwidth (>= 1)
  *   "xcd_remap"        1/0: map workgroups to XCD-contiguous row ranges
+ *   "pair_rows"        1/0: colour steps with two adjacent rows per thread
+ *                      (16-byte matrix loads)
  *   "stream_min_bytes" matrices larger than this are read with non-temporal
  *                      loads (default 512e6)
  * Synchronous.  Unknown keys return HPCGX_EINVAL. */

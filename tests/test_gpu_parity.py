@@ -170,6 +170,25 @@ def test_symgs_bitexact(hp, dims, ordering):
         assert np.array_equal(P.to_numpy(x, l), want), l
 
 
+@pytest.mark.parametrize("dims", [(16, 16, 16), (32, 48, 64)])
+def test_pair_rows_bitexact(hp, dims):
+    """Colour steps with two rows per thread (16-byte matrix loads) keep the
+    colour-ordered result bit for bit."""
+    say("pair rows", dims)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    P.tune("pair_rows", 1)
+    h = hier(dims, color=True)
+    for l in range(4):
+        m = h.level(l)
+        n = h.nrow(l)
+        r, x0 = rnd(150 + l, n), rnd(160 + l, n)
+        want = O.symgs(m, r, x0, O.color_order(O.colors(m)[0]))
+        x = P.to_device_order(x0, l)
+        P.symgs(P.to_device_order(r, l), x, level=l, ordering="multicolor")
+        assert np.array_equal(P.to_numpy(x, l), want), l
+    del P
+
+
 def test_symgs_golden(hp, g16):
     say("symgs golden 16^3")
     P = problem(hp, (16, 16, 16))
