@@ -10,6 +10,9 @@ Prints one JSON line:
   value      HPCG GFLOP/s by the reference's flop model (ReportResults.cpp:75-99)
   roofline   the finest level's colour-step Gauss-Seidel kernel: algorithmic
              bytes per launch / mean launch time (HIP events on its stream)
+  convergence  FP64 PCG vs the mixed-precision PCG (FP32 V-cycle on FP16
+             matrix values, FP64 outer loop) from x0 = 0 to ||r||/||r0|| <= 1e-9:
+             iterations, time, error against the all-ones solution
   cpu_baseline  the reference itself (oracle/_ref/ref_time, OpenMP on the host
              cores), one CG_ref set on a bounded grid
 """
@@ -36,6 +39,10 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-grid", type=int, default=128, help="grid edge of the CPU sample")
     ap.add_argument("--cpu-iters", type=int, default=25, help="CG_ref iterations of the CPU sample")
+    ap.add_argument("--tol-compare", type=float, default=1e-9,
+                    help="FP64 vs mixed-precision time to this residual (0 = skip)")
+    ap.add_argument("--max-iters", type=int, default=2000,
+                    help="iteration cap of the time-to-tolerance runs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device (the known-size kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -68,6 +75,47 @@ def cpu_baseline(grid, iters):
     fl = model.hpcg_model([l["n"] for l in lv], [l["nnz"] for l in lv], 1, iters, iters)["total"]
     return {"value": round(fl / dt * 1e-9, 4), "unit": "GFLOP/s", "cores": 1, "kind": "port",
             "sample": "oracle port CG, %d^3, %d iterations, %.1f s" % (g, iters, dt)}
+
+
+def converge(P, b, lv, a, log):
+    """FP64 PCG and the mixed-precision PCG (FP32 V-cycle on FP16 values, FP64
+    outer loop, FP64 residual recomputed every 10 iterations), each from
+    x0 = 0 to ||r||/||r0|| <= tol: iterations, time, error vs the all-ones
+    solution, and the mixed solution's distance from the FP64 one."""
+    import torch
+    from hpcg_amd import model
+    nrow = [l["n"] for l in lv]
+    nnz = [l["nnz"] for l in lv]
+    res = {"tolerance": a.tol_compare, "recompute_every": 10}
+    xs = {}
+    P.prepare_mixed()
+    for mode in ("fp64", "mixed"):
+        x = torch.zeros_like(b)
+        if mode == "fp64":
+            P.cg(b, x, max_iter=2, tolerance=0.0)  # warm
+        else:
+            P.cg_mixed(b, x, max_iter=2, tolerance=0.0)
+        x.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if mode == "fp64":
+            r = P.cg(b, x, max_iter=a.max_iters, tolerance=a.tol_compare)
+        else:
+            r = P.cg_mixed(b, x, max_iter=a.max_iters, tolerance=a.tol_compare, recompute=10)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        k = r["niters"]
+        fl = model.hpcg_model(nrow, nnz, 1, k, k)["total"]
+        res[mode] = {"iterations": k, "seconds": round(dt, 4),
+                     "ms_per_iteration": round(dt / max(k, 1) * 1e3, 4),
+                     "normr_over_normr0": r["normr"] / r["normr0"],
+                     "max_abs_err_vs_ones": float((x - 1.0).abs().max().item()),
+                     "hpcg_gflops_equiv": round(fl / dt * 1e-9, 3)}
+        xs[mode] = x
+        log("%s: %d iterations to %.1e in %.3f s" % (mode, k, a.tol_compare, dt))
+    res["mixed"]["max_abs_diff_vs_fp64"] = float((xs["mixed"] - xs["fp64"]).abs().max().item())
+    res["speedup_time_to_tolerance"] = round(res["fp64"]["seconds"] / res["mixed"]["seconds"], 4)
+    return res
 
 
 def main():
@@ -178,6 +226,8 @@ def main():
             "launches": ks_n,
         },
     }
+    if a.tol_compare > 0:
+        out["convergence"] = converge(P, b, lv, a, log)
     if not a.no_cpu_baseline:
         log("cpu baseline")
         try:

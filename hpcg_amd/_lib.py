@@ -69,6 +69,11 @@ SIGNATURES = {
     "hpcgx_dot_host": (I, [P, I64, P, P, P, P]),
     "hpcgx_waxpby": (I, [I64, D, P, D, P, P, P]),
     "hpcgx_cg": (I, [P, I, P, P, I, D, I, I, P, P, P, P, P]),
+    "hpcgx_prepare_mixed": (I, [P]),
+    "hpcgx_spmv32": (I, [P, I, P, P, P]),
+    "hpcgx_symgs32": (I, [P, I, I, P, P, P]),
+    "hpcgx_mg32": (I, [P, I, I, P, P, P]),
+    "hpcgx_cg_mixed": (I, [P, I, P, P, I, D, I, P, P, P, P, P]),
     "hpcgx_tune": (I, [P, ctypes.c_char_p, D]),
     "hpcgx_check_status": (I, [P, P]),
     "hpcgx_kernel_timing": (I, [P, I]),

@@ -291,6 +291,7 @@ int hpcgx_replace_diagonal(hpcgx_problem *p, int level, const double *diag, void
   return guard([&] {
     hpcgx::Level &L = level_of(p, level);
     hpcgx::launch_replace_diag(L, diag, S(stream));
+    hpcgx::replace_diag32(L, diag, S(stream));
   });
 }
 
@@ -454,6 +455,53 @@ int hpcgx_kernel_timing_read(hpcgx_problem *p, double *total_ms, int64_t *launch
     }
     *total_ms = t;
     *launches = (int64_t)(K.used / 2);
+  });
+}
+
+int hpcgx_prepare_mixed(hpcgx_problem *p) {
+  return guard([&] {
+    if (!p) throw Error(HPCGX_EINVAL, "NULL problem");
+    hpcgx::prepare_mixed(p->P);
+  });
+}
+
+int hpcgx_spmv32(hpcgx_problem *p, int level, const float *x, float *y, void *stream) {
+  return guard([&] {
+    hpcgx::Level &L = level_of(p, level);
+    hpcgx::prepare_mixed(p->P);
+    hpcgx::spmv32(L, x, y, S(stream));
+  });
+}
+
+int hpcgx_symgs32(hpcgx_problem *p, int level, int ordering, const float *r, float *x,
+                  void *stream) {
+  return guard([&] {
+    level_of(p, level);
+    check_order(ordering);
+    hpcgx::prepare_mixed(p->P);
+    hpcgx::symgs32(p->P, level, ordering, r, x, S(stream));
+  });
+}
+
+int hpcgx_mg32(hpcgx_problem *p, int level, int ordering, const float *r, float *x,
+               void *stream) {
+  return guard([&] {
+    level_of(p, level);
+    check_order(ordering);
+    hpcgx::prepare_mixed(p->P);
+    hpcgx::mg32(p->P, level, ordering, r, x, S(stream));
+  });
+}
+
+int hpcgx_cg_mixed(hpcgx_problem *p, int ordering, const double *b, double *x,
+                   int max_iter, double tolerance, int recompute, int *niters,
+                   double *normr, double *normr0, double *times, void *stream) {
+  return guard([&] {
+    if (!p || !b || !x || !niters || !normr || !normr0) throw Error(HPCGX_EINVAL, "NULL argument");
+    check_order(ordering);
+    if (max_iter < 0 || recompute < 0) throw Error(HPCGX_EINVAL, "negative count");
+    hpcgx::cg_mixed(p->P, ordering, b, x, max_iter, tolerance, recompute, niters, normr,
+                    normr0, times, S(stream));
   });
 }
 

@@ -114,6 +114,11 @@ struct Level {
   unsigned *done = nullptr;          // [ncolors*nslabs] finished chunks per tile
   // optional per-row ids (natural order) kept for inspection
   int32_t *level_of = nullptr, *color_of = nullptr;
+  // mixed precision (built on first use): FP16 values with the diagonal slot
+  // zeroed, the FP32 diagonal, FP32 coarse-level vectors
+  _Float16 *val16 = nullptr;
+  float *diag32 = nullptr;
+  float *rc32 = nullptr, *xc32 = nullptr;
   // transfer to the next level: f2c[coarse device row] = fine device row
   int64_t nc = 0;
   int32_t *f2c = nullptr;
@@ -135,6 +140,8 @@ struct Problem {
   int *wave_err = nullptr;    // set when a wavefront wait gives up
   int wave_grid = 0;          // co-resident workgroups of the wavefront kernel
   bool use_wave = false;      // multicolour sweeps as wavefronts (measured slower)
+  bool mixed_ready = false;   // FP16/FP32 copies built
+  float *r32 = nullptr, *z32 = nullptr;  // FP32 input / output of the V-cycle
   // event log of the finest level's multicolour Gauss-Seidel launches
   struct KernelLog {
     bool on = false;
@@ -202,6 +209,25 @@ void build_wave(Problem &P, Level &L, int slab_factor, int rows, hipStream_t s);
 void drop_wave(Problem &P, Level &L);
 void problem_vectors(const Problem &P, double *b, double *x, double *xexact,
                      hipStream_t s);
+
+// ---- mixed precision (mixed.hip) ----
+struct TempBufD {  // a temporary device buffer freed on scope exit
+  void *p = nullptr;
+  explicit TempBufD(size_t bytes) { HPCGX_HIP(hipMalloc(&p, bytes ? bytes : 1)); }
+  ~TempBufD() { (void)hipFree(p); }
+};
+void prepare_mixed(Problem &P);
+void replace_diag32(Level &L, const double *d, hipStream_t s);
+void symgs32(Problem &P, int l, int ordering, const float *r, float *x,
+             hipStream_t s);
+void spmv32(const Level &L, const float *x, float *y, hipStream_t s);
+void mg32(Problem &P, int l, int ordering, const float *r, float *x,
+          hipStream_t s);
+void to_f32(int64_t n, const double *in, float *out, hipStream_t s);
+void to_f64(int64_t n, const float *in, double *out, hipStream_t s);
+void cg_mixed(Problem &P, int ordering, const double *b, double *x,
+              int max_iter, double tol, int recompute, int *niters,
+              double *normr, double *normr0, double *times, hipStream_t s);
 
 // ---- solver (solver.hip) ----
 void symgs(Problem &P, int l, int ordering, const double *r, double *x,

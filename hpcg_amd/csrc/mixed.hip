@@ -1,0 +1,327 @@
+// mixed.hip -- the mixed-precision preconditioner: the V-cycle in FP32 with
+// matrix values stored in FP16, under the FP64 outer PCG.
+//
+// Storage per level (built on first use): val16 = the ELL values rounded to
+// FP16 (double -> float -> half, round to nearest even each time) with the
+// diagonal slot set to 0, and diag32 = the diagonal in FP32, so that a
+// diagonal scaled beyond the FP16 range (TestCG multiplies it by up to
+// 1.1e7) stays exact to FP32.  Column ids are shared with the FP64 ELL, so
+// the V-cycle streams 6 bytes per stored entry instead of 12.
+//
+// Row arithmetic (no fused multiply-add, correctly rounded division):
+//   Gauss-Seidel  sum = r_i; sum -= h_ij * x_j over the slots (the diagonal
+//                 and unused slots hold 0); x_i = sum / d_i
+//   A*x           s = sum_j h_ij * x_j over the slots; s += d_i * x_i
+// The outer PCG (CG.cpp:59-143 of the reference) stays in FP64: SpMV with the
+// FP64 matrix, dot products, alpha, beta, x and r; every `recompute`-th
+// iteration r = b - A x is recomputed in FP64 instead of updated.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+#include "internal.hpp"
+
+namespace hpcgx {
+namespace {
+
+typedef _Float16 half_t;
+
+inline unsigned grid_for(int64_t n) {
+  return (unsigned)((n + kBlock - 1) / kBlock);
+}
+__device__ inline int64_t gtid() {
+  return (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+}
+
+template <bool STREAM, class T>
+__device__ inline T mload(const T *p) {
+  if (STREAM) return __builtin_nontemporal_load(p);
+  return *p;
+}
+
+__global__ __launch_bounds__(kBlock) void k_make_half(
+    int64_t n, const double *__restrict__ val, const uint8_t *__restrict__ dslot,
+    half_t *__restrict__ val16, float *__restrict__ diag32, int *overflow) {
+  const int64_t i = gtid();
+  if (i >= n) return;
+  const int ds = dslot[i];
+  bool bad = false;
+  for (int j = 0; j < kSlots; ++j) {
+    const int64_t k = (int64_t)j * n + i;
+    const float f = (float)val[k];
+    const half_t h = (j == ds) ? (half_t)0.0f : (half_t)f;
+    bad |= (j != ds) && __builtin_isinf((float)h) && !__builtin_isinf(f);
+    val16[k] = h;
+  }
+  diag32[i] = (float)val[(int64_t)ds * n + i];
+  if (bad) atomicOr(overflow, 1);
+}
+
+__global__ __launch_bounds__(kBlock) void k_diag32(int64_t n, const double *d,
+                                                    float *diag32) {
+  const int64_t i = gtid();
+  if (i < n) diag32[i] = (float)d[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill32(float *x, int64_t n) {
+  const int64_t i = gtid();
+  if (i < n) x[i] = 0.0f;
+}
+
+__global__ __launch_bounds__(kBlock) void k_to_f32(int64_t n, const double *in,
+                                                    float *out) {
+  const int64_t i = gtid();
+  if (i < n) out[i] = (float)in[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_to_f64(int64_t n, const float *in,
+                                                    double *out) {
+  const int64_t i = gtid();
+  if (i < n) out[i] = (double)in[i];
+}
+
+template <bool STREAM>
+__device__ inline void gs32_row(int64_t i, int64_t n,
+                                const int32_t *__restrict__ col,
+                                const half_t *__restrict__ val16,
+                                const float *__restrict__ diag32,
+                                const float *__restrict__ r, float *x) {
+  float sum = r[i];
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int64_t k = (int64_t)j * n + i;
+    sum -= (float)mload<STREAM>(val16 + k) * x[mload<STREAM>(col + k)];
+  }
+  x[i] = sum / diag32[i];
+}
+
+template <bool STREAM>
+__device__ inline float row_dot32(int64_t i, int64_t n,
+                                  const int32_t *__restrict__ col,
+                                  const half_t *__restrict__ val16,
+                                  const float *__restrict__ diag32,
+                                  const float *__restrict__ x) {
+  float s = 0.0f;
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int64_t k = (int64_t)j * n + i;
+    s += (float)mload<STREAM>(val16 + k) * x[mload<STREAM>(col + k)];
+  }
+  s += diag32[i] * x[i];
+  return s;
+}
+
+template <bool STREAM, bool FINEST>
+__global__ __launch_bounds__(kBlock) void k_gs32_range(
+    int64_t begin, int64_t end, int64_t n, const int32_t *__restrict__ col,
+    const half_t *__restrict__ val16, const float *__restrict__ diag32,
+    const float *__restrict__ r, float *x) {
+  const int64_t i = begin + gtid();
+  if (i < end) gs32_row<STREAM>(i, n, col, val16, diag32, r, x);
+}
+
+__global__ __launch_bounds__(kBlock) void k_gs32_list(
+    const int32_t *__restrict__ rows, int64_t count, int64_t n,
+    const int32_t *__restrict__ col, const half_t *__restrict__ val16,
+    const float *__restrict__ diag32, const float *__restrict__ r, float *x) {
+  const int64_t k = gtid();
+  if (k < count) gs32_row<false>(rows[k], n, col, val16, diag32, r, x);
+}
+
+template <bool STREAM>
+__global__ __launch_bounds__(kBlock) void k_spmv32(
+    const int64_t *__restrict__ chunks, int64_t n,
+    const int32_t *__restrict__ col, const half_t *__restrict__ val16,
+    const float *__restrict__ diag32, const float *__restrict__ x,
+    float *__restrict__ y) {
+  const int64_t b = blockIdx.x;
+  const int64_t i = chunks[2 * b] + threadIdx.x;
+  if (i < chunks[2 * b + 1]) y[i] = row_dot32<STREAM>(i, n, col, val16, diag32, x);
+}
+
+template <bool STREAM>
+__global__ __launch_bounds__(kBlock) void k_resid_restrict32(
+    int64_t nc, int64_t n, const int32_t *__restrict__ f2c,
+    const int32_t *__restrict__ col, const half_t *__restrict__ val16,
+    const float *__restrict__ diag32, const float *__restrict__ rf,
+    const float *__restrict__ xf, float *__restrict__ rc) {
+  const int64_t c = gtid();
+  if (c < nc) {
+    const int64_t f = f2c[c];
+    rc[c] = rf[f] - row_dot32<STREAM>(f, n, col, val16, diag32, xf);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_prolong32(
+    int64_t nc, const int32_t *__restrict__ f2c, const float *__restrict__ xc,
+    float *__restrict__ xf) {
+  const int64_t c = gtid();
+  if (c < nc) xf[f2c[c]] += xc[c];
+}
+
+void gs32_step(const Level &L, int64_t begin, int64_t end, const float *r,
+               float *x, hipStream_t s) {
+  if (end <= begin) return;
+  const unsigned g = grid_for(end - begin);
+  if (L.finest)
+    k_gs32_range<true, true><<<g, kBlock, 0, s>>>(begin, end, L.n, L.col,
+                                                  L.val16, L.diag32, r, x);
+  else if (L.stream)
+    k_gs32_range<true, false><<<g, kBlock, 0, s>>>(begin, end, L.n, L.col,
+                                                   L.val16, L.diag32, r, x);
+  else
+    k_gs32_range<false, false><<<g, kBlock, 0, s>>>(begin, end, L.n, L.col,
+                                                    L.val16, L.diag32, r, x);
+  HPCGX_LAUNCHED();
+}
+
+}  // namespace
+
+void prepare_mixed(Problem &P) {
+  if (P.mixed_ready) return;
+  TempBufD overflow(sizeof(int));
+  HPCGX_HIP(hipMemset(overflow.p, 0, sizeof(int)));
+  for (int l = 0; l < P.nlevels; ++l) {
+    Level &L = P.L[l];
+    L.val16 = reinterpret_cast<half_t *>(P.arena.alloc<uint16_t>((size_t)kSlots * L.n));
+    L.diag32 = P.arena.alloc<float>(L.n);
+    k_make_half<<<grid_for(L.n), kBlock>>>(L.n, L.val, L.dslot, L.val16, L.diag32,
+                                           static_cast<int *>(overflow.p));
+    HPCGX_LAUNCHED();
+    if (L.nc) {
+      L.rc32 = P.arena.alloc<float>(L.nc);
+      L.xc32 = P.arena.alloc<float>(L.nc);
+    }
+  }
+  P.r32 = P.arena.alloc<float>(P.L[0].n);
+  P.z32 = P.arena.alloc<float>(P.L[0].n);
+  int bad = 0;
+  HPCGX_HIP(hipMemcpy(&bad, overflow.p, sizeof bad, hipMemcpyDeviceToHost));
+  if (bad) throw Error(HPCGX_EUNSUP, "an off-diagonal value overflows FP16");
+  P.mixed_ready = true;
+}
+
+void replace_diag32(Level &L, const double *d, hipStream_t s) {
+  if (!L.diag32) return;
+  k_diag32<<<grid_for(L.n), kBlock, 0, s>>>(L.n, d, L.diag32);
+  HPCGX_LAUNCHED();
+}
+
+void symgs32(Problem &P, int l, int ordering, const float *r, float *x,
+             hipStream_t s) {
+  const Level &L = P.L[l];
+  if (ordering == HPCGX_ORDER_MULTICOLOR) {
+    for (int c = 0; c < L.ncolors; ++c)
+      gs32_step(L, L.color_ptr[c], L.color_ptr[c + 1], r, x, s);
+    for (int c = L.ncolors - 1; c >= 0; --c)
+      gs32_step(L, L.color_ptr[c], L.color_ptr[c + 1], r, x, s);
+  } else {
+    for (int pass = 0; pass < 2; ++pass)
+      for (int k = 0; k < L.nsched; ++k) {
+        const int t = pass == 0 ? k : L.nsched - 1 - k;
+        const int64_t cnt = L.level_ptr[t + 1] - L.level_ptr[t];
+        k_gs32_list<<<grid_for(cnt), kBlock, 0, s>>>(
+            L.level_rows + L.level_ptr[t], cnt, L.n, L.col, L.val16, L.diag32,
+            r, x);
+        HPCGX_LAUNCHED();
+      }
+  }
+}
+
+void spmv32(const Level &L, const float *x, float *y, hipStream_t s) {
+  if (L.stream)
+    k_spmv32<true><<<(unsigned)L.nchunks, kBlock, 0, s>>>(L.chunks, L.n, L.col,
+                                                          L.val16, L.diag32, x, y);
+  else
+    k_spmv32<false><<<(unsigned)L.nchunks, kBlock, 0, s>>>(L.chunks, L.n, L.col,
+                                                           L.val16, L.diag32, x, y);
+  HPCGX_LAUNCHED();
+}
+
+// ComputeMG_ref.cpp:39-63 in FP32.
+void mg32(Problem &P, int l, int ordering, const float *r, float *x,
+          hipStream_t s) {
+  Level &L = P.L[l];
+  k_fill32<<<grid_for(L.n), kBlock, 0, s>>>(x, L.n);
+  HPCGX_LAUNCHED();
+  symgs32(P, l, ordering, r, x, s);
+  if (l + 1 < P.nlevels) {
+    if (L.stream)
+      k_resid_restrict32<true><<<grid_for(L.nc), kBlock, 0, s>>>(
+          L.nc, L.n, L.f2c, L.col, L.val16, L.diag32, r, x, L.rc32);
+    else
+      k_resid_restrict32<false><<<grid_for(L.nc), kBlock, 0, s>>>(
+          L.nc, L.n, L.f2c, L.col, L.val16, L.diag32, r, x, L.rc32);
+    HPCGX_LAUNCHED();
+    mg32(P, l + 1, ordering, L.rc32, L.xc32, s);
+    k_prolong32<<<grid_for(L.nc), kBlock, 0, s>>>(L.nc, L.f2c, L.xc32, x);
+    HPCGX_LAUNCHED();
+    symgs32(P, l, ordering, r, x, s);
+  }
+}
+
+void to_f32(int64_t n, const double *in, float *out, hipStream_t s) {
+  k_to_f32<<<grid_for(n), kBlock, 0, s>>>(n, in, out);
+  HPCGX_LAUNCHED();
+}
+
+void to_f64(int64_t n, const float *in, double *out, hipStream_t s) {
+  k_to_f64<<<grid_for(n), kBlock, 0, s>>>(n, in, out);
+  HPCGX_LAUNCHED();
+}
+
+namespace {
+double read_scalar(const Problem &P, int slot, hipStream_t s) {
+  double v = 0.0;
+  HPCGX_HIP(hipMemcpyAsync(&v, P.scalars + slot, sizeof v,
+                           hipMemcpyDeviceToHost, s));
+  HPCGX_HIP(hipStreamSynchronize(s));
+  return v;
+}
+}  // namespace
+
+// CG.cpp:59-143 with the FP32 V-cycle as M^-1 and the FP64 residual
+// recomputed every `recompute` iterations.
+void cg_mixed(Problem &P, int ordering, const double *b, double *x,
+              int max_iter, double tol, int recompute, int *niters,
+              double *normr, double *normr0, double *times, hipStream_t s) {
+  prepare_mixed(P);
+  const auto t_begin = std::chrono::steady_clock::now();
+  const Level &A = P.L[0];
+  const int64_t n = A.n;
+  *niters = 0;
+  launch_copy(x, P.p, n, s);
+  launch_spmv(A, P.p, P.Ap, s);
+  launch_residual0(P, n, b, P.Ap, P.r, s);
+  double nr = std::sqrt(read_scalar(P, kNormr2, s));
+  *normr0 = nr;
+  const bool check_each = tol > 0.0;
+  for (int k = 1; k <= max_iter && nr / *normr0 > tol; ++k) {
+    to_f32(n, P.r, P.r32, s);
+    mg32(P, 0, ordering, P.r32, P.z32, s);
+    to_f64(n, P.z32, P.z, s);
+    launch_dot(P, n, P.r, P.z, nullptr, kFinishBeta, s);
+    if (k == 1)
+      launch_copy(P.z, P.p, n, s);
+    else
+      launch_update_p(n, P.z, P.p, P.scalars, s);
+    launch_spmv_dot(P, A, P.p, P.Ap, s);
+    launch_update_xr(P, n, x, P.r, P.p, P.Ap, s);
+    if (recompute > 0 && k % recompute == 0) {
+      // r = b - A x in FP64 (replaces the recursively updated r)
+      launch_spmv(A, x, P.Ap, s);
+      launch_residual0(P, n, b, P.Ap, P.r, s);
+    }
+    *niters = k;
+    if (check_each || k == max_iter) nr = std::sqrt(read_scalar(P, kNormr2, s));
+  }
+  if (!check_each) nr = std::sqrt(read_scalar(P, kNormr2, s));
+  HPCGX_HIP(hipStreamSynchronize(s));
+  *normr = nr;
+  if (times)
+    times[0] += std::chrono::duration<double>(std::chrono::steady_clock::now() -
+                                              t_begin).count();
+  if (!std::isfinite(nr)) throw Error(HPCGX_EINVAL, "CG residual is not finite");
+}
+
+}  // namespace hpcgx

@@ -32,15 +32,16 @@ def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
-def _ptr(t, n=None, what="vector"):
-    """Device pointer of a contiguous float64 CUDA tensor of length >= n."""
+def _ptr(t, n=None, what="vector", dtype="float64"):
+    """Device pointer of a contiguous CUDA tensor (float64 unless stated)
+    of length >= n."""
     if t is None:
         return None
     torch = _torch()
     if not isinstance(t, torch.Tensor) or not t.is_cuda:
         raise TypeError("%s must be a CUDA tensor" % what)
-    if t.dtype != torch.float64 or not t.is_contiguous():
-        raise TypeError("%s must be contiguous float64" % what)
+    if t.dtype != getattr(torch, dtype) or not t.is_contiguous():
+        raise TypeError("%s must be contiguous %s" % (what, dtype))
     if n is not None and t.numel() < n:
         raise ValueError("%s has %d entries, needs %d" % (what, t.numel(), n))
     return ctypes.c_void_p(t.data_ptr())
@@ -240,6 +241,40 @@ class Problem:
         out = self.empty(level)
         call("hpcgx_copy_diagonal", self._h, level, _ptr(out), _stream())
         return out
+
+    # ---- mixed precision (FP32 V-cycle on FP16 values) --------------------
+    def prepare_mixed(self):
+        return call("hpcgx_prepare_mixed", self._h)
+
+    def zeros32(self, level=0):
+        torch = _torch()
+        return torch.zeros(self.nrow(level), dtype=torch.float32, device="cuda")
+
+    def spmv32(self, x, y, level=0):
+        n = self.nrow(level)
+        return call("hpcgx_spmv32", self._h, level, _ptr(x, n, "x", "float32"),
+                    _ptr(y, n, "y", "float32"), _stream())
+
+    def symgs32(self, r, x, level=0, ordering="multicolor"):
+        n = self.nrow(level)
+        return call("hpcgx_symgs32", self._h, level, _order(ordering),
+                    _ptr(r, n, "r", "float32"), _ptr(x, n, "x", "float32"), _stream())
+
+    def mg32(self, r, x, level=0, ordering="multicolor"):
+        n = self.nrow(level)
+        return call("hpcgx_mg32", self._h, level, _order(ordering),
+                    _ptr(r, n, "r", "float32"), _ptr(x, n, "x", "float32"), _stream())
+
+    def cg_mixed(self, b, x, max_iter=50, tolerance=0.0, recompute=10, ordering="multicolor"):
+        """FP64 PCG with the FP32 V-cycle (FP16 values) as preconditioner and
+        r = b - A x recomputed in FP64 every `recompute` iterations."""
+        n = self.nrow(0)
+        ni, nr, nr0 = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        t = (ctypes.c_double * 6)()
+        call("hpcgx_cg_mixed", self._h, _order(ordering), _ptr(b, n, "b"), _ptr(x, n, "x"),
+             int(max_iter), float(tolerance), int(recompute), ctypes.byref(ni),
+             ctypes.byref(nr), ctypes.byref(nr0), t, _stream())
+        return dict(niters=ni.value, normr=nr.value, normr0=nr0.value, times=list(t))
 
     def tune(self, key, value):
         """Set a tuning knob ("wave", "wave_min_rows", "wave_slab_factor",

@@ -200,6 +200,34 @@ int hpcgx_cg(hpcgx_problem *p, int ordering, const double *b, double *x,
              int *niters, double *normr, double *normr0, double *times,
              void *stream);
 
+/* ---- mixed precision -------------------------------------------------------- */
+/* No counterpart in the reference (its specification requires FP64
+ * throughout; this is a reported variant).  The V-cycle runs in FP32 on FP16
+ * matrix values (diagonal slot zeroed) and an FP32 diagonal; vectors of
+ * these calls are FP32 device arrays in device order. */
+
+/* Builds the FP16 values, FP32 diagonal and FP32 coarse vectors of every
+ * level (also done by the first call below).  HPCGX_EUNSUP if an
+ * off-diagonal value overflows FP16. */
+int hpcgx_prepare_mixed(hpcgx_problem *p);
+/* y = A x with the FP16 values and FP32 diagonal (FP32 accumulation). */
+int hpcgx_spmv32(hpcgx_problem *p, int level, const float *x, float *y,
+                 void *stream);
+/* One symmetric Gauss-Seidel sweep in FP32 (ComputeSYMGS_ref.cpp:67-100's
+ * order of rows, FP32 row arithmetic). */
+int hpcgx_symgs32(hpcgx_problem *p, int level, int ordering, const float *r,
+                  float *x, void *stream);
+/* The V-cycle of ComputeMG_ref.cpp:39-63 in FP32. */
+int hpcgx_mg32(hpcgx_problem *p, int level, int ordering, const float *r,
+               float *x, void *stream);
+/* CG.cpp:59-143 in FP64 with the FP32 V-cycle as preconditioner (r rounded
+ * to FP32 on the way in, z widened on the way out); after every
+ * `recompute`-th iteration r = b - A x is recomputed in FP64 (0 = never).
+ * times[0] accumulates the wall time.  Synchronous. */
+int hpcgx_cg_mixed(hpcgx_problem *p, int ordering, const double *b, double *x,
+                   int max_iter, double tolerance, int recompute, int *niters,
+                   double *normr, double *normr0, double *times, void *stream);
+
 /* ---- tuning ----------------------------------------------------------------- */
 
 /* No counterpart in the reference (tuning knobs of this implementation):

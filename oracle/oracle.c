@@ -515,3 +515,164 @@ void or_hpcg_model(int nlevels, const int64_t *nrow, const int64_t *nnz,
   out[8] = fnreads * ((double)ref_max_iters) / ((double)opt_max_iters);
   out[9] = fnwrites * ((double)ref_max_iters) / ((double)opt_max_iters);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Mixed precision (see oracle.h)                                            */
+/* ------------------------------------------------------------------------ */
+/* float -> IEEE binary16 (round to nearest even) -> float, by bit
+ * manipulation (no compiler half type needed). */
+float or_half_round(double v) {
+  float f = (float)v;
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  const uint32_t sign = u & 0x80000000u;
+  uint32_t a = u & 0x7fffffffu;
+  float out;
+  if (a >= 0x7f800000u) { /* inf / nan */
+    return f;
+  }
+  if (a >= 0x477ff000u) { /* rounds to >= 65520: overflow to inf */
+    uint32_t inf = sign | 0x7f800000u;
+    memcpy(&out, &inf, 4);
+    return out;
+  }
+  if (a < 0x33000000u) { /* < 2^-25: rounds to zero */
+    uint32_t z = sign;
+    memcpy(&out, &z, 4);
+    return out;
+  }
+  int e = (int)(a >> 23) - 127;
+  uint32_t h; /* half bits without sign */
+  if (e < -14) { /* subnormal half: value = m * 2^-24 */
+    const uint32_t mant = (a & 0x7fffffu) | 0x800000u;
+    const int shift = -e - 14 + 13; /* bits dropped */
+    uint32_t q = mant >> shift;
+    const uint32_t rem = mant & ((1u << shift) - 1u), half = 1u << (shift - 1);
+    if (rem > half || (rem == half && (q & 1u))) q++;
+    h = q;
+  } else {
+    uint32_t q = ((uint32_t)(e + 15) << 10) | ((a & 0x7fffffu) >> 13);
+    const uint32_t rem = a & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (q & 1u))) q++;
+    h = q;
+  }
+  /* half bits -> float */
+  const uint32_t he = (h >> 10) & 0x1fu, hm = h & 0x3ffu;
+  uint32_t r;
+  if (he == 0) {
+    float m = (float)hm * (1.0f / 16777216.0f); /* hm * 2^-24, exact */
+    memcpy(&r, &m, 4);
+    r |= sign;
+  } else if (he == 31) {
+    r = sign | 0x7f800000u;
+  } else {
+    r = sign | ((he - 15 + 127) << 23) | (hm << 13);
+  }
+  memcpy(&out, &r, 4);
+  return out;
+}
+
+static inline void symgs32_row(int64_t i, const uint8_t *nnz_row,
+                               const int32_t *cols, const double *vals,
+                               const uint8_t *diag_slot, const float *r,
+                               float *x) {
+  const int32_t *c = cols + OR_SLOTS * i;
+  const double *v = vals + OR_SLOTS * i;
+  float sum = r[i];
+  for (int j = 0; j < nnz_row[i]; j++)
+    if (j != diag_slot[i]) sum -= or_half_round(v[j]) * x[c[j]];
+  x[i] = sum / (float)v[diag_slot[i]];
+}
+
+void or_symgs32(int64_t n, const uint8_t *nnz_row, const int32_t *cols,
+                const double *vals, const uint8_t *diag_slot,
+                const int32_t *order, const float *r, float *x) {
+  for (int64_t k = 0; k < n; k++)
+    symgs32_row(order ? order[k] : k, nnz_row, cols, vals, diag_slot, r, x);
+  for (int64_t k = n - 1; k >= 0; k--)
+    symgs32_row(order ? order[k] : k, nnz_row, cols, vals, diag_slot, r, x);
+}
+
+void or_spmv32(int64_t n, const uint8_t *nnz_row, const int32_t *cols,
+               const double *vals, const uint8_t *diag_slot, const float *x,
+               float *y) {
+  for (int64_t i = 0; i < n; i++) {
+    const int32_t *c = cols + OR_SLOTS * i;
+    const double *v = vals + OR_SLOTS * i;
+    float s = 0.0f;
+    for (int j = 0; j < nnz_row[i]; j++)
+      if (j != diag_slot[i]) s += or_half_round(v[j]) * x[c[j]];
+    s += (float)v[diag_slot[i]] * x[i];
+    y[i] = s;
+  }
+}
+
+int or_mg32(or_hier *h, int l, const float *r, float *x) {
+  or_level *L = &h->lv[l];
+  memset(x, 0, sizeof(float) * L->n);
+  if (l + 1 < h->nlevels) {
+    or_symgs32(L->n, L->nnz_row, L->cols, L->vals, L->diag_slot, L->order, r, x);
+    float *ax = (float *)malloc(sizeof(float) * L->n);
+    float *rc = (float *)malloc(sizeof(float) * L->nc);
+    float *xc = (float *)malloc(sizeof(float) * L->nc);
+    or_spmv32(L->n, L->nnz_row, L->cols, L->vals, L->diag_slot, x, ax);
+    for (int64_t c = 0; c < L->nc; c++) rc[c] = r[L->f2c[c]] - ax[L->f2c[c]];
+    or_mg32(h, l + 1, rc, xc);
+    for (int64_t c = 0; c < L->nc; c++) x[L->f2c[c]] += xc[c];
+    or_symgs32(L->n, L->nnz_row, L->cols, L->vals, L->diag_slot, L->order, r, x);
+    free(ax); free(rc); free(xc);
+  } else {
+    or_symgs32(L->n, L->nnz_row, L->cols, L->vals, L->diag_slot, L->order, r, x);
+  }
+  return 0;
+}
+
+int or_cg_mixed(or_hier *h, const double *b, double *x, int max_iter,
+                double tol, int recompute, int *niters, double *normr,
+                double *normr0, double *traj) {
+  or_level *A = &h->lv[0];
+  const int64_t n = A->n;
+  double *r = (double *)malloc(sizeof(double) * n);
+  double *z = (double *)malloc(sizeof(double) * n);
+  double *p = (double *)malloc(sizeof(double) * n);
+  double *Ap = (double *)malloc(sizeof(double) * n);
+  float *r32 = (float *)malloc(sizeof(float) * n);
+  float *z32 = (float *)malloc(sizeof(float) * n);
+  double rtz = 0.0, oldrtz = 0.0, alpha, beta, pAp;
+  *niters = 0;
+  memcpy(p, x, sizeof(double) * n);
+  or_spmv(n, A->nnz_row, A->cols, A->vals, p, Ap);
+  or_waxpby(n, 1.0, b, -1.0, Ap, r);
+  *normr = sqrt(or_dot(n, r, r));
+  *normr0 = *normr;
+  if (traj) traj[0] = 1.0;
+  for (int k = 1; k <= max_iter && *normr / *normr0 > tol; k++) {
+    for (int64_t i = 0; i < n; i++) r32[i] = (float)r[i];
+    or_mg32(h, 0, r32, z32);
+    for (int64_t i = 0; i < n; i++) z[i] = (double)z32[i];
+    if (k == 1) {
+      memcpy(p, z, sizeof(double) * n);
+      rtz = or_dot(n, r, z);
+    } else {
+      oldrtz = rtz;
+      rtz = or_dot(n, r, z);
+      beta = rtz / oldrtz;
+      or_waxpby(n, 1.0, z, beta, p, p);
+    }
+    or_spmv(n, A->nnz_row, A->cols, A->vals, p, Ap);
+    pAp = or_dot(n, p, Ap);
+    alpha = rtz / pAp;
+    or_waxpby(n, 1.0, x, alpha, p, x);
+    if (recompute > 0 && k % recompute == 0) {
+      or_spmv(n, A->nnz_row, A->cols, A->vals, x, Ap);
+      or_waxpby(n, 1.0, b, -1.0, Ap, r);
+    } else {
+      or_waxpby(n, 1.0, r, -alpha, Ap, r);
+    }
+    *normr = sqrt(or_dot(n, r, r));
+    if (traj) traj[k] = *normr / *normr0;
+    *niters = k;
+  }
+  free(r); free(z); free(p); free(Ap); free(r32); free(z32);
+  return 0;
+}

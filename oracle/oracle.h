@@ -141,6 +141,34 @@ void or_hpcg_model(int nlevels, const int64_t *nrow, const int64_t *nnz,
                    int nsets, int ref_max_iters, int opt_max_iters,
                    double *out);
 
+/* ---- mixed precision (no counterpart in the reference) -------------------
+ * The V-cycle in FP32 with matrix values rounded to FP16 (double -> float ->
+ * half, round to nearest even each time) and the diagonal kept separately
+ * in FP32.  Row arithmetic of the FP32 kernels:
+ *   SYMGS row:   sum = r_i; for each stored j != diag: sum -= h(a_ij) * x_j;
+ *                x_i = sum / d_i                  (d_i = (float)a_ii)
+ *   A*x row:     s = 0; for each stored j != diag: s += h(a_ij) * x_j;
+ *                s += d_i * x_i
+ *   restriction: rc = r[f] - (A*x)_f (fused, only at injected rows)
+ * These orders are this project's own; they are pinned against the FP64
+ * functions and dense algebra (tests/test_oracle.py), not against the
+ * reference. */
+float or_half_round(double v);
+void or_symgs32(int64_t n, const uint8_t *nnz_row, const int32_t *cols,
+                const double *vals, const uint8_t *diag_slot,
+                const int32_t *order, const float *r, float *x);
+void or_spmv32(int64_t n, const uint8_t *nnz_row, const int32_t *cols,
+               const double *vals, const uint8_t *diag_slot, const float *x,
+               float *y);
+/* V-cycle of ComputeMG_ref.cpp:39-63 in FP32 (orders as set on the hierarchy). */
+int or_mg32(or_hier *h, int level, const float *r, float *x);
+/* CG_ref.cpp:58-144 with M^-1 = the FP32 V-cycle (r rounded to float on the
+ * way in, z widened on the way out) and the residual recomputed as
+ * r = b - A x in FP64 after every `recompute`-th iteration (0 = never). */
+int or_cg_mixed(or_hier *h, const double *b, double *x, int max_iter,
+                double tol, int recompute, int *niters, double *normr,
+                double *normr0, double *traj);
+
 #ifdef __cplusplus
 }
 #endif

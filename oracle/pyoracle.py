@@ -265,3 +265,69 @@ class Hierarchy:
                               _p(np.ascontiguousarray(yr, np.float64)), _p(out))
         return dict(depsym_spmv=out[0], depsym_mg=out[1], count_fail=int(out[2]),
                     residuals=(out[3], out[4]))
+
+
+# ---- mixed precision ---------------------------------------------------------
+_MIXED_SIGS = {
+    "or_half_round": (ctypes.c_float, [ctypes.c_double]),
+    "or_symgs32": (None, [ctypes.c_int64] + [ctypes.c_void_p] * 7),
+    "or_spmv32": (None, [ctypes.c_int64] + [ctypes.c_void_p] * 6),
+    "or_mg32": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "or_cg_mixed": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                   ctypes.c_double, ctypes.c_int, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+}
+
+
+def _mixed_lib():
+    L = lib()
+    if not getattr(L, "_mixed_sigs", False):
+        for name, (res, args) in _MIXED_SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        L._mixed_sigs = True
+    return L
+
+
+def half_round(v):
+    return _mixed_lib().or_half_round(float(v))
+
+
+def symgs32(m, r, x, order=None):
+    r = np.ascontiguousarray(r, np.float32)
+    x = np.array(x, np.float32, copy=True)
+    o = None if order is None else np.ascontiguousarray(order, np.int32)
+    _mixed_lib().or_symgs32(x.size, _p(m["nnz_row"]), _p(m["cols"]), _p(m["vals"]),
+                            _p(m["diag_slot"]), _p(o), _p(r), _p(x))
+    return x
+
+
+def spmv32(m, x):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.zeros(m["nnz_row"].size, np.float32)
+    _mixed_lib().or_spmv32(y.size, _p(m["nnz_row"]), _p(m["cols"]), _p(m["vals"]),
+                           _p(m["diag_slot"]), _p(x), _p(y))
+    return y
+
+
+def _hier_mg32(self, r, level=0):
+    r = np.ascontiguousarray(r, np.float32)
+    x = np.zeros(self.nrow(level), np.float32)
+    _mixed_lib().or_mg32(self._h, level, _p(r), _p(x))
+    return x
+
+
+def _hier_cg_mixed(self, b=None, x=None, max_iter=50, tol=0.0, recompute=10):
+    b = self.b.copy() if b is None else np.ascontiguousarray(b, np.float64)
+    x = np.zeros(self.nrow(0)) if x is None else np.array(x, np.float64, copy=True)
+    traj = np.zeros(max_iter + 1)
+    ni, nr, nr0 = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+    _mixed_lib().or_cg_mixed(self._h, _p(b), _p(x), max_iter, tol, recompute, ctypes.byref(ni),
+                             ctypes.byref(nr), ctypes.byref(nr0), _p(traj))
+    return dict(x=x, niters=ni.value, normr=nr.value, normr0=nr0.value,
+                traj=traj[: ni.value + 1])
+
+
+Hierarchy.mg32 = _hier_mg32
+Hierarchy.cg_mixed = _hier_cg_mixed
