@@ -1,0 +1,169 @@
+// DeviceProblem.cpp -- device problem lifetime and host<->device copies for
+// the C++ surface.
+#include "DeviceProblem.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <chrono>
+#include <cstdio>
+#include <vector>
+
+namespace hpcg_amd {
+
+int optimized_ordering = HPCGX_ORDER_MULTICOLOR;
+bool cg_phase_timers = false;
+
+namespace {
+DeviceContext *g_current = nullptr;
+
+int hip_report(const char *where, hipError_t e) {
+  std::fprintf(stderr, "hpcg_amd: %s: %s\n", where, hipGetErrorString(e));
+  return HPCGX_EHIP;
+}
+
+void free_context(DeviceContext *c) {
+  if (!c) return;
+  if (c->stage) (void)hipFree(c->stage);
+  for (double *v : c->vec)
+    if (v) (void)hipFree(v);
+  if (c->problem) (void)hpcgx_problem_destroy(c->problem);
+  if (g_current == c) g_current = nullptr;
+  delete c;
+}
+}  // namespace
+
+int Report(const char *where, int rc) {
+  std::fprintf(stderr, "hpcg_amd: %s: %s\n", where, hpcgx_last_error());
+  return rc ? rc : HPCGX_EHIP;
+}
+
+DeviceContext *EnsureDeviceProblem(SparseMatrix &A) {
+  if (A.optimizationData) return static_cast<DeviceLevel *>(A.optimizationData)->ctx;
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<hpcgx_host_level> levels;
+  std::vector<SparseMatrix *> mats;
+  for (SparseMatrix *M = &A; M; M = M->Ac) {
+    if (levels.size() == 8) {
+      std::fprintf(stderr, "hpcg_amd: OptimizeProblem: more than 8 levels\n");
+      return nullptr;
+    }
+    hpcgx_host_level h{};
+    h.nx = M->geom->nx;
+    h.ny = M->geom->ny;
+    h.nz = M->geom->nz;
+    h.nrow = M->localNumberOfRows;
+    h.nonzerosInRow = M->nonzerosInRow;
+    h.mtxIndL = M->mtxIndL;
+    h.matrixValues = M->matrixValues;
+    h.matrixDiagonal = M->matrixDiagonal;
+    h.f2cOperator = (M->Ac && M->mgData) ? M->mgData->f2cOperator : nullptr;
+    levels.push_back(h);
+    mats.push_back(M);
+    if (M->Ac && !M->mgData) break;  // a coarse level without transfer data ends the chain
+  }
+  auto *c = new DeviceContext{};
+  int rc = hpcgx_optimize_problem(levels.data(), (int)levels.size(), 0, &c->problem);
+  if (rc) {
+    Report("OptimizeProblem", rc);
+    free_context(c);
+    return nullptr;
+  }
+  c->nlevels = (int)levels.size();
+  c->fine = &A;
+  for (int l = 0; l < c->nlevels; ++l) {
+    c->nrow[l] = levels[l].nrow;
+    c->levels[l] = DeviceLevel{c, l};
+  }
+  const size_t bytes = sizeof(double) * (size_t)c->nrow[0];
+  hipError_t e = hipMalloc(&c->stage, bytes);
+  for (int k = 0; k < 4 && e == hipSuccess; ++k) e = hipMalloc(&c->vec[k], bytes);
+  if (e != hipSuccess) {
+    hip_report("OptimizeProblem: scratch vectors", e);
+    free_context(c);
+    return nullptr;
+  }
+  for (int l = 0; l < c->nlevels; ++l) mats[l]->optimizationData = &c->levels[l];
+  c->build_seconds =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  g_current = c;
+  return c;
+}
+
+const DeviceLevel *LevelOf(const SparseMatrix &A) {
+  return static_cast<const DeviceLevel *>(A.optimizationData);
+}
+
+const DeviceLevel *LevelFor(const SparseMatrix &A, const char *where) {
+  if (const DeviceLevel *L = LevelOf(A)) return L;
+  if (A.geom && A.localNumberOfRows > 0 && A.nonzerosInRow) {
+    // the caller's matrix stays logically const: only optimizationData,
+    // the slot the reference reserves for this, is written
+    if (EnsureDeviceProblem(const_cast<SparseMatrix &>(A))) return LevelOf(A);
+  }
+  std::fprintf(stderr, "hpcg_amd: %s: matrix has no device problem\n", where);
+  return nullptr;
+}
+
+DeviceContext *CurrentContext() { return g_current; }
+
+int Upload(const DeviceLevel &L, const Vector &v, double *dev) {
+  const int64_t n = L.ctx->nrow[L.level];
+  if (v.localLength < n) {
+    std::fprintf(stderr, "hpcg_amd: vector of %d values on a level of %lld rows\n",
+                 v.localLength, (long long)n);
+    return HPCGX_EINVAL;
+  }
+  hipError_t e = hipMemcpy(L.ctx->stage, v.values, sizeof(double) * n, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_report("host to device copy", e);
+  int rc = hpcgx_to_device_order(L.ctx->problem, L.level, L.ctx->stage, dev, nullptr);
+  return rc ? Report("to_device_order", rc) : 0;
+}
+
+int Download(const DeviceLevel &L, const double *dev, Vector &v) {
+  const int64_t n = L.ctx->nrow[L.level];
+  if (v.localLength < n) {
+    std::fprintf(stderr, "hpcg_amd: vector of %d values on a level of %lld rows\n",
+                 v.localLength, (long long)n);
+    return HPCGX_EINVAL;
+  }
+  int rc = hpcgx_to_natural_order(L.ctx->problem, L.level, dev, L.ctx->stage, nullptr);
+  if (rc) return Report("to_natural_order", rc);
+  hipError_t e = hipMemcpy(v.values, L.ctx->stage, sizeof(double) * n, hipMemcpyDeviceToHost);
+  return e == hipSuccess ? 0 : hip_report("device to host copy", e);
+}
+
+int UploadRaw(const double *host, double *dev, int64_t n) {
+  hipError_t e = hipMemcpy(dev, host, sizeof(double) * n, hipMemcpyHostToDevice);
+  return e == hipSuccess ? 0 : hip_report("host to device copy", e);
+}
+
+int DownloadRaw(const double *dev, double *host, int64_t n) {
+  hipError_t e = hipMemcpy(host, dev, sizeof(double) * n, hipMemcpyDeviceToHost);
+  return e == hipSuccess ? 0 : hip_report("device to host copy", e);
+}
+
+}  // namespace hpcg_amd
+
+using namespace hpcg_amd;
+
+void MatrixReplaceDiagonalOnDevice(SparseMatrix &A, const Vector &diagonal) {
+  const DeviceLevel *L = LevelOf(A);
+  if (!L) return;  // no device copy yet: OptimizeProblem will read the host values
+  double *d = L->ctx->vec[3];
+  int rc = Upload(*L, diagonal, d);
+  if (!rc) rc = hpcgx_replace_diagonal(L->ctx->problem, L->level, d, nullptr);
+  if (rc) {
+    Report("ReplaceMatrixDiagonal", rc);
+    std::abort();  // the device matrix would silently differ from the host one
+  }
+}
+
+void MatrixReleaseDevice(SparseMatrix &A) {
+  const DeviceLevel *L = LevelOf(A);
+  if (!L) return;
+  DeviceContext *c = L->ctx;
+  if (c->fine != &A) return;  // released with the fine level
+  for (SparseMatrix *M = &A; M; M = M->Ac) M->optimizationData = nullptr;
+  (void)hipDeviceSynchronize();
+  free_context(c);
+}

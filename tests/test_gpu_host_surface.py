@@ -1,0 +1,185 @@
+"""The C++ surface (hpcg_amd/host: ComputeSPMV, ComputeSYMGS, ComputeMG,
+ComputeDotProduct, ComputeWAXPBY, CG, OptimizeProblem, Copy/Replace-
+MatrixDiagonal) and the xhpcg_amd driver, against the oracle.
+
+The surface runs in its own process (tests/host/_bin/surface_check, built by
+__graft_entry__.build()); inputs and outputs cross through files.  Bars:
+SpMV, SYMGS and MG (multicolour order, the optimized ordering), WAXPBY and
+the diagonal round trip bit-exact; dot within 1e-14 of sum|x_i y_i| (fixed
+tree summation); 10 CG iterations within 1e-9 relative (the dots'
+reassociation is the only difference).  The driver's reference tolerance
+(50 natural-order CG iterations) must equal the oracle's within 1e-6
+relative at 64^3 (|log| growth of the same reassociation over 50
+iterations; the value itself is ~1e-11).
+"""
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HARNESS = os.path.join(ROOT, "tests", "host", "_bin", "surface_check")
+DRIVER = os.path.join(ROOT, "hpcg_amd", "xhpcg_amd")
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _gpu_visible():
+    try:
+        import torch
+        return torch.cuda.device_count() > 0
+    except Exception:
+        return False
+
+
+def _run(cmd, timeout):
+    say(" ".join(cmd))
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+    print(r.stdout[-4000:], r.stderr[-2000:], flush=True)
+    return r
+
+
+@pytest.mark.skipif(_gpu_visible(), reason="checks the behaviour without a GPU")
+def test_driver_fails_loudly_without_gpu():
+    if not os.path.exists(DRIVER):
+        pytest.skip("xhpcg_amd not built")
+    r = subprocess.run([DRIVER, "16", "16", "16"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 3
+    assert "no CPU path" in r.stderr
+
+
+def test_driver_rejects_bad_sizes():
+    if not os.path.exists(DRIVER):
+        pytest.skip("xhpcg_amd not built")
+    r = subprocess.run([DRIVER, "12", "16", "16"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "multiples of 8" in r.stderr
+    r = subprocess.run([DRIVER, "8", "16", "128"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 127 and "too stretched" in r.stderr
+
+
+def _read(d, name, n):
+    v = np.fromfile(os.path.join(d, name), np.float64)
+    assert v.size == n, name
+    return v
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dims", [(16, 16, 16), (16, 24, 32)])
+def test_cpp_surface_matches_oracle(dims, tmp_path):
+    assert os.path.exists(HARNESS), "surface_check not built (run __graft_entry__.build())"
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    rng = np.random.default_rng(nx * 1000 + ny * 10 + nz)
+    x, y, r = rng.random(n) + 0.5, rng.random(n) - 0.5, rng.random(n) * 2.0
+    for name, v in (("x", x), ("y", y), ("r", r)):
+        v.tofile(str(tmp_path / (name + ".bin")))
+    res = _run([HARNESS, str(nx), str(ny), str(nz), str(tmp_path)], timeout=300)
+    assert res.returncode == 0, res.stderr
+    d = str(tmp_path)
+
+    H = O.Hierarchy(nx, ny, nz)
+    H.use_color_order()                      # the optimized (multicolour) ordering
+    L0 = {k: np.array(v) for k, v in H.level(0).items()}
+    c0, _ = O.colors(L0)
+    order0 = O.color_order(c0)
+
+    assert np.array_equal(_read(d, "spmv.bin", n), O.spmv(L0, x))
+    assert np.array_equal(_read(d, "symgs.bin", n), O.symgs(L0, r, x, order0))
+    assert np.array_equal(_read(d, "mg.bin", n), H.mg(r))
+    assert np.array_equal(_read(d, "waxpby.bin", n), O.waxpby(2.5, x, -0.75, y))
+    assert np.array_equal(_read(d, "waxpby_a1.bin", n), O.waxpby(1.0, x, 0.3, y))
+    assert np.array_equal(_read(d, "waxpby_alias.bin", n), O.waxpby(0.7, y, 1.0, x))
+
+    sc = {}
+    with open(os.path.join(d, "scalars.txt")) as f:
+        for line in f:
+            k, *vals = line.split()
+            sc[k] = [float(v) for v in vals]
+    for key, a, b in (("dot_xy", x, y), ("dot_xx", x, x)):
+        exact = float(np.sum(a.astype(np.longdouble) * b.astype(np.longdouble)))
+        assert abs(sc[key][0] - exact) <= 1e-14 * float(np.sum(np.abs(a * b)))
+
+    n1 = H.nrow(1)
+    L1 = {k: np.array(v) for k, v in H.level(1).items()}
+    assert np.array_equal(_read(d, "spmv1.bin", n1), O.spmv(L1, x[:n1]))
+    assert np.array_equal(_read(d, "mg1.bin", n1), H.mg(x[:n1], level=1))
+
+    # doubled diagonal (diagonal slot of every row), then restored
+    rows = np.arange(n)
+    slot = L0["diag_slot"].astype(np.int64)
+    L0d = dict(L0)
+    vals = L0["vals"].copy()
+    vals[rows * 27 + slot] *= 2.0            # oracle rows hold 27 slots each
+    L0d["vals"] = vals
+    assert np.array_equal(_read(d, "spmv_diag2.bin", n), O.spmv(L0d, x))
+    live = H.level(0)
+    live["vals"][rows * 27 + slot] *= 2.0
+    try:
+        assert np.array_equal(_read(d, "mg_diag2.bin", n), H.mg(r))
+    finally:
+        live["vals"][rows * 27 + slot] /= 2.0
+    assert np.array_equal(_read(d, "spmv_restored.bin", n), O.spmv(L0, x))
+
+    for key, prec in (("cg10", True), ("cg10_noprec", False)):
+        ref = H.cg(max_iter=10, tol=0.0, precond=prec)
+        it, normr, normr0 = sc[key]
+        assert int(it) == 10
+        assert abs(normr0 - ref["normr0"]) <= 1e-12 * ref["normr0"]
+        assert abs(normr / normr0 - ref["traj"][-1]) <= 1e-9 * ref["traj"][-1]
+        got = _read(d, key + ".bin", n)
+        assert np.max(np.abs(got - ref["x"])) <= 1e-9 * np.max(np.abs(ref["x"]))
+    it, normr, normr0 = sc["cg_tol"]
+    assert normr / normr0 <= 1e-8 and it < 500
+    assert sc["memory"][0] > 12.0 * 27 * n  # at least the fine ELL
+
+
+def _field(out, pattern):
+    m = re.search(pattern, out)
+    assert m, "missing in driver output: " + pattern
+    return m.groups()
+
+
+@pytest.mark.gpu
+def test_driver_valid_run_matches_reference_tolerance():
+    assert os.path.exists(DRIVER), "xhpcg_amd not built (run __graft_entry__.build())"
+    res = _run([DRIVER, "64", "64", "64"], timeout=600)
+    assert res.returncode == 0, res.stderr
+    out = res.stdout
+    assert "HPCG result is VALID" in out
+    it, ref_tol = _field(out, r"Reference CG: (\d+) iterations, scaled residual (\S+)")
+    assert int(it) == 50
+    oracle = O.Hierarchy(64, 64, 64).cg(max_iter=50, tol=0.0, precond=True)
+    want = oracle["normr"] / oracle["normr0"]
+    assert abs(float(ref_tol) - want) <= 1e-6 * want, (ref_tol, want)
+    passed, failed, mx_no, mx_p = map(int, _field(
+        out, r"TestCG passed = (\d+), failed = (\d+), max iterations without / with "
+             r"preconditioning = (\d+) / (\d+)"))
+    assert (passed, failed) == (4, 0) and mx_no <= 12 and mx_p <= 2
+    spmv_dep, mg_dep = map(float, _field(out, r"departure from symmetry SpMV = (\S+), MG = (\S+),"))
+    assert spmv_dep <= 1.0 and mg_dep <= 1.0
+    for k in (0, 1):
+        assert float(_field(out, r"SpMV call \[%d\] Residual \[(\S+)\]" % k)[0]) == 0.0
+    opt_it = int(_field(out, r"Optimized CG: (\d+) iterations")[0])
+    assert 1 <= opt_it <= 500
+    rating = float(_field(out, r"GFLOP/s rating of = (\S+)")[0])
+    assert rating > 0.0
+
+
+@pytest.mark.gpu
+def test_driver_lexicographic_ordering():
+    assert os.path.exists(DRIVER), "xhpcg_amd not built (run __graft_entry__.build())"
+    res = _run([DRIVER, "16", "16", "24", "--ordering=lexicographic", "--phase-timers"],
+               timeout=300)
+    assert res.returncode == 0, res.stderr
+    assert "HPCG result is VALID" in res.stdout
+    # the natural ordering is the reference's: 50 iterations reach its tolerance
+    assert int(_field(res.stdout, r"Optimized CG: (\d+) iterations")[0]) == 50
+    assert "raw MG =" in res.stdout
