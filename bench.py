@@ -43,6 +43,14 @@ def parse():
                     help="FP64 vs mixed-precision time to this residual (0 = skip)")
     ap.add_argument("--max-iters", type=int, default=2000,
                     help="iteration cap of the time-to-tolerance runs")
+    ap.add_argument("--col16", type=int, default=1, choices=[0, 1],
+                    help="16-bit column ids (tile base + offset) where built")
+    ap.add_argument("--band-lines", type=int, default=0,
+                    help="grid lines per band of the in-colour row order (0 = default, 1 = natural)")
+    ap.add_argument("--xcd-remap", type=int, default=0, choices=[0, 1])
+    ap.add_argument("--spmv-dot-blocks", type=int, default=0, help="0 = library default")
+    ap.add_argument("--stream-min-bytes", type=float, default=0,
+                    help="non-temporal matrix loads above this matrix size (0 = library default)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device (the known-size kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -142,10 +150,16 @@ def main():
     log("generate %dx%dx%d on device" % (a.nx, a.ny, a.nz))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    P = hpcg_amd.Problem.generate(a.nx, a.ny, a.nz, nlevels=4)
+    P = hpcg_amd.Problem.generate(a.nx, a.ny, a.nz, nlevels=4, band_lines=a.band_lines)
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
     b, x, _ = P.vectors()
+    P.tune("col16", a.col16)
+    P.tune("xcd_remap", a.xcd_remap)
+    if a.spmv_dot_blocks:
+        P.tune("spmv_dot_blocks", a.spmv_dot_blocks)
+    if a.stream_min_bytes:
+        P.tune("stream_min_bytes", a.stream_min_bytes)
     log("setup %.2f s, device memory %.1f GB" % (setup_s, P.memory_use() / 1e9))
 
     if a.warmup > 0:
@@ -173,9 +187,16 @@ def main():
     # dominant kernel: the finest level's multicolour Gauss-Seidel launches
     # (two symmetric sweeps per iteration: pre- and post-smoother)
     info0 = P.info(0)
-    kname = ("k_gs_wave<true,true> (finest-level wavefront sweep, one direction)"
-             if info0["wave"] else "k_gs_range<true,true> (finest-level colour step)")
+    c16 = bool(a.col16 and info0["col16"])
+    ksym = ("k_gs_wave<true,true>" if info0["wave"] else
+            "k_gs_range16<true,true>" if c16 else "k_gs_range<true,true>")
+    kname = ksym + (" (finest-level wavefront sweep, one direction)" if info0["wave"]
+                    else " (finest-level colour step)")
     per_launch = K * 2 * cb["symgs_L0"] / max(ks_n, 1)
+    # the same sweep in the stored format: every ELL slot (padding included)
+    # with 8-byte values and 2- (16-bit ids) or 4-byte column ids
+    a_st = 8 + (2 if c16 else 4)
+    stored = K * 2 * (2 * a_st * 27 * lv[0]["n"] + 6 * 8 * lv[0]["n"]) / max(ks_n, 1)
     launch_ms = ks_ms / max(ks_n, 1)
     achieved = per_launch / (launch_ms * 1e-3) * 1e-9 if ks_n else None
     rating = flops["total"] / (dt + setup_s / 10.0) * 1e-9  # ReportResults.cpp:353, one set
@@ -184,10 +205,9 @@ def main():
     traffic = None
     try:
         pm = json.load(open(os.path.join(ROOT, "profiles", "pmc_traffic.json")))
-        kk = "k_gs_range<true,true>"
-        if (not info0["wave"] and pm["workload"] == "%dx%dx%d fp64 %s" % (a.nx, a.ny, a.nz, a.ordering)
-                and kk in pm["kernels"]):
-            traffic = pm["kernels"][kk]["traffic_bytes"]
+        if (pm["workload"] == "%dx%dx%d fp64 %s" % (a.nx, a.ny, a.nz, a.ordering)
+                and ksym in pm["kernels"]):
+            traffic = pm["kernels"][ksym]["traffic_bytes"]
     except (OSError, ValueError, KeyError):
         traffic = None
 
@@ -222,6 +242,8 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
             "traffic": round(traffic) if traffic else None,
             "bytes_per_launch": per_launch,
+            "stored_format_bytes_per_launch": stored,
+            "column_ids": "16-bit" if c16 else "32-bit",
             "mean_launch_ms": round(launch_ms, 5),
             "launches": ks_n,
         },

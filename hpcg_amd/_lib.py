@@ -38,7 +38,8 @@ class LevelInfo(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
                 ("nrow", ctypes.c_int64), ("nnz", ctypes.c_int64),
                 ("ncolors", ctypes.c_int), ("nlevels_sched", ctypes.c_int),
-                ("ncoarse", ctypes.c_int64), ("wave", ctypes.c_int), ("nslabs", ctypes.c_int)]
+                ("ncoarse", ctypes.c_int64), ("wave", ctypes.c_int), ("nslabs", ctypes.c_int),
+                ("col16", ctypes.c_int), ("wide_tiles", ctypes.c_int64)]
 
 
 # name -> (restype, argtypes)

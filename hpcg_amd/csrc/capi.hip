@@ -98,7 +98,7 @@ void alloc_solver_state(hpcgx::Problem &P) {
   P.z = P.arena.alloc<double>(n);
   P.p = P.arena.alloc<double>(n);
   P.Ap = P.arena.alloc<double>(n);
-  P.partials = P.arena.alloc<double>(hpcgx::kReduceBlocks);
+  P.partials = P.arena.alloc<double>(hpcgx::kMaxPartials);
   P.ticket = P.arena.alloc<unsigned>(1);
   P.scalars = P.arena.alloc<double>(hpcgx::kNumScalars);
   P.wave_err = P.arena.alloc<int>(1);
@@ -243,6 +243,8 @@ int hpcgx_problem_info(const hpcgx_problem *p, int level, hpcgx_level_info *info
     info->ncoarse = L.nc;
     info->wave = (L.wave && p->P.use_wave) ? 1 : 0;
     info->nslabs = L.nslabs;
+    info->col16 = L.c16 ? 1 : 0;
+    info->wide_tiles = L.wide_tiles;
   });
 }
 
@@ -304,7 +306,7 @@ int hpcgx_copy_diagonal(const hpcgx_problem *p, int level, double *diag, void *s
 }
 
 int hpcgx_spmv(const hpcgx_problem *p, int level, const double *x, double *y, void *stream) {
-  return guard([&] { hpcgx::launch_spmv(level_of(p, level), x, y, S(stream)); });
+  return guard([&] { hpcgx::launch_spmv(p->P, level_of(p, level), x, y, S(stream)); });
 }
 
 int hpcgx_symgs(const hpcgx_problem *p, int level, int ordering, const double *r,
@@ -386,6 +388,16 @@ int hpcgx_tune(hpcgx_problem *p, const char *key, double value) {
     if (!p || !key) throw Error(HPCGX_EINVAL, "NULL argument");
     const std::string k(key);
     hpcgx::Problem &P = p->P;
+    if (k == "col16") {
+      P.use_c16 = value != 0.0;
+      return;
+    }
+    if (k == "spmv_dot_blocks") {
+      if (value < 8 || value > hpcgx::kMaxPartials)
+        throw Error(HPCGX_EINVAL, "spmv_dot_blocks must be in [8, 8192]");
+      P.spmv_dot_blocks = (int)value / 8 * 8;
+      return;
+    }
     if (k == "wave") {
       P.use_wave = value != 0.0;
       if (P.use_wave)
@@ -469,7 +481,7 @@ int hpcgx_spmv32(hpcgx_problem *p, int level, const float *x, float *y, void *st
   return guard([&] {
     hpcgx::Level &L = level_of(p, level);
     hpcgx::prepare_mixed(p->P);
-    hpcgx::spmv32(L, x, y, S(stream));
+    hpcgx::spmv32(p->P, L, x, y, S(stream));
   });
 }
 

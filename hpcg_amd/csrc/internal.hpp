@@ -18,10 +18,13 @@ constexpr int kMaxLevels = 8;
 constexpr int kMaxColors = 32;    // first-fit colours of a <=27-entry row: <=27
 constexpr int kBlock = 256;       // threads per workgroup for streaming kernels
 constexpr int kReduceBlocks = 1024;  // partial sums of a grid reduction
+constexpr int kMaxPartials = 8192;   // partial-sum slots (largest reducing grid)
 constexpr double kStreamBytes = 512e6;  // matrices above this are streamed
 constexpr int64_t kWaveMinRows = 1 << 23;  // levels swept as a wavefront
 constexpr int kWaveSlabFactor = 2;         // slab width / matrix bandwidth
 constexpr int kWaveRows = 2;               // rows per thread of a wavefront task
+constexpr int kTile = 64;  // rows sharing 16-bit column bases (one wavefront)
+constexpr int kBandLines = 1;  // grid lines per band of the in-colour order (1 = natural)
 
 struct Error : std::runtime_error {
   int code;
@@ -92,6 +95,8 @@ struct Level {
   int32_t *perm = nullptr;  // natural row -> device row
   int32_t *iperm = nullptr; // device row -> natural row
   // multicolour schedule: colour c owns device rows [color_ptr[c], color_ptr[c+1])
+  // in natural order (natural_in_colour) or band by band (k_band_key)
+  bool natural_in_colour = true;
   int ncolors = 0;
   std::vector<int64_t> color_ptr;
   // lexicographic schedule: level s owns level_rows[level_ptr[s] .. level_ptr[s+1])
@@ -117,8 +122,24 @@ struct Level {
   // mixed precision (built on first use): FP16 values with the diagonal slot
   // zeroed, the FP32 diagonal, FP32 coarse-level vectors
   _Float16 *val16 = nullptr;
+  uint32_t *vc = nullptr;  // c16 levels: col16 << 16 | FP16 bits, one word per slot
   float *diag32 = nullptr;
   float *rc32 = nullptr, *xc32 = nullptr;
+  // 16-bit column ids.  Tile t = device rows [kTile t, kTile t + kTile).  The
+  // slots of a tile are aligned: slot j holds the tile's j-th distinct
+  // natural column offset (col - row in natural numbering, ascending); a row
+  // without that offset holds a valid column and the value 0 there, so each
+  // row still meets its stored entries in ascending column order and the
+  // extra terms add exact zeros.  Column of slot j of row i in tile t:
+  // cbase[j * ntiles + t] + col16[j * n + i], unless tinfo[t] != 0 (more
+  // than 27 distinct offsets, or a slot spanning more than 65535 columns:
+  // the tile keeps its 32-bit ids, which `col` holds for every tile).
+  bool c16 = false;
+  int64_t ntiles = 0;
+  int64_t wide_tiles = 0;
+  uint16_t *col16 = nullptr;
+  int32_t *cbase = nullptr;
+  int32_t *tinfo = nullptr;
   // transfer to the next level: f2c[coarse device row] = fine device row
   int64_t nc = 0;
   int32_t *f2c = nullptr;
@@ -140,6 +161,8 @@ struct Problem {
   int *wave_err = nullptr;    // set when a wavefront wait gives up
   int wave_grid = 0;          // co-resident workgroups of the wavefront kernel
   bool use_wave = false;      // multicolour sweeps as wavefronts (measured slower)
+  bool use_c16 = true;        // read 16-bit column ids where a level has them
+  int spmv_dot_blocks = 4096; // persistent grid of the fused SpMV + dot
   bool mixed_ready = false;   // FP16/FP32 copies built
   float *r32 = nullptr, *z32 = nullptr;  // FP32 input / output of the V-cycle
   // event log of the finest level's multicolour Gauss-Seidel launches
@@ -153,6 +176,21 @@ struct Problem {
   } klog;
 };
 
+// Kernel view of a level's 16-bit column ids.  Kernels that use it start
+// every wavefront at a multiple of kTile rows, so a wavefront's rows share
+// one tile and its bases are wave-uniform (read through the scalar cache).
+struct Col16 {
+  const uint16_t *off;
+  const int32_t *base;
+  const int32_t *info;
+  int64_t ntiles;
+};
+inline Col16 col16_of(const Level &L) { return Col16{L.col16, L.cbase, L.tinfo, L.ntiles}; }
+// 16-bit kernels apply to rows [begin, ...) of L when this holds.
+inline bool use_col16(const Problem &P, const Level &L, int64_t begin) {
+  return P.use_c16 && L.c16 && begin % kTile == 0;
+}
+
 // Slots of Problem::scalars used by CG.
 enum CgScalar { kRtz = 0, kOldRtz, kPAp, kAlpha, kBeta, kNormr2, kDot, kNumScalars };
 
@@ -161,8 +199,9 @@ void launch_fill(double *x, int64_t n, double v, hipStream_t s);
 void launch_copy(const double *x, double *y, int64_t n, hipStream_t s);
 void launch_gather(const int32_t *idx, const double *src, double *dst,
                    int64_t n, hipStream_t s);
-void launch_spmv(const Level &L, const double *x, double *y, hipStream_t s);
-void launch_symgs_rows(const Level &L, int64_t begin, int64_t end,
+void launch_spmv(const Problem &P, const Level &L, const double *x, double *y,
+                 hipStream_t s);
+void launch_symgs_rows(const Problem &P, const Level &L, int64_t begin, int64_t end,
                        const double *r, double *x, hipStream_t s);
 void launch_symgs_list(const Level &L, const int32_t *rows, int64_t count,
                        const double *r, double *x, hipStream_t s);
@@ -220,7 +259,7 @@ void prepare_mixed(Problem &P);
 void replace_diag32(Level &L, const double *d, hipStream_t s);
 void symgs32(Problem &P, int l, int ordering, const float *r, float *x,
              hipStream_t s);
-void spmv32(const Level &L, const float *x, float *y, hipStream_t s);
+void spmv32(const Problem &P, const Level &L, const float *x, float *y, hipStream_t s);
 void mg32(Problem &P, int l, int ordering, const float *r, float *x,
           hipStream_t s);
 void to_f32(int64_t n, const double *in, float *out, hipStream_t s);

@@ -108,21 +108,30 @@ __device__ inline double row_dot(const int32_t *__restrict__ col,
 // colour blocks (chunk k of every colour before chunk k+1 of any), so the x
 // values gathered from all colour blocks stay within a narrow window.
 template <bool STREAM>
+__device__ inline double row_dot16(const Col16 &C, const int32_t *__restrict__ col,
+                                   const double *__restrict__ val,
+                                   const double *__restrict__ x, int64_t n,
+                                   int64_t i);
+
+// C16: 16-bit column ids (chunks start at multiples of kTile).
+template <bool STREAM, bool C16>
 __global__ __launch_bounds__(kBlock) void k_spmv(
-    const int64_t *__restrict__ chunks, int64_t n,
+    const int64_t *__restrict__ chunks, int64_t n, Col16 C,
     const int32_t *__restrict__ col, const double *__restrict__ val,
     const double *__restrict__ x, double *__restrict__ y, bool xcd) {
   const int64_t b = xcd_block(blockIdx.x, gridDim.x, xcd);
   const int64_t i = chunks[2 * b] + threadIdx.x;
-  if (i < chunks[2 * b + 1]) y[i] = row_dot<STREAM>(col, val, x, n, i);
+  if (i < chunks[2 * b + 1])
+    y[i] = C16 ? row_dot16<STREAM>(C, col, val, x, n, i) : row_dot<STREAM>(col, val, x, n, i);
 }
 
 // One Gauss-Seidel row update in the reference's form
 // (ComputeSYMGS_ref.cpp:71-80): subtract every stored term including the
 // diagonal one, add the diagonal term back, divide by the diagonal.  The
-// diagonal value is taken from the row's own slots (the first one whose
-// column is the row: unused slots, which also point at the row, come after
-// it), so no separate diagonal array is read.
+// diagonal value is taken from the row's own slots: the stored entry whose
+// column is the row (unused slots hold the value 0, and a stored zero
+// diagonal would make the reference divide by zero as well), so no separate
+// diagonal array is read.
 template <bool STREAM>
 __device__ inline void gs_row(int64_t i, int64_t n,
                               const int32_t *__restrict__ col,
@@ -130,17 +139,62 @@ __device__ inline void gs_row(int64_t i, int64_t n,
                               const double *__restrict__ r, double *x) {
   double sum = r[i];
   double d = 0.0;
-  bool have = false;
 #pragma unroll
   for (int j = 0; j < kSlots; ++j) {
     const int64_t k = (int64_t)j * n + i;
     const double v = mload<STREAM>(val + k);
     const int32_t c = mload<STREAM>(col + k);
     sum -= v * x[c];
-    if (!have && c == i) {
-      d = v;
-      have = true;
-    }
+    if (c == i && v != 0.0) d = v;
+  }
+  sum += x[i] * d;
+  x[i] = sum / d;
+}
+
+// 16-bit column ids (Level::col16).  The caller's wavefront lies in one tile,
+// so the tile id, its "32-bit only" flag and its slot bases are uniform.
+__device__ inline int tile_of(int64_t i) {
+  return __builtin_amdgcn_readfirstlane((int)(i / kTile));
+}
+
+template <bool STREAM>
+__device__ inline double row_dot16(const Col16 &C, const int32_t *__restrict__ col,
+                                   const double *__restrict__ val,
+                                   const double *__restrict__ x, int64_t n,
+                                   int64_t i) {
+  const int t = tile_of(i);
+  if (C.info[t]) return row_dot<STREAM>(col, val, x, n, i);
+  const int32_t *__restrict__ base = C.base + t;
+  double sum = 0.0;
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int64_t k = (int64_t)j * n + i;
+    const int32_t c = base[(int64_t)j * C.ntiles] + (int32_t)mload<STREAM>(C.off + k);
+    sum += mload<STREAM>(val + k) * x[c];
+  }
+  return sum;
+}
+
+template <bool STREAM>
+__device__ inline void gs_row16(const Col16 &C, int64_t i, int64_t n,
+                                const int32_t *__restrict__ col,
+                                const double *__restrict__ val,
+                                const double *__restrict__ r, double *x) {
+  const int t = tile_of(i);
+  if (C.info[t]) {
+    gs_row<STREAM>(i, n, col, val, r, x);
+    return;
+  }
+  const int32_t *__restrict__ base = C.base + t;
+  double sum = r[i];
+  double d = 0.0;
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int64_t k = (int64_t)j * n + i;
+    const double v = mload<STREAM>(val + k);
+    const int32_t c = base[(int64_t)j * C.ntiles] + (int32_t)mload<STREAM>(C.off + k);
+    sum -= v * x[c];
+    if (c == i && v != 0.0) d = v;
   }
   sum += x[i] * d;
   x[i] = sum / d;
@@ -160,6 +214,15 @@ __global__ __launch_bounds__(kBlock) void k_gs_range(
   if (i < end) gs_row<STREAM>(i, n, col, val, r, x);
 }
 
+// k_gs_range with 16-bit column ids (begin is a multiple of kTile).
+template <bool STREAM, bool FINEST>
+__global__ __launch_bounds__(kBlock) void k_gs_range16(
+    int64_t begin, int64_t end, int64_t n, Col16 C, const int32_t *__restrict__ col,
+    const double *__restrict__ val, const double *__restrict__ r, double *x) {
+  const int64_t i = begin + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < end) gs_row16<STREAM>(C, i, n, col, val, r, x);
+}
+
 // Two adjacent rows per thread: the matrix is read with 16-byte (values) and
 // 8-byte (column ids) loads per lane.  Needs an even level size and an even
 // range start, so that slot j of the pair is 16-byte aligned.
@@ -173,7 +236,6 @@ __device__ inline void gs_row_pair(int64_t i, int64_t n,
                                    const double *__restrict__ r, double *x) {
   const f64x2 rr = *reinterpret_cast<const f64x2 *>(r + i);
   double s0 = rr.x, s1 = rr.y, d0 = 0.0, d1 = 0.0;
-  bool h0 = false, h1 = false;
 #pragma unroll
   for (int j = 0; j < kSlots; ++j) {
     const int64_t k = (int64_t)j * n + i;
@@ -181,14 +243,8 @@ __device__ inline void gs_row_pair(int64_t i, int64_t n,
     const i32x2 c = mload<STREAM>(reinterpret_cast<const i32x2 *>(col + k));
     s0 -= v.x * x[c.x];
     s1 -= v.y * x[c.y];
-    if (!h0 && c.x == i) {
-      d0 = v.x;
-      h0 = true;
-    }
-    if (!h1 && c.y == i + 1) {
-      d1 = v.y;
-      h1 = true;
-    }
+    if (c.x == i && v.x != 0.0) d0 = v.x;
+    if (c.y == i + 1 && v.y != 0.0) d1 = v.y;
   }
   s0 += x[i] * d0;
   s1 += x[i + 1] * d1;
@@ -244,7 +300,6 @@ __device__ inline void gs_row_wave(int64_t i, int64_t n, int64_t lo, int64_t hi,
                                    const double *__restrict__ r, double *x) {
   double sum = r[i];
   double d = 0.0;
-  bool have = false;
 #pragma unroll
   for (int j = 0; j < kSlots; ++j) {
     const int64_t k = (int64_t)j * n + i;
@@ -253,10 +308,7 @@ __device__ inline void gs_row_wave(int64_t i, int64_t n, int64_t lo, int64_t hi,
     const bool fresh = DIR == 0 ? c < lo : c >= hi;
     const double xc = fresh ? xload(x + c) : x[c];
     sum -= v * xc;
-    if (!have && c == i) {
-      d = v;
-      have = true;
-    }
+    if (c == i && v != 0.0) d = v;
   }
   sum += x[i] * d;
   xstore(x + i, sum / d);
@@ -444,9 +496,9 @@ __global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const double *x,
 // multiple of 8 workgroups): XCD x (= blockIdx % 8) owns the x-th eighth of
 // the colour-interleaved chunk list and its workgroups stride through it, so
 // both the partial sums and each XCD's gather window are fixed.
-template <bool STREAM>
+template <bool STREAM, bool C16>
 __global__ __launch_bounds__(kBlock) void k_spmv_dot(
-    int64_t nchunks, const int64_t *__restrict__ chunks, int64_t n,
+    int64_t nchunks, const int64_t *__restrict__ chunks, int64_t n, Col16 C,
     const int32_t *__restrict__ col, const double *__restrict__ val,
     const double *__restrict__ p, double *__restrict__ Ap, double *partials,
     unsigned *ticket, double *scalars) {
@@ -456,7 +508,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(
   for (int64_t w = lo + slot; w < hi; w += per) {
     const int64_t i = chunks[2 * w] + threadIdx.x;
     if (i < chunks[2 * w + 1]) {
-      const double y = row_dot<STREAM>(col, val, p, n, i);
+      const double y = C16 ? row_dot16<STREAM>(C, col, val, p, n, i)
+                           : row_dot<STREAM>(col, val, p, n, i);
       Ap[i] = y;
       s += p[i] * y;
     }
@@ -532,18 +585,37 @@ void launch_gather(const int32_t *idx, const double *src, double *dst,
   HPCGX_LAUNCHED();
 }
 
-void launch_spmv(const Level &L, const double *x, double *y, hipStream_t s) {
+void launch_spmv(const Problem &P, const Level &L, const double *x, double *y,
+                 hipStream_t s) {
   const unsigned g = (unsigned)L.nchunks;
-  if (L.stream)
-    k_spmv<true><<<g, kBlock, 0, s>>>(L.chunks, L.n, L.col, L.val, x, y, L.xcd);
+  const Col16 C = col16_of(L);
+  const bool c16 = use_col16(P, L, 0);
+  if (L.stream && c16)
+    k_spmv<true, true><<<g, kBlock, 0, s>>>(L.chunks, L.n, C, L.col, L.val, x, y, L.xcd);
+  else if (L.stream)
+    k_spmv<true, false><<<g, kBlock, 0, s>>>(L.chunks, L.n, C, L.col, L.val, x, y, L.xcd);
+  else if (c16)
+    k_spmv<false, true><<<g, kBlock, 0, s>>>(L.chunks, L.n, C, L.col, L.val, x, y, L.xcd);
   else
-    k_spmv<false><<<g, kBlock, 0, s>>>(L.chunks, L.n, L.col, L.val, x, y, L.xcd);
+    k_spmv<false, false><<<g, kBlock, 0, s>>>(L.chunks, L.n, C, L.col, L.val, x, y, L.xcd);
   HPCGX_LAUNCHED();
 }
 
-void launch_symgs_rows(const Level &L, int64_t begin, int64_t end,
+void launch_symgs_rows(const Problem &P, const Level &L, int64_t begin, int64_t end,
                        const double *r, double *x, hipStream_t s) {
   if (end <= begin) return;
+  if (use_col16(P, L, begin)) {
+    const Col16 C = col16_of(L);
+    const unsigned g = grid_for(end - begin);
+    if (L.finest)
+      k_gs_range16<true, true><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col, L.val, r, x);
+    else if (L.stream)
+      k_gs_range16<true, false><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col, L.val, r, x);
+    else
+      k_gs_range16<false, false><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col, L.val, r, x);
+    HPCGX_LAUNCHED();
+    return;
+  }
   if (L.pair_rows && L.n % 2 == 0 && begin % 2 == 0) {
     const unsigned g2 = grid_for((end - begin + 1) / 2);
     if (L.finest)
@@ -620,18 +692,23 @@ void launch_dot(Problem &P, int64_t n, const double *x, const double *y,
 
 void launch_spmv_dot(Problem &P, const Level &L, const double *p, double *Ap,
                      hipStream_t s) {
-  int64_t g = std::min<int64_t>(L.nchunks, kReduceBlocks);
+  int64_t g = std::min<int64_t>(L.nchunks, std::min(P.spmv_dot_blocks, kMaxPartials));
   g = std::max<int64_t>(8, (g + 7) / 8 * 8);
-  if (L.stream)
-    k_spmv_dot<true><<<(unsigned)g, kBlock, 0, s>>>(L.nchunks, L.chunks, L.n,
-                                                   L.col, L.val, p, Ap,
-                                                   P.partials, P.ticket,
-                                                   P.scalars);
+  const Col16 C = col16_of(L);
+  const bool c16 = use_col16(P, L, 0);
+  const unsigned G = (unsigned)g;
+  if (L.stream && c16)
+    k_spmv_dot<true, true><<<G, kBlock, 0, s>>>(L.nchunks, L.chunks, L.n, C, L.col, L.val, p,
+                                                Ap, P.partials, P.ticket, P.scalars);
+  else if (L.stream)
+    k_spmv_dot<true, false><<<G, kBlock, 0, s>>>(L.nchunks, L.chunks, L.n, C, L.col, L.val, p,
+                                                 Ap, P.partials, P.ticket, P.scalars);
+  else if (c16)
+    k_spmv_dot<false, true><<<G, kBlock, 0, s>>>(L.nchunks, L.chunks, L.n, C, L.col, L.val, p,
+                                                 Ap, P.partials, P.ticket, P.scalars);
   else
-    k_spmv_dot<false><<<(unsigned)g, kBlock, 0, s>>>(L.nchunks, L.chunks, L.n,
-                                                    L.col, L.val, p, Ap,
-                                                    P.partials, P.ticket,
-                                                    P.scalars);
+    k_spmv_dot<false, false><<<G, kBlock, 0, s>>>(L.nchunks, L.chunks, L.n, C, L.col, L.val,
+                                                  p, Ap, P.partials, P.ticket, P.scalars);
   HPCGX_LAUNCHED();
 }
 

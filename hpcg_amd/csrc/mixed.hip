@@ -111,6 +111,90 @@ __device__ inline float row_dot32(int64_t i, int64_t n,
   return s;
 }
 
+// 16-bit column ids (Level::col16): the caller's wavefront lies in one tile.
+__device__ inline int tile_of(int64_t i) {
+  return __builtin_amdgcn_readfirstlane((int)(i / kTile));
+}
+
+// Entry of a narrow tile packed in one 32-bit word (Level::vc):
+// high half = 16-bit column offset, low half = the FP16 value.  One load per
+// slot instead of two: the FP32 colour step is bound by the number of lane
+// addresses the memory pipeline processes, not by bytes.
+__device__ inline half_t vc_val(uint32_t w) {
+  return __builtin_bit_cast(half_t, (uint16_t)(w & 0xffffu));
+}
+
+template <bool STREAM>
+__device__ inline void gs32_row16(const Col16 &C, const uint32_t *__restrict__ vc,
+                                  int64_t i, int64_t n, const int32_t *__restrict__ col,
+                                  const half_t *__restrict__ val16,
+                                  const float *__restrict__ diag32,
+                                  const float *__restrict__ r, float *x) {
+  const int t = tile_of(i);
+  if (C.info[t]) {
+    gs32_row<STREAM>(i, n, col, val16, diag32, r, x);
+    return;
+  }
+  const int32_t *__restrict__ base = C.base + t;
+  float sum = r[i];
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const uint32_t w = mload<STREAM>(vc + (int64_t)j * n + i);
+    const int32_t c = base[(int64_t)j * C.ntiles] + (int32_t)(w >> 16);
+    sum -= (float)vc_val(w) * x[c];
+  }
+  x[i] = sum / diag32[i];
+}
+
+template <bool STREAM>
+__device__ inline float row_dot32_16(const Col16 &C, const uint32_t *__restrict__ vc,
+                                     int64_t i, int64_t n, const int32_t *__restrict__ col,
+                                     const half_t *__restrict__ val16,
+                                     const float *__restrict__ diag32,
+                                     const float *__restrict__ x) {
+  const int t = tile_of(i);
+  if (C.info[t]) return row_dot32<STREAM>(i, n, col, val16, diag32, x);
+  const int32_t *__restrict__ base = C.base + t;
+  float s = 0.0f;
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const uint32_t w = mload<STREAM>(vc + (int64_t)j * n + i);
+    const int32_t c = base[(int64_t)j * C.ntiles] + (int32_t)(w >> 16);
+    s += (float)vc_val(w) * x[c];
+  }
+  s += diag32[i] * x[i];
+  return s;
+}
+
+// vc[k] = col16[k] << 16 | bits of val16[k]
+__global__ __launch_bounds__(kBlock) void k_pack_vc(int64_t m, const uint16_t *__restrict__ off,
+                                                    const half_t *__restrict__ val16,
+                                                    uint32_t *__restrict__ vc) {
+  const int64_t k = gtid();
+  if (k < m) vc[k] = ((uint32_t)off[k] << 16) | __builtin_bit_cast(uint16_t, val16[k]);
+}
+
+template <bool STREAM, bool FINEST>
+__global__ __launch_bounds__(kBlock) void k_gs32_range16(
+    int64_t begin, int64_t end, int64_t n, Col16 C, const uint32_t *__restrict__ vc,
+    const int32_t *__restrict__ col, const half_t *__restrict__ val16,
+    const float *__restrict__ diag32, const float *__restrict__ r, float *x) {
+  const int64_t i = begin + gtid();
+  if (i < end) gs32_row16<STREAM>(C, vc, i, n, col, val16, diag32, r, x);
+}
+
+template <bool STREAM>
+__global__ __launch_bounds__(kBlock) void k_spmv32_16(
+    const int64_t *__restrict__ chunks, int64_t n, Col16 C, const uint32_t *__restrict__ vc,
+    const int32_t *__restrict__ col, const half_t *__restrict__ val16,
+    const float *__restrict__ diag32, const float *__restrict__ x,
+    float *__restrict__ y) {
+  const int64_t b = blockIdx.x;
+  const int64_t i = chunks[2 * b] + threadIdx.x;
+  if (i < chunks[2 * b + 1])
+    y[i] = row_dot32_16<STREAM>(C, vc, i, n, col, val16, diag32, x);
+}
+
 template <bool STREAM, bool FINEST>
 __global__ __launch_bounds__(kBlock) void k_gs32_range(
     int64_t begin, int64_t end, int64_t n, const int32_t *__restrict__ col,
@@ -159,10 +243,24 @@ __global__ __launch_bounds__(kBlock) void k_prolong32(
   if (c < nc) xf[f2c[c]] += xc[c];
 }
 
-void gs32_step(const Level &L, int64_t begin, int64_t end, const float *r,
-               float *x, hipStream_t s) {
+void gs32_step(const Problem &P, const Level &L, int64_t begin, int64_t end,
+               const float *r, float *x, hipStream_t s) {
   if (end <= begin) return;
   const unsigned g = grid_for(end - begin);
+  if (use_col16(P, L, begin)) {
+    const Col16 C = col16_of(L);
+    if (L.finest)
+      k_gs32_range16<true, true><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.vc, L.col, L.val16,
+                                                      L.diag32, r, x);
+    else if (L.stream)
+      k_gs32_range16<true, false><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.vc, L.col, L.val16,
+                                                       L.diag32, r, x);
+    else
+      k_gs32_range16<false, false><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.vc, L.col, L.val16,
+                                                        L.diag32, r, x);
+    HPCGX_LAUNCHED();
+    return;
+  }
   if (L.finest)
     k_gs32_range<true, true><<<g, kBlock, 0, s>>>(begin, end, L.n, L.col,
                                                   L.val16, L.diag32, r, x);
@@ -188,6 +286,12 @@ void prepare_mixed(Problem &P) {
     k_make_half<<<grid_for(L.n), kBlock>>>(L.n, L.val, L.dslot, L.val16, L.diag32,
                                            static_cast<int *>(overflow.p));
     HPCGX_LAUNCHED();
+    if (L.c16) {
+      const int64_t m = (int64_t)kSlots * L.n;
+      L.vc = P.arena.alloc<uint32_t>((size_t)m);
+      k_pack_vc<<<grid_for(m), kBlock>>>(m, L.col16, L.val16, L.vc);
+      HPCGX_LAUNCHED();
+    }
     if (L.nc) {
       L.rc32 = P.arena.alloc<float>(L.nc);
       L.xc32 = P.arena.alloc<float>(L.nc);
@@ -212,9 +316,9 @@ void symgs32(Problem &P, int l, int ordering, const float *r, float *x,
   const Level &L = P.L[l];
   if (ordering == HPCGX_ORDER_MULTICOLOR) {
     for (int c = 0; c < L.ncolors; ++c)
-      gs32_step(L, L.color_ptr[c], L.color_ptr[c + 1], r, x, s);
+      gs32_step(P, L, L.color_ptr[c], L.color_ptr[c + 1], r, x, s);
     for (int c = L.ncolors - 1; c >= 0; --c)
-      gs32_step(L, L.color_ptr[c], L.color_ptr[c + 1], r, x, s);
+      gs32_step(P, L, L.color_ptr[c], L.color_ptr[c + 1], r, x, s);
   } else {
     for (int pass = 0; pass < 2; ++pass)
       for (int k = 0; k < L.nsched; ++k) {
@@ -228,7 +332,18 @@ void symgs32(Problem &P, int l, int ordering, const float *r, float *x,
   }
 }
 
-void spmv32(const Level &L, const float *x, float *y, hipStream_t s) {
+void spmv32(const Problem &P, const Level &L, const float *x, float *y, hipStream_t s) {
+  if (use_col16(P, L, 0)) {
+    const Col16 C = col16_of(L);
+    if (L.stream)
+      k_spmv32_16<true><<<(unsigned)L.nchunks, kBlock, 0, s>>>(L.chunks, L.n, C, L.vc,
+                                                               L.col, L.val16, L.diag32, x, y);
+    else
+      k_spmv32_16<false><<<(unsigned)L.nchunks, kBlock, 0, s>>>(L.chunks, L.n, C, L.vc,
+                                                                L.col, L.val16, L.diag32, x, y);
+    HPCGX_LAUNCHED();
+    return;
+  }
   if (L.stream)
     k_spmv32<true><<<(unsigned)L.nchunks, kBlock, 0, s>>>(L.chunks, L.n, L.col,
                                                           L.val16, L.diag32, x, y);
@@ -291,7 +406,7 @@ void cg_mixed(Problem &P, int ordering, const double *b, double *x,
   const int64_t n = A.n;
   *niters = 0;
   launch_copy(x, P.p, n, s);
-  launch_spmv(A, P.p, P.Ap, s);
+  launch_spmv(P, A, P.p, P.Ap, s);
   launch_residual0(P, n, b, P.Ap, P.r, s);
   double nr = std::sqrt(read_scalar(P, kNormr2, s));
   *normr0 = nr;
@@ -309,7 +424,7 @@ void cg_mixed(Problem &P, int ordering, const double *b, double *x,
     launch_update_xr(P, n, x, P.r, P.p, P.Ap, s);
     if (recompute > 0 && k % recompute == 0) {
       // r = b - A x in FP64 (replaces the recursively updated r)
-      launch_spmv(A, x, P.Ap, s);
+      launch_spmv(P, A, x, P.Ap, s);
       launch_residual0(P, n, b, P.Ap, P.r, s);
     }
     *niters = k;

@@ -13,11 +13,15 @@
 //    j < i), computed in parallel because a row's colour depends only on
 //    rows of lower levels.
 // Rows are then renumbered colour by colour (natural order inside a colour)
-// and the 27-slot ELL is rebuilt in that order.
+// and the 27-slot ELL is rebuilt in that order, with the slots of every tile
+// of 64 rows aligned by natural column offset so that the column ids of a
+// slot fit 16 bits above a per-tile base (k_tile_layout).
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <numeric>
+
+#include <climits>
 
 #include "internal.hpp"
 
@@ -232,6 +236,93 @@ __global__ __launch_bounds__(kBlock) void k_map(int64_t n, const int32_t *map,
   if (k < n) out[k] = map[in[k]];
 }
 
+__device__ inline int wave_min_i32(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ inline int wave_max_i32(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Tile-aligned slots and 16-bit column ids (Level::col16), one wavefront per
+// tile of kTile device rows, in place on the device-order ELL.
+//  1. The tile's distinct natural column offsets (column - row, natural
+//     numbering), ascending: repeated wavefront minimum of each row's next
+//     offset.  A row's offsets ascend with its columns, so entry e of a row
+//     lands in a slot >= e and the row's entries keep their order.
+//  2. Entries move to their slots (highest slot first, so nothing is
+//     overwritten before it is read); a row without an entry in a slot gets
+//     a column already in that slot's range and the value 0.  The slot's base
+//     is its smallest column; a slot spanning more than 65535 columns, or more
+//     than 27 distinct offsets in the tile, marks the tile 32-bit only.
+__global__ __launch_bounds__(kBlock) void k_tile_layout(
+    int64_t n, int64_t ntiles, const int32_t *__restrict__ iperm, int32_t *col,
+    double *val, const uint8_t *__restrict__ nnz_row, uint8_t *dslot,
+    uint16_t *col16, int32_t *cbase, int32_t *tinfo) {
+  const int64_t t = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (t >= ntiles) return;  // whole wavefronts
+  const int lane = threadIdx.x & 63;
+  const int64_t i = t * kTile + lane;
+  const bool row = i < n;
+  const int m = row ? nnz_row[i] : 0;
+  const int64_t inat = row ? iperm[i] : 0;
+
+  unsigned mask = 0;  // slots holding one of this row's entries
+  int ptr = 0, nu = 0;
+  bool fits = true;
+  for (;;) {
+    int head = INT_MAX;
+    if (ptr < m) head = (int)((int64_t)iperm[col[(int64_t)ptr * n + i]] - inat);
+    const int mn = wave_min_i32(head);
+    if (mn == INT_MAX) break;
+    if (nu == kSlots) {
+      fits = false;
+      break;
+    }
+    if (head == mn) {
+      mask |= 1u << nu;
+      ++ptr;
+    }
+    ++nu;
+  }
+  if (!fits) {
+    if (lane == 0) tinfo[t] = 1;
+    return;
+  }
+
+  bool narrow = true;
+  int ds = 0;
+#pragma unroll
+  for (int u = kSlots - 1; u >= 0; --u) {
+    const bool has = (mask >> u) & 1u;
+    int32_t c = 0;
+    double v = 0.0;
+    if (has) {
+      const int e = __popc(mask & ((1u << u) - 1u));
+      c = col[(int64_t)e * n + i];
+      v = val[(int64_t)e * n + i];
+    }
+    const int lo = wave_min_i32(has ? c : INT_MAX);
+    const int hi = wave_max_i32(has ? c : INT_MIN);
+    const bool used = lo != INT_MAX;
+    if (used && hi - lo > 65535) narrow = false;
+    const int32_t base = used ? lo : (int32_t)(t * kTile);
+    if (row) {
+      const int32_t cc = has ? c : (used ? lo : (int32_t)i);
+      col[(int64_t)u * n + i] = cc;
+      val[(int64_t)u * n + i] = has ? v : 0.0;
+      col16[(int64_t)u * n + i] = (uint16_t)(cc - base);
+      if (has && c == i) ds = u;
+    }
+    if (lane == 0) cbase[(int64_t)u * ntiles + t] = base;
+  }
+  if (row) dslot[i] = (uint8_t)ds;
+  if (lane == 0) tinfo[t] = narrow ? 0 : 1;
+}
+
 // Rebuild the ELL in device order: device row k is natural row iperm[k];
 // columns are renumbered through perm; unused slots become (k, 0.0).
 __global__ __launch_bounds__(kBlock) void k_permute_ell(
@@ -288,6 +379,20 @@ int bits_for(int64_t v) {
   int b = 1;
   while (((int64_t)1 << b) <= v) ++b;
   return b;
+}
+
+// Device-order key: colour, then the band of `band` grid lines (y) the
+// natural row lies in.  Inside a colour, rows then run band by band, each
+// band plane by plane: the x values a colour step gathers from the
+// neighbouring planes are reused one band later instead of one whole plane
+// later, which keeps them in the L2 cache.
+__global__ __launch_bounds__(kBlock) void k_band_key(int64_t n, int nx, int ny, int band,
+                                                     int nbands, const int32_t *color,
+                                                     int32_t *key) {
+  const int64_t i = gtid();
+  if (i >= n) return;
+  const int y = (int)((i / nx) % ny);
+  key[i] = color[i] * nbands + y / band;
 }
 
 // Stable sort of row ids by a small non-negative integer key (< nbins):
@@ -480,11 +585,28 @@ void finalize_level(Problem &P, int l, NaturalLevel &N, int flags,
   }
   L.nsched = nsched;
 
-  // device order = stable sort of rows by colour
+  // device order = stable sort of rows by colour; inside a colour, by band
+  // of grid lines when the level is an nx*ny*nz grid in natural order
   L.iperm = P.arena.alloc<int32_t>(n);
   L.perm = P.arena.alloc<int32_t>(n);
   {
-    std::vector<int64_t> cp = sort_rows_by_key(n, color.as<int32_t>(), kMaxColors, L.iperm, s);
+    int band = (flags >> 8) & 0xff;
+    if (band == 0) band = kBandLines;
+    const bool grid = (int64_t)N.nx * N.ny * N.nz == n && N.nx > 0;
+    const int nbands = (grid && band > 1 && N.ny >= 2 * band) ? (N.ny + band - 1) / band : 1;
+    std::vector<int64_t> cp;
+    if (nbands > 1) {
+      TempBuf key(sizeof(int32_t) * n);
+      k_band_key<<<grid_for(n), kBlock, 0, s>>>(n, N.nx, N.ny, band, nbands,
+                                                color.as<int32_t>(), key.as<int32_t>());
+      HPCGX_LAUNCHED();
+      const std::vector<int64_t> bp =
+          sort_rows_by_key(n, key.as<int32_t>(), kMaxColors * nbands, L.iperm, s);
+      for (int c = 0; c <= kMaxColors; ++c) cp.push_back(bp[(size_t)c * nbands]);
+    } else {
+      cp = sort_rows_by_key(n, color.as<int32_t>(), kMaxColors, L.iperm, s);
+    }
+    L.natural_in_colour = nbands == 1;
     int ncolors = 0;
     for (int c = 0; c < kMaxColors; ++c)
       if (cp[c + 1] > cp[c]) ncolors = c + 1;
@@ -523,6 +645,27 @@ void finalize_level(Problem &P, int l, NaturalLevel &N, int flags,
                                                N.nnz_row, L.col, L.val, L.diag,
                                                L.dslot, L.nnz_row);
   HPCGX_LAUNCHED();
+  // 16-bit column ids where every colour block starts on a tile boundary
+  // (the colour-step and SpMV launches then keep each wavefront in a tile)
+  bool aligned = n >= kTile;
+  for (int c = 0; c <= L.ncolors; ++c) aligned = aligned && L.color_ptr[c] % kTile == 0;
+  if (aligned) {
+    const int64_t T = (n + kTile - 1) / kTile;
+    L.ntiles = T;
+    L.col16 = P.arena.alloc<uint16_t>((size_t)kSlots * n);
+    L.cbase = P.arena.alloc<int32_t>((size_t)kSlots * T);
+    L.tinfo = P.arena.alloc<int32_t>(T);
+    const int64_t wpb = kBlock / 64;
+    k_tile_layout<<<(unsigned)((T + wpb - 1) / wpb), kBlock, 0, s>>>(
+        n, T, L.iperm, L.col, L.val, L.nnz_row, L.dslot, L.col16, L.cbase, L.tinfo);
+    HPCGX_LAUNCHED();
+    std::vector<int32_t> info(T);
+    HPCGX_HIP(hipMemcpyAsync(info.data(), L.tinfo, sizeof(int32_t) * T,
+                             hipMemcpyDeviceToHost, s));
+    HPCGX_HIP(hipStreamSynchronize(s));
+    L.wide_tiles = std::count(info.begin(), info.end(), 1);
+    L.c16 = true;
+  }
   HPCGX_HIP(hipStreamSynchronize(s));
   if (P.use_wave && n >= kWaveMinRows)
     build_wave(P, L, kWaveSlabFactor, kWaveRows, s);
@@ -551,6 +694,10 @@ void drop_wave(Problem &P, Level &L) {
 }
 
 void build_wave(Problem &P, Level &L, int slab_factor, int rows, hipStream_t s) {
+  if (!L.natural_in_colour)
+    throw Error(HPCGX_EUNSUP,
+                "wavefront sweeps need natural row order inside each colour "
+                "(create the problem with HPCGX_BAND_LINES(1))");
   drop_wave(P, L);
   const int C = L.ncolors;
   const int CH = kBlock * std::max(1, rows);

@@ -45,7 +45,7 @@ struct LaunchLog {
 void colour_step(Problem &P, const Level &L, int c, const double *r, double *x,
                  hipStream_t s) {
   LaunchLog log(P, L, s);
-  launch_symgs_rows(L, L.color_ptr[c], L.color_ptr[c + 1], r, x, s);
+  launch_symgs_rows(P, L, L.color_ptr[c], L.color_ptr[c + 1], r, x, s);
   log.done();
 }
 
@@ -157,7 +157,7 @@ void cg(Problem &P, int ordering, const double *b, double *x, int max_iter,
   *niters = 0;
   launch_copy(x, P.p, n, s);                             // :86
   T.tick();
-  launch_spmv(A, P.p, P.Ap, s);                          // :87
+  launch_spmv(P, A, P.p, P.Ap, s);                          // :87
   T.tock(t3);
   T.tick();
   launch_residual0(P, n, b, P.Ap, P.r, s);               // :88-89

@@ -54,6 +54,12 @@ def _order(o):
         raise ValueError("unknown ordering %r" % (o,))
 
 
+def _flags(keep_schedule, band_lines):
+    if not 0 <= band_lines <= 255:
+        raise ValueError("band_lines must be in [0, 255]")
+    return (KEEP_SCHEDULE if keep_schedule else 0) | (band_lines << 8)
+
+
 class Problem:
     """Device-resident HPCG problem (all levels of the MG hierarchy)."""
 
@@ -64,16 +70,18 @@ class Problem:
 
     # ---- construction --------------------------------------------------
     @classmethod
-    def generate(cls, nx, ny, nz, nlevels=4, keep_schedule=False):
+    def generate(cls, nx, ny, nz, nlevels=4, keep_schedule=False, band_lines=0):
         """GenerateProblem + GenerateCoarseProblem + OptimizeProblem on the
-        device (main.cpp:133-144, OptimizeProblem.cpp:37)."""
+        device (main.cpp:133-144, OptimizeProblem.cpp:37).  band_lines: rows
+        inside a colour run in bands of this many grid lines (0 = default,
+        1 = natural order; see HPCGX_BAND_LINES)."""
         h = ctypes.c_void_p()
         call("hpcgx_generate_problem", nx, ny, nz, nlevels,
-             KEEP_SCHEDULE if keep_schedule else 0, ctypes.byref(h))
+             _flags(keep_schedule, band_lines), ctypes.byref(h))
         return cls(h)
 
     @classmethod
-    def from_reference_levels(cls, levels, keep_schedule=False):
+    def from_reference_levels(cls, levels, keep_schedule=False, band_lines=0):
         """OptimizeProblem from host matrices in the reference's layout.
 
         ``levels`` is a list of dicts with numpy arrays ``nnz_row`` (n),
@@ -105,7 +113,7 @@ class Problem:
             arr[l].f2cOperator = None if f2c is None else f2c.ctypes.data
         h = ctypes.c_void_p()
         call("hpcgx_optimize_problem", arr, len(levels),
-             KEEP_SCHEDULE if keep_schedule else 0, ctypes.byref(h))
+             _flags(keep_schedule, band_lines), ctypes.byref(h))
         del keep
         return cls(h)
 
@@ -126,7 +134,7 @@ class Problem:
         call("hpcgx_problem_info", self._h, level, ctypes.byref(i))
         return dict(nx=i.nx, ny=i.ny, nz=i.nz, nrow=i.nrow, nnz=i.nnz, ncolors=i.ncolors,
                     nlevels_sched=i.nlevels_sched, ncoarse=i.ncoarse, wave=bool(i.wave),
-                    nslabs=i.nslabs)
+                    nslabs=i.nslabs, col16=bool(i.col16), wide_tiles=i.wide_tiles)
 
     def nrow(self, level=0):
         return self.levels[level]["nrow"]
@@ -277,8 +285,9 @@ class Problem:
         return dict(niters=ni.value, normr=nr.value, normr0=nr0.value, times=list(t))
 
     def tune(self, key, value):
-        """Set a tuning knob ("wave", "wave_min_rows", "wave_slab_factor",
-        "xcd_remap", "stream_min_bytes"); see include/hpcg_amd.h."""
+        """Set a tuning knob ("col16", "wave", "wave_min_rows",
+        "wave_slab_factor", "wave_rows", "xcd_remap", "pair_rows",
+        "stream_min_bytes"); see include/hpcg_amd.h."""
         return call("hpcgx_tune", self._h, key.encode(), float(value))
 
     def check(self):

@@ -45,6 +45,14 @@ enum {
   HPCGX_KEEP_SCHEDULE = 1 /* keep per-row level and colour ids so that
                              hpcgx_get_schedule can return them */
 };
+/* Order of the rows inside a colour, for a level that is an nx*ny*nz grid in
+ * natural order: by bands of `lines` grid lines (y), band by band, so that
+ * a colour step can reuse gathered x values while they are cached.  0 = the
+ * default, natural order inside each colour (bands measured neutral on
+ * MI355X); 1 = natural order as well.  The "wave" tuning option needs
+ * natural order.  Results do not depend on it: rows of one colour are
+ * independent. */
+#define HPCGX_BAND_LINES(lines) (((lines) & 0xff) << 8)
 
 typedef struct hpcgx_problem hpcgx_problem; /* device-resident MG hierarchy */
 
@@ -70,6 +78,8 @@ typedef struct {
   int64_t ncoarse;       /* rows of the next level (0 on coarsest) */
   int wave;              /* 1: multicolour sweeps run as a wavefront */
   int nslabs;            /* slabs of the wavefront schedule */
+  int col16;             /* 1: 16-bit column ids (tile base + offset) built */
+  int64_t wide_tiles;    /* tiles of 64 rows that keep 32-bit column ids */
 } hpcgx_level_info;
 
 const char *hpcgx_last_error(void);
@@ -244,6 +254,11 @@ int hpcgx_cg_mixed(hpcgx_problem *p, int ordering, const double *b, double *x,
  *                      (16-byte matrix loads)
  *   "stream_min_bytes" matrices larger than this are read with non-temporal
  *                      loads (default 512e6)
+ *   "col16"            1/0: read 16-bit column ids (a per-tile base plus a
+ *                      16-bit offset) where a level has them (default 1)
+ *   "spmv_dot_blocks"  workgroups of the persistent SpMV + dot kernel
+ *                      (8..8192, rounded down to a multiple of 8; default
+ *                      4096).  Changes the dot's summation order.
  * Synchronous.  Unknown keys return HPCGX_EINVAL. */
 int hpcgx_tune(hpcgx_problem *p, const char *key, double value);
 

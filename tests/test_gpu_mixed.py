@@ -93,6 +93,27 @@ def test_symgs32_bitexact(hp, dims, ordering):
         assert np.array_equal(host32(P, x, l), want), l
 
 
+@pytest.mark.parametrize("dims", [(32, 48, 64), (64, 64, 64), (8, 8, 24)])
+@pytest.mark.parametrize("c16", [1, 0])
+def test_symgs32_col16(hp, dims, c16):
+    """Multicolour FP32 sweeps with the packed 16-bit column / FP16 value
+    words (the default where a level has 16-bit ids) and with 32-bit ids:
+    the oracle's bits on every level, including colour blocks whose length
+    is not a multiple of 128 rows."""
+    say("symgs32 col16", dims, c16)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    P.tune("col16", c16)
+    h = hier(dims, False)
+    for l in range(4):
+        m = h.level(l)
+        r, x0 = rnd32(40 + l, h.nrow(l)), rnd32(50 + l, h.nrow(l))
+        want = O.symgs32(m, r, x0, O.color_order(O.colors(m)[0]))
+        x = dev32(P, x0, l)
+        P.symgs32(dev32(P, r, l), x, level=l, ordering="multicolor")
+        assert np.array_equal(host32(P, x, l), want), (c16, l)
+    del P
+
+
 @pytest.mark.parametrize("dims", DIMS)
 @pytest.mark.parametrize("ordering", ["lexicographic", "multicolor"])
 def test_mg32_bitexact(hp, dims, ordering):

@@ -176,6 +176,7 @@ def test_pair_rows_bitexact(hp, dims):
     colour-ordered result bit for bit."""
     say("pair rows", dims)
     P = hp.Problem.generate(*dims, nlevels=4)
+    P.tune("col16", 0)       # the pair kernel reads 32-bit column ids
     P.tune("pair_rows", 1)
     h = hier(dims, color=True)
     for l in range(4):
@@ -186,6 +187,46 @@ def test_pair_rows_bitexact(hp, dims):
         x = P.to_device_order(x0, l)
         P.symgs(P.to_device_order(r, l), x, level=l, ordering="multicolor")
         assert np.array_equal(P.to_numpy(x, l), want), l
+    del P
+
+
+@pytest.mark.parametrize("dims", SMALL + [(64, 64, 64)])
+def test_col16_layout(hp, dims):
+    """Tile-aligned slots with 16-bit column ids: built wherever the colour
+    blocks start on 64-row tiles, no tile of the 27-point problem needs
+    32-bit ids, and SpMV, both SYMGS orderings and the V-cycle give the
+    oracle's bits with the 16-bit ids and with the 32-bit ids of the same
+    layout."""
+    say("col16 layout", dims)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    h = hier(dims, color=True)
+    for l in range(4):
+        inf = P.info(l)
+        n = h.nrow(l)
+        col, nc = O.colors(h.level(l))
+        aligned = n >= 64 and all(c % 64 == 0 for c in np.concatenate(
+            [[0], np.cumsum(np.bincount(col, minlength=nc))]))
+        assert inf["col16"] == aligned, (l, inf)
+        if aligned:
+            assert inf["wide_tiles"] == 0, (l, inf)
+    r0 = rnd(301, h.nrow(0))
+    want_mg = h.mg(r0)
+    for c16 in (1, 0):
+        P.tune("col16", c16)
+        for l in range(4):
+            m = h.level(l)
+            n = h.nrow(l)
+            xv, r, x0 = rnd(310 + l, n), rnd(320 + l, n), rnd(330 + l, n)
+            y = P.empty(l)
+            P.spmv(P.to_device_order(xv, l), y, level=l)
+            assert np.array_equal(P.to_numpy(y, l), O.spmv(m, xv)), (c16, l)
+            want = O.symgs(m, r, x0, O.color_order(O.colors(m)[0]))
+            x = P.to_device_order(x0, l)
+            P.symgs(P.to_device_order(r, l), x, level=l, ordering="multicolor")
+            assert np.array_equal(P.to_numpy(x, l), want), (c16, l)
+        z = P.zeros()
+        P.mg(P.to_device_order(r0), z, ordering="multicolor")
+        assert np.array_equal(P.to_numpy(z), want_mg), c16
     del P
 
 
@@ -253,7 +294,7 @@ def test_wavefront_bitexact(hp, dims, slab_factor):
     direction (forced onto every level) give the colour-ordered Gauss-Seidel
     result bit for bit."""
     say("wavefront", dims, slab_factor)
-    P = hp.Problem.generate(*dims, nlevels=4)
+    P = hp.Problem.generate(*dims, nlevels=4, band_lines=1)  # natural order in colours
     P.tune("wave", 1)
     P.tune("wave_min_rows", 1)
     P.tune("wave_slab_factor", slab_factor)
@@ -445,17 +486,21 @@ def test_256_properties(hp, torch):
     # the 8-colour smoother converges more slowly than the lexicographic one
     # (oracle, 64^3, 50 iterations: 3.9e-10 vs 1.1e-11); at 256^3 the
     # residual after 50 iterations is about 7e-3
+    say("wavefront vs banded colour steps", dims)
+    Pb = hp.Problem.generate(*dims, nlevels=4, band_lines=16)  # rows in bands of 16 lines
+    with pytest.raises(hp.HpcgxError):
+        Pb.tune("wave", 1)                    # the banded order has no wavefront schedule
     P.tune("wave", 1)
     assert P.info(0)["wave"]
-    say("wavefront vs colour-step kernels", dims)
-    rr = P.to_device_order(rnd(140, P.nrow(0)))
-    xw = P.to_device_order(rnd(141, P.nrow(0)))
-    xk = xw.clone()
-    P.symgs(rr, xw, ordering="multicolor")
+    r_nat, x_nat = rnd(140, P.nrow(0)), rnd(141, P.nrow(0))
+    xw = P.to_device_order(x_nat)
+    P.symgs(P.to_device_order(r_nat), xw, ordering="multicolor")
     P.check()
     P.tune("wave", 0)
-    P.symgs(rr, xk, ordering="multicolor")
-    assert torch.equal(xw, xk)
+    xk = Pb.to_device_order(x_nat)
+    Pb.symgs(Pb.to_device_order(r_nat), xk, ordering="multicolor")
+    assert np.array_equal(P.to_numpy(xw), Pb.to_numpy(xk))
+    del Pb
     say("cg 256^3")
     r25 = P.cg(b, x, max_iter=25, tolerance=0.0, ordering="multicolor")
     x.zero_()
