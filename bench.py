@@ -51,6 +51,13 @@ def parse():
     ap.add_argument("--spmv-dot-blocks", type=int, default=0, help="0 = library default")
     ap.add_argument("--stream-min-bytes", type=float, default=0,
                     help="non-temporal matrix loads above this matrix size (0 = library default)")
+    ap.add_argument("--graph", type=int, default=0, choices=[0, 1],
+                    help="replay a captured hipGraph of the CG iteration in the timed "
+                         "region (the colour-step launches are then timed in a separate "
+                         "eager pass of the same length)")
+    ap.add_argument("--block", type=int, nargs="*", default=[],
+                    help="also solve s right-hand sides at once (block PCG, FP32 V-cycle) "
+                         "for each s given, to --tol-compare")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device (the known-size kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -126,6 +133,54 @@ def converge(P, b, lv, a, log):
     return res
 
 
+def splitmix_uniform(seed, n):
+    """splitmix64 stream mapped to uniform [-1, 1) (vectorised)."""
+    import numpy as np
+    with np.errstate(over="ignore"):
+        s = np.uint64(seed) + np.uint64(0x9E3779B97F4A7C15) * np.arange(1, n + 1, dtype=np.uint64)
+        z = (s ^ (s >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (2.0 / 9007199254740992.0) - 1.0
+
+
+def block_solve(P, s, a, log):
+    """Block PCG over s right-hand sides (splitmix64 uniform(-1, 1) from seed
+    42, read row-major into [n][s]) with the FP32 V-cycle, to ||r||/||r0|| <=
+    tol for every system, against s single-vector mixed solves of the same
+    systems one after another."""
+    import torch
+    n = P.nrow(0)
+    B = P.to_device_order_multi(splitmix_uniform(42, n * s).reshape(n, s))
+    X = torch.zeros_like(B)
+    P.cg_block_mixed(B, X, max_iter=2, tolerance=0.0)  # warm
+    X.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = P.cg_block_mixed(B, X, max_iter=a.max_iters, tolerance=a.tol_compare, recompute=10)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    log("block s=%d: %d iterations in %.3f s" % (s, r["niters"], dt))
+    seq, its = 0.0, []
+    for k in range(s):
+        b = B[:, k].contiguous()
+        x = torch.zeros_like(b)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        q = P.cg_mixed(b, x, max_iter=a.max_iters, tolerance=a.tol_compare, recompute=10)
+        torch.cuda.synchronize()
+        seq += time.perf_counter() - t1
+        its.append(q["niters"])
+    Y = torch.zeros_like(B)
+    P.spmv_block(X, Y)
+    true_res = max(float((B[:, k] - Y[:, k]).norm() / B[:, k].norm()) for k in range(s))
+    return {"s": s, "iterations": r["niters"], "seconds": round(dt, 4),
+            "ms_per_iteration": round(dt / max(r["niters"], 1) * 1e3, 4),
+            "max_true_relative_residual": true_res,
+            "single_rhs_seconds_total": round(seq, 4), "single_rhs_iterations": its,
+            "speedup_vs_single_rhs": round(seq / dt, 4)}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -163,14 +218,19 @@ def main():
     log("setup %.2f s, device memory %.1f GB" % (setup_s, P.memory_use() / 1e9))
 
     if a.warmup > 0:
-        P.cg(b, x, max_iter=a.warmup, tolerance=0.0, ordering=a.ordering)
+        P.cg(b, x, max_iter=a.warmup, tolerance=0.0, ordering=a.ordering, graph=bool(a.graph))
     x.zero_()
-    P.kernel_timing(True)
+    if not a.graph:
+        P.kernel_timing(True)  # HIP events around each fine colour step, inside the timed region
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    r = P.cg(b, x, max_iter=a.steps, tolerance=0.0, ordering=a.ordering)
+    r = P.cg(b, x, max_iter=a.steps, tolerance=0.0, ordering=a.ordering, graph=bool(a.graph))
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    if a.graph:  # graph replays carry no per-launch events: time the launches eagerly
+        x.zero_()
+        P.kernel_timing(True)
+        P.cg(b, x, max_iter=a.steps, tolerance=0.0, ordering=a.ordering)
     P.kernel_timing(False)
     ks_ms, ks_n = P.kernel_timing_read()
     log("timed %d iterations in %.3f s; normr/normr0 %.3e" % (r["niters"], dt, r["normr"] / r["normr0"]))
@@ -245,11 +305,14 @@ def main():
             "stored_format_bytes_per_launch": stored,
             "column_ids": "16-bit" if c16 else "32-bit",
             "mean_launch_ms": round(launch_ms, 5),
+            "launch_timing": "eager pass after the timed region" if a.graph else "timed region",
             "launches": ks_n,
         },
     }
     if a.tol_compare > 0:
         out["convergence"] = converge(P, b, lv, a, log)
+    if a.block:
+        out["block_pcg"] = [block_solve(P, s, a, log) for s in a.block]
     if not a.no_cpu_baseline:
         log("cpu baseline")
         try:

@@ -13,6 +13,7 @@ ORDER_LEXICOGRAPHIC = 0
 ORDER_MULTICOLOR = 1
 KEEP_SCHEDULE = 1
 CG_TIMERS = 1
+CG_GRAPH = 2
 
 ERRORS = {1: "EINVAL", 2: "EHIP", 3: "ENOMEM", 4: "EUNSUP"}
 
@@ -39,7 +40,8 @@ class LevelInfo(ctypes.Structure):
                 ("nrow", ctypes.c_int64), ("nnz", ctypes.c_int64),
                 ("ncolors", ctypes.c_int), ("nlevels_sched", ctypes.c_int),
                 ("ncoarse", ctypes.c_int64), ("wave", ctypes.c_int), ("nslabs", ctypes.c_int),
-                ("col16", ctypes.c_int), ("wide_tiles", ctypes.c_int64)]
+                ("col16", ctypes.c_int), ("wide_tiles", ctypes.c_int64),
+                ("byte_tiles", ctypes.c_int64)]
 
 
 # name -> (restype, argtypes)
@@ -75,6 +77,9 @@ SIGNATURES = {
     "hpcgx_symgs32": (I, [P, I, I, P, P, P]),
     "hpcgx_mg32": (I, [P, I, I, P, P, P]),
     "hpcgx_cg_mixed": (I, [P, I, P, P, I, D, I, P, P, P, P, P]),
+    "hpcgx_spmv_block": (I, [P, I, I, P, P, P]),
+    "hpcgx_mg32_block": (I, [P, I, I, I, P, P, P]),
+    "hpcgx_cg_block_mixed": (I, [P, I, I, P, P, I, D, I, P, P, P, P, P]),
     "hpcgx_tune": (I, [P, ctypes.c_char_p, D]),
     "hpcgx_check_status": (I, [P, P]),
     "hpcgx_kernel_timing": (I, [P, I]),

@@ -1,0 +1,444 @@
+// block.hip -- PCG over s independent right-hand sides at once.
+//
+// No counterpart in the reference (its CG.cpp:59-143 solves one system).
+// The s systems share the matrix: every kernel below walks a row's ELL slots
+// once and applies them to all s vectors, so a sweep streams the matrix once
+// for all right-hand sides.  Each right-hand side keeps its own recurrence
+// (its own alpha, beta and residual norm, CG.cpp:102-124) and gets exactly
+// the arithmetic the single-vector path gives it: the FP32 V-cycle on FP16
+// values (mixed.hip) as preconditioner, FP64 outer iteration, r = b - A x
+// recomputed in FP64 every `recompute` iterations.
+//
+// Multivectors are [n][s] row-major in the level's device row order: the s
+// values of a row are adjacent, so the s lanes that own a row read one
+// neighbour's s values with one coalesced access and share its matrix entry.
+// s must divide 64 (a wavefront then holds whole rows).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+#include "internal.hpp"
+
+namespace hpcgx {
+namespace {
+
+typedef _Float16 half_t;
+
+inline unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+__device__ inline int64_t gtid() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+
+template <bool STREAM, class T>
+__device__ inline T mload(const T *p) {
+  if (STREAM) return __builtin_nontemporal_load(p);
+  return *p;
+}
+
+// ---- per-right-hand-side reductions -------------------------------------
+// Thread t of a workgroup owns right-hand side t % s (the grid stride is a
+// multiple of s).  The workgroup's s sums go to partials[block * s + k]; the
+// last workgroup (ticket) adds them in block order for each k and applies
+// the finishing step to scalar slot k of each array in BlockScalars.
+enum BlockFinish { kBStore = 0, kBBeta = 1, kBAlpha = 2 };
+
+__device__ inline void block_finish_s(double v, int s, double *partials, unsigned *ticket,
+                                      double *sc, int finish) {
+  __shared__ double red[kBlock];
+  __shared__ int is_last;
+  red[threadIdx.x] = v;
+  __syncthreads();
+  if (threadIdx.x < (unsigned)s) {  // fixed order: rows of the block ascending
+    double t = 0.0;
+    for (int q = threadIdx.x; q < kBlock; q += s) t += red[q];
+    partials[(int64_t)blockIdx.x * s + threadIdx.x] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    is_last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  if (threadIdx.x < (unsigned)s) {
+    const int k = threadIdx.x;
+    double t = 0.0;
+    for (unsigned b = 0; b < gridDim.x; ++b)
+      t += __hip_atomic_load(&partials[(int64_t)b * s + k], __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    double *rtz = sc, *oldrtz = sc + s, *alpha = sc + 2 * s, *beta = sc + 3 * s,
+           *normr2 = sc + 4 * s;
+    if (finish == kBStore) {
+      normr2[k] = t;
+    } else if (finish == kBBeta) {  // CG.cpp:112-114
+      oldrtz[k] = rtz[k];
+      rtz[k] = t;
+      beta[k] = t / oldrtz[k];
+    } else {  // CG.cpp:119-120
+      alpha[k] = rtz[k] / t;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *ticket = 0u;
+}
+
+// ---- FP64 outer kernels ----------------------------------------------------
+// Y = A X and p_k . (A p)_k (CG.cpp:118-120), slots in the reference's order.
+template <bool STREAM>
+__global__ __launch_bounds__(kBlock) void k_bspmv_dot(int64_t n, int s,
+                                                      const int32_t *__restrict__ col,
+                                                      const double *__restrict__ val,
+                                                      const double *__restrict__ X,
+                                                      double *__restrict__ Y, bool dot,
+                                                      double *partials, unsigned *ticket,
+                                                      double *sc) {
+  const int64_t total = n * s, stride = (int64_t)gridDim.x * blockDim.x;
+  double acc = 0.0;
+  for (int64_t t = gtid(); t < total; t += stride) {
+    const int64_t i = t / s;
+    const int k = (int)(t - i * s);
+    double sum = 0.0;
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const int64_t e = (int64_t)j * n + i;
+      sum += mload<STREAM>(val + e) * X[(int64_t)mload<STREAM>(col + e) * s + k];
+    }
+    Y[t] = sum;
+    acc += X[t] * sum;
+  }
+  if (dot) block_finish_s(acc, s, partials, ticket, sc, kBAlpha);
+}
+
+// r = b - Ap, r32 = (float)r, |r|^2 per right-hand side (CG.cpp:88-89)
+__global__ __launch_bounds__(kBlock) void k_bresidual0(int64_t total, int s,
+                                                       const double *__restrict__ B,
+                                                       const double *__restrict__ AP,
+                                                       double *__restrict__ R,
+                                                       float *__restrict__ R32, double *partials,
+                                                       unsigned *ticket, double *sc) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  double acc = 0.0;
+  for (int64_t t = gtid(); t < total; t += stride) {
+    const double r = B[t] - AP[t];
+    R[t] = r;
+    R32[t] = (float)r;
+    acc += r * r;
+  }
+  block_finish_s(acc, s, partials, ticket, sc, kBStore);
+}
+
+// r_k . z_k with z the FP32 V-cycle output widened exactly; beta (CG.cpp:110-114)
+__global__ __launch_bounds__(kBlock) void k_bdot32(int64_t total, int s,
+                                                   const double *__restrict__ R,
+                                                   const float *__restrict__ Z32,
+                                                   double *partials, unsigned *ticket,
+                                                   double *sc) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  double acc = 0.0;
+  for (int64_t t = gtid(); t < total; t += stride) acc += R[t] * (double)Z32[t];
+  block_finish_s(acc, s, partials, ticket, sc, kBBeta);
+}
+
+// p = z (first iteration) or p = z + beta_k p (CG.cpp:109, 115)
+__global__ __launch_bounds__(kBlock) void k_bupdate_p32(int64_t total, int s,
+                                                        const float *__restrict__ Z32,
+                                                        double *__restrict__ Pv,
+                                                        const double *__restrict__ sc,
+                                                        bool first) {
+  const int64_t t = gtid();
+  if (t >= total) return;
+  const int k = (int)(t % s);
+  Pv[t] = first ? (double)Z32[t] : (double)Z32[t] + sc[3 * s + k] * Pv[t];
+}
+
+// x += alpha_k p; r -= alpha_k Ap; r32 = (float)r; |r|^2 (CG.cpp:121-124)
+__global__ __launch_bounds__(kBlock) void k_bupdate_xr(int64_t total, int s,
+                                                       double *__restrict__ X,
+                                                       double *__restrict__ R,
+                                                       const double *__restrict__ Pv,
+                                                       const double *__restrict__ AP,
+                                                       float *__restrict__ R32, double *partials,
+                                                       unsigned *ticket, double *sc) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  double acc = 0.0;
+  for (int64_t t = gtid(); t < total; t += stride) {
+    const int k = (int)(t % s);
+    const double alpha = sc[2 * s + k];
+    X[t] = X[t] + alpha * Pv[t];
+    const double r = R[t] + (-alpha) * AP[t];
+    R[t] = r;
+    R32[t] = (float)r;
+    acc += r * r;
+  }
+  block_finish_s(acc, s, partials, ticket, sc, kBStore);
+}
+
+// ---- FP32 V-cycle kernels (mixed.hip's arithmetic, s vectors) ------------
+__global__ __launch_bounds__(kBlock) void k_bfill32(float *x, int64_t total) {
+  const int64_t t = gtid();
+  if (t < total) x[t] = 0.0f;
+}
+
+// One colour step: rows [begin, end), all s right-hand sides.
+template <bool STREAM>
+__global__ __launch_bounds__(kBlock) void k_bgs32(int64_t begin, int64_t end, int64_t n, int s,
+                                                  const int32_t *__restrict__ col,
+                                                  const half_t *__restrict__ val16,
+                                                  const float *__restrict__ diag32,
+                                                  const float *__restrict__ R, float *X) {
+  const int64_t t = gtid();
+  const int64_t i = begin + t / s;
+  if (i >= end) return;
+  const int k = (int)(t % s);
+  float sum = R[i * s + k];
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int64_t e = (int64_t)j * n + i;
+    sum -= (float)mload<STREAM>(val16 + e) * X[(int64_t)mload<STREAM>(col + e) * s + k];
+  }
+  X[i * s + k] = sum / diag32[i];
+}
+
+// One level of the natural-order schedule (rows listed), all s vectors.
+__global__ __launch_bounds__(kBlock) void k_bgs32_list(const int32_t *__restrict__ rows,
+                                                       int64_t count, int64_t n, int s,
+                                                       const int32_t *__restrict__ col,
+                                                       const half_t *__restrict__ val16,
+                                                       const float *__restrict__ diag32,
+                                                       const float *__restrict__ R, float *X) {
+  const int64_t t = gtid();
+  if (t >= count * s) return;
+  const int64_t i = rows[t / s];
+  const int k = (int)(t % s);
+  float sum = R[i * s + k];
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int64_t e = (int64_t)j * n + i;
+    sum -= (float)val16[e] * X[(int64_t)col[e] * s + k];
+  }
+  X[i * s + k] = sum / diag32[i];
+}
+
+// rc = rf[f2c] - (A xf)[f2c] in FP32 (mixed.hip's k_resid_restrict32)
+template <bool STREAM>
+__global__ __launch_bounds__(kBlock) void k_bresid_restrict32(
+    int64_t nc, int64_t n, int s, const int32_t *__restrict__ f2c,
+    const int32_t *__restrict__ col, const half_t *__restrict__ val16,
+    const float *__restrict__ diag32, const float *__restrict__ RF, const float *__restrict__ XF,
+    float *__restrict__ RC) {
+  const int64_t t = gtid();
+  if (t >= nc * s) return;
+  const int64_t c = t / s;
+  const int k = (int)(t % s);
+  const int64_t f = f2c[c];
+  float a = 0.0f;
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int64_t e = (int64_t)j * n + f;
+    a += (float)mload<STREAM>(val16 + e) * XF[(int64_t)mload<STREAM>(col + e) * s + k];
+  }
+  a += diag32[f] * XF[f * s + k];
+  RC[t] = RF[f * s + k] - a;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bprolong32(int64_t nc, int s,
+                                                       const int32_t *__restrict__ f2c,
+                                                       const float *__restrict__ XC,
+                                                       float *__restrict__ XF) {
+  const int64_t t = gtid();
+  if (t >= nc * s) return;
+  const int64_t c = t / s;
+  const int k = (int)(t % s);
+  XF[(int64_t)f2c[c] * s + k] += XC[t];
+}
+
+unsigned block_reduce_grid(int64_t total, int s) {
+  // whole workgroups of kBlock threads; the stride kBlock*G is a multiple of s
+  const int64_t g = std::min<int64_t>((total + kBlock - 1) / kBlock, kReduceBlocks);
+  return (unsigned)std::max<int64_t>(1, g);
+}
+
+void bsymgs32(Problem &P, int l, int ordering, int s, const float *R, float *X,
+              hipStream_t st) {
+  const Level &L = P.L[l];
+  if (ordering == HPCGX_ORDER_MULTICOLOR) {
+    for (int pass = 0; pass < 2; ++pass)
+      for (int q = 0; q < L.ncolors; ++q) {
+        const int c = pass == 0 ? q : L.ncolors - 1 - q;
+        const int64_t b = L.color_ptr[c], e = L.color_ptr[c + 1];
+        if (e <= b) continue;
+        const unsigned g = grid_for((e - b) * s);
+        if (L.stream)
+          k_bgs32<true><<<g, kBlock, 0, st>>>(b, e, L.n, s, L.col, L.val16, L.diag32, R, X);
+        else
+          k_bgs32<false><<<g, kBlock, 0, st>>>(b, e, L.n, s, L.col, L.val16, L.diag32, R, X);
+        HPCGX_LAUNCHED();
+      }
+  } else {
+    for (int pass = 0; pass < 2; ++pass)
+      for (int q = 0; q < L.nsched; ++q) {
+        const int t = pass == 0 ? q : L.nsched - 1 - q;
+        const int64_t cnt = L.level_ptr[t + 1] - L.level_ptr[t];
+        k_bgs32_list<<<grid_for(cnt * s), kBlock, 0, st>>>(L.level_rows + L.level_ptr[t], cnt,
+                                                             L.n, s, L.col, L.val16, L.diag32,
+                                                             R, X);
+        HPCGX_LAUNCHED();
+      }
+  }
+}
+
+double read_block_scalars(const Problem &P, int s, std::vector<double> &norm2, hipStream_t st) {
+  norm2.resize(s);
+  HPCGX_HIP(hipMemcpyAsync(norm2.data(), P.blk.scalars + 4 * s, sizeof(double) * s,
+                           hipMemcpyDeviceToHost, st));
+  HPCGX_HIP(hipStreamSynchronize(st));
+  return 0.0;
+}
+
+}  // namespace
+
+void prepare_block(Problem &P, int s) {
+  HPCGX_REQUIRE(s >= 1 && s <= 64 && 64 % s == 0, "s must divide 64");
+  prepare_mixed(P);
+  auto &B = P.blk;
+  if (B.s >= s) return;
+  B.release(P);
+  const int64_t n = P.L[0].n;
+  B.R = P.arena.alloc<double>((size_t)n * s);
+  B.Pv = P.arena.alloc<double>((size_t)n * s);
+  B.AP = P.arena.alloc<double>((size_t)n * s);
+  B.R32 = P.arena.alloc<float>((size_t)n * s);
+  B.Z32 = P.arena.alloc<float>((size_t)n * s);
+  for (int l = 0; l + 1 < P.nlevels; ++l) {
+    B.rc32[l] = P.arena.alloc<float>((size_t)P.L[l].nc * s);
+    B.xc32[l] = P.arena.alloc<float>((size_t)P.L[l].nc * s);
+  }
+  B.partials = P.arena.alloc<double>((size_t)kReduceBlocks * s);
+  B.ticket = P.arena.alloc<unsigned>(1);
+  HPCGX_HIP(hipMemset(B.ticket, 0, sizeof(unsigned)));
+  B.scalars = P.arena.alloc<double>((size_t)5 * s);
+  HPCGX_HIP(hipMemset(B.scalars, 0, sizeof(double) * 5 * s));
+  B.s = s;
+}
+
+void Problem::BlockState::release(Problem &P) {
+  const int64_t n = P.L[0].n;
+  if (!s) return;
+  P.arena.release(R, sizeof(double) * n * s);
+  P.arena.release(Pv, sizeof(double) * n * s);
+  P.arena.release(AP, sizeof(double) * n * s);
+  P.arena.release(R32, sizeof(float) * n * s);
+  P.arena.release(Z32, sizeof(float) * n * s);
+  for (int l = 0; l + 1 < P.nlevels; ++l) {
+    P.arena.release(rc32[l], sizeof(float) * P.L[l].nc * s);
+    P.arena.release(xc32[l], sizeof(float) * P.L[l].nc * s);
+  }
+  P.arena.release(partials, sizeof(double) * kReduceBlocks * s);
+  P.arena.release(ticket, sizeof(unsigned));
+  P.arena.release(scalars, sizeof(double) * 5 * s);
+  s = 0;
+}
+
+// ComputeMG_ref.cpp:39-63 in FP32 for s vectors (mixed.hip's mg32).
+void mg32_block(Problem &P, int l, int ordering, int s, const float *R, float *X,
+                hipStream_t st) {
+  Level &L = P.L[l];
+  k_bfill32<<<grid_for(L.n * s), kBlock, 0, st>>>(X, L.n * s);
+  HPCGX_LAUNCHED();
+  bsymgs32(P, l, ordering, s, R, X, st);
+  if (l + 1 < P.nlevels) {
+    float *rc = P.blk.rc32[l], *xc = P.blk.xc32[l];
+    const unsigned g = grid_for(L.nc * s);
+    if (L.stream)
+      k_bresid_restrict32<true><<<g, kBlock, 0, st>>>(L.nc, L.n, s, L.f2c, L.col, L.val16,
+                                                      L.diag32, R, X, rc);
+    else
+      k_bresid_restrict32<false><<<g, kBlock, 0, st>>>(L.nc, L.n, s, L.f2c, L.col, L.val16,
+                                                       L.diag32, R, X, rc);
+    HPCGX_LAUNCHED();
+    mg32_block(P, l + 1, ordering, s, rc, xc, st);
+    k_bprolong32<<<g, kBlock, 0, st>>>(L.nc, s, L.f2c, xc, X);
+    HPCGX_LAUNCHED();
+    bsymgs32(P, l, ordering, s, R, X, st);
+  }
+}
+
+void spmv_block(Problem &P, int l, int s, const double *X, double *Y, hipStream_t st) {
+  const Level &L = P.L[l];
+  const unsigned g = grid_for(L.n * s);
+  if (L.stream)
+    k_bspmv_dot<true><<<g, kBlock, 0, st>>>(L.n, s, L.col, L.val, X, Y, false, nullptr,
+                                            nullptr, nullptr);
+  else
+    k_bspmv_dot<false><<<g, kBlock, 0, st>>>(L.n, s, L.col, L.val, X, Y, false, nullptr,
+                                             nullptr, nullptr);
+  HPCGX_LAUNCHED();
+}
+
+// CG.cpp:59-143 for s right-hand sides with the FP32 V-cycle (cg_mixed's
+// arithmetic for each of them).  Iterates until every system reaches the
+// tolerance or max_iter; normr/normr0 per right-hand side.
+void cg_block_mixed(Problem &P, int ordering, int s, const double *Bv, double *X, int max_iter,
+                    double tol, int recompute, int *niters, double *normr, double *normr0,
+                    double *times, hipStream_t st) {
+  prepare_block(P, s);
+  const auto t0 = std::chrono::steady_clock::now();
+  auto &B = P.blk;
+  const Level &A = P.L[0];
+  const int64_t total = A.n * s;
+  const unsigned gr = block_reduce_grid(total, s);
+  auto spmv_dot = [&](const double *x, double *y, bool dot) {
+    const unsigned g = dot ? gr : grid_for(total);
+    if (A.stream)
+      k_bspmv_dot<true><<<g, kBlock, 0, st>>>(A.n, s, A.col, A.val, x, y, dot, B.partials,
+                                              B.ticket, B.scalars);
+    else
+      k_bspmv_dot<false><<<g, kBlock, 0, st>>>(A.n, s, A.col, A.val, x, y, dot, B.partials,
+                                               B.ticket, B.scalars);
+    HPCGX_LAUNCHED();
+  };
+  auto residual = [&]() {  // r = b - A x, r32, |r|^2
+    spmv_dot(X, B.AP, false);
+    k_bresidual0<<<gr, kBlock, 0, st>>>(total, s, Bv, B.AP, B.R, B.R32, B.partials, B.ticket,
+                                        B.scalars);
+    HPCGX_LAUNCHED();
+  };
+  std::vector<double> n2;
+  *niters = 0;
+  residual();  // :86-89 (p = x, Ap = A p, r = b - Ap)
+  read_block_scalars(P, s, n2, st);
+  for (int k = 0; k < s; ++k) normr0[k] = normr[k] = std::sqrt(n2[k]);
+  auto done = [&]() {
+    for (int k = 0; k < s; ++k)
+      if (normr[k] / normr0[k] > tol) return false;
+    return true;
+  };
+  const bool check_each = tol > 0.0;
+  for (int it = 1; it <= max_iter && !done(); ++it) {
+    mg32_block(P, 0, ordering, s, B.R32, B.Z32, st);
+    k_bdot32<<<gr, kBlock, 0, st>>>(total, s, B.R, B.Z32, B.partials, B.ticket, B.scalars);
+    HPCGX_LAUNCHED();
+    k_bupdate_p32<<<grid_for(total), kBlock, 0, st>>>(total, s, B.Z32, B.Pv, B.scalars, it == 1);
+    HPCGX_LAUNCHED();
+    spmv_dot(B.Pv, B.AP, true);
+    k_bupdate_xr<<<gr, kBlock, 0, st>>>(total, s, X, B.R, B.Pv, B.AP, B.R32, B.partials,
+                                        B.ticket, B.scalars);
+    HPCGX_LAUNCHED();
+    if (recompute > 0 && it % recompute == 0) residual();
+    *niters = it;
+    if (check_each || it == max_iter) {
+      read_block_scalars(P, s, n2, st);
+      for (int k = 0; k < s; ++k) normr[k] = std::sqrt(n2[k]);
+    }
+  }
+  if (!check_each) {
+    read_block_scalars(P, s, n2, st);
+    for (int k = 0; k < s; ++k) normr[k] = std::sqrt(n2[k]);
+  }
+  HPCGX_HIP(hipStreamSynchronize(st));
+  if (times)
+    times[0] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (int k = 0; k < s; ++k)
+    if (!std::isfinite(normr[k])) throw Error(HPCGX_EINVAL, "CG residual is not finite");
+}
+
+}  // namespace hpcgx

@@ -245,6 +245,7 @@ int hpcgx_problem_info(const hpcgx_problem *p, int level, hpcgx_level_info *info
     info->nslabs = L.nslabs;
     info->col16 = L.c16 ? 1 : 0;
     info->wide_tiles = L.wide_tiles;
+    info->byte_tiles = L.byte_tiles;
   });
 }
 
@@ -388,8 +389,13 @@ int hpcgx_tune(hpcgx_problem *p, const char *key, double value) {
     if (!p || !key) throw Error(HPCGX_EINVAL, "NULL argument");
     const std::string k(key);
     hpcgx::Problem &P = p->P;
+    P.graph.reset();  // kernel choices may change: recapture on next use
     if (k == "col16") {
       P.use_c16 = value != 0.0;
+      return;
+    }
+    if (k == "col8") {
+      P.use_c8 = value != 0.0;
       return;
     }
     if (k == "spmv_dot_blocks") {
@@ -526,6 +532,39 @@ int hpcgx_cg(hpcgx_problem *p, int ordering, const double *b, double *x, int max
     if (max_iter < 0) throw Error(HPCGX_EINVAL, "negative max_iter");
     hpcgx::cg(p->P, ordering, b, x, max_iter, tolerance, precondition != 0, flags, niters,
               normr, normr0, times, S(stream));
+  });
+}
+
+int hpcgx_spmv_block(hpcgx_problem *p, int level, int s, const double *X, double *Y,
+                     void *stream) {
+  return guard([&] {
+    level_of(p, level);
+    if (!X || !Y) throw Error(HPCGX_EINVAL, "NULL vector");
+    if (s < 1 || s > 64 || 64 % s) throw Error(HPCGX_EINVAL, "s must divide 64");
+    hpcgx::spmv_block(p->P, level, s, X, Y, S(stream));
+  });
+}
+
+int hpcgx_mg32_block(hpcgx_problem *p, int level, int ordering, int s, const float *R,
+                     float *X, void *stream) {
+  return guard([&] {
+    level_of(p, level);
+    check_order(ordering);
+    if (!R || !X) throw Error(HPCGX_EINVAL, "NULL vector");
+    hpcgx::prepare_block(p->P, s);
+    hpcgx::mg32_block(p->P, level, ordering, s, R, X, S(stream));
+  });
+}
+
+int hpcgx_cg_block_mixed(hpcgx_problem *p, int ordering, int s, const double *B, double *X,
+                         int max_iter, double tolerance, int recompute, int *niters,
+                         double *normr, double *normr0, double *times, void *stream) {
+  return guard([&] {
+    if (!p || !B || !X || !niters || !normr || !normr0) throw Error(HPCGX_EINVAL, "NULL argument");
+    check_order(ordering);
+    if (max_iter < 0) throw Error(HPCGX_EINVAL, "negative max_iter");
+    hpcgx::cg_block_mixed(p->P, ordering, s, B, X, max_iter, tolerance, recompute, niters, normr,
+                          normr0, times, S(stream));
   });
 }
 

@@ -17,13 +17,15 @@ constexpr int kSlots = 27;        // ELL width: the 27-point stencil's maximum
 constexpr int kMaxLevels = 8;
 constexpr int kMaxColors = 32;    // first-fit colours of a <=27-entry row: <=27
 constexpr int kBlock = 256;       // threads per workgroup for streaming kernels
-constexpr int kReduceBlocks = 1024;  // partial sums of a grid reduction
+constexpr int kReduceBlocks = 4096;  // workgroups of a grid-stride reduction
 constexpr int kMaxPartials = 8192;   // partial-sum slots (largest reducing grid)
 constexpr double kStreamBytes = 512e6;  // matrices above this are streamed
 constexpr int64_t kWaveMinRows = 1 << 23;  // levels swept as a wavefront
 constexpr int kWaveSlabFactor = 2;         // slab width / matrix bandwidth
 constexpr int kWaveRows = 2;               // rows per thread of a wavefront task
-constexpr int kTile = 64;  // rows sharing 16-bit column bases (one wavefront)
+constexpr int kTile = 64;  // rows sharing narrow column bases (one wavefront)
+constexpr int kSlotWords = (kSlots + 3) / 4;  // 8-bit offsets of 4 slots per word
+enum TileClass { kTile8 = 0, kTile32 = 1, kTile16 = 2 };
 constexpr int kBandLines = 1;  // grid lines per band of the in-colour order (1 = natural)
 
 struct Error : std::runtime_error {
@@ -123,21 +125,30 @@ struct Level {
   // zeroed, the FP32 diagonal, FP32 coarse-level vectors
   _Float16 *val16 = nullptr;
   uint32_t *vc = nullptr;  // c16 levels: col16 << 16 | FP16 bits, one word per slot
+  // c16 levels: per row and group g of 4 slots, 12 bytes: FP16 values of
+  // slots 4g, 4g+1 | 4g+2, 4g+3 and the 8-bit offsets (kTile8 tiles)
+  uint32_t *q4 = nullptr;
   float *diag32 = nullptr;
   float *rc32 = nullptr, *xc32 = nullptr;
-  // 16-bit column ids.  Tile t = device rows [kTile t, kTile t + kTile).  The
+  // Narrow column ids.  Tile t = device rows [kTile t, kTile t + kTile).  The
   // slots of a tile are aligned: slot j holds the tile's j-th distinct
   // natural column offset (col - row in natural numbering, ascending); a row
   // without that offset holds a valid column and the value 0 there, so each
   // row still meets its stored entries in ascending column order and the
   // extra terms add exact zeros.  Column of slot j of row i in tile t:
-  // cbase[j * ntiles + t] + col16[j * n + i], unless tinfo[t] != 0 (more
-  // than 27 distinct offsets, or a slot spanning more than 65535 columns:
-  // the tile keeps its 32-bit ids, which `col` holds for every tile).
+  //   tinfo[t] == kTile8:  cbase[j * ntiles + t] + byte (j % 4) of
+  //                        off8[(j / 4) * n + i]   (every slot spans <= 255)
+  //   tinfo[t] == kTile16: cbase[j * ntiles + t] + col16[j * n + i]
+  //   tinfo[t] == kTile32: col[j * n + i] (more than 27 distinct offsets, or
+  //                        a slot spanning more than 65535 columns)
+  // `col` holds the 32-bit ids of every tile, col16 those of every 8- and
+  // 16-bit tile.
   bool c16 = false;
   int64_t ntiles = 0;
-  int64_t wide_tiles = 0;
+  int64_t wide_tiles = 0;   // kTile32 tiles
+  int64_t byte_tiles = 0;   // kTile8 tiles
   uint16_t *col16 = nullptr;
+  uint32_t *off8 = nullptr;  // [kSlotWords * n]
   int32_t *cbase = nullptr;
   int32_t *tinfo = nullptr;
   // transfer to the next level: f2c[coarse device row] = fine device row
@@ -161,10 +172,42 @@ struct Problem {
   int *wave_err = nullptr;    // set when a wavefront wait gives up
   int wave_grid = 0;          // co-resident workgroups of the wavefront kernel
   bool use_wave = false;      // multicolour sweeps as wavefronts (measured slower)
-  bool use_c16 = true;        // read 16-bit column ids where a level has them
+  bool use_c16 = true;        // read narrow column ids where a level has them
+  bool use_c8 = true;         // ... 8-bit ones where a tile has them
   int spmv_dot_blocks = 4096; // persistent grid of the fused SpMV + dot
   bool mixed_ready = false;   // FP16/FP32 copies built
   float *r32 = nullptr, *z32 = nullptr;  // FP32 input / output of the V-cycle
+  // block PCG over s right-hand sides (block.hip), allocated on first use:
+  // [n][s] multivectors of the fine level, FP32 coarse multivectors,
+  // per-right-hand-side scalars (rtz, old rtz, alpha, beta, |r|^2: s each)
+  struct BlockState {
+    int s = 0;
+    double *R = nullptr, *Pv = nullptr, *AP = nullptr;
+    float *R32 = nullptr, *Z32 = nullptr;
+    float *rc32[kMaxLevels] = {}, *xc32[kMaxLevels] = {};
+    double *partials = nullptr;
+    unsigned *ticket = nullptr;
+    double *scalars = nullptr;
+    void release(Problem &P);
+  } blk;
+  // captured CG iteration (HPCGX_CG_GRAPH): body of iterations 2.. for one
+  // (x, ordering, preconditioning) key; captured on a private stream
+  struct IterGraph {
+    hipGraphExec_t exec = nullptr;
+    hipStream_t cap = nullptr;
+    const double *x = nullptr;
+    int ordering = -1;
+    bool precond = false;
+    void reset() {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      exec = nullptr;
+      x = nullptr;
+    }
+    ~IterGraph() {
+      reset();
+      if (cap) (void)hipStreamDestroy(cap);
+    }
+  } graph;
   // event log of the finest level's multicolour Gauss-Seidel launches
   struct KernelLog {
     bool on = false;
@@ -181,11 +224,16 @@ struct Problem {
 // one tile and its bases are wave-uniform (read through the scalar cache).
 struct Col16 {
   const uint16_t *off;
+  const uint32_t *off8;
+  const uint32_t *q4;  // FP32 V-cycle records of kTile8 tiles (Level::q4)
   const int32_t *base;
   const int32_t *info;
   int64_t ntiles;
+  int use8;  // read 8-bit offsets in kTile8 tiles (else their 16-bit ids)
 };
-inline Col16 col16_of(const Level &L) { return Col16{L.col16, L.cbase, L.tinfo, L.ntiles}; }
+inline Col16 col16_of(const Problem &P, const Level &L) {
+  return Col16{L.col16, L.off8, L.q4, L.cbase, L.tinfo, L.ntiles, P.use_c8 ? 1 : 0};
+}
 // 16-bit kernels apply to rows [begin, ...) of L when this holds.
 inline bool use_col16(const Problem &P, const Level &L, int64_t begin) {
   return P.use_c16 && L.c16 && begin % kTile == 0;
@@ -220,7 +268,13 @@ void launch_dot(Problem &P, int64_t n, const double *x, const double *y,
 void launch_spmv_dot(Problem &P, const Level &L, const double *p, double *Ap,
                      hipStream_t s);
 void launch_update_xr(Problem &P, int64_t n, double *x, double *r,
-                      const double *p, const double *Ap, hipStream_t s);
+                      const double *p, const double *Ap, hipStream_t s,
+                      float *r32 = nullptr);
+// mixed precision: dot with, and p update from, the FP32 V-cycle output
+void launch_dot32(Problem &P, int64_t n, const double *x, const float *y, double *out,
+                  int finish, hipStream_t s);
+void launch_update_p32(int64_t n, const float *z, double *p, const double *scalars,
+                       hipStream_t s);
 void launch_update_p(int64_t n, const double *z, double *p,
                      const double *scalars, hipStream_t s);
 void launch_residual0(Problem &P, int64_t n, const double *b, const double *Ap,
@@ -263,6 +317,14 @@ void spmv32(const Problem &P, const Level &L, const float *x, float *y, hipStrea
 void mg32(Problem &P, int l, int ordering, const float *r, float *x,
           hipStream_t s);
 void to_f32(int64_t n, const double *in, float *out, hipStream_t s);
+// ---- block PCG (block.hip) ----
+void prepare_block(Problem &P, int s);
+void mg32_block(Problem &P, int l, int ordering, int s, const float *R, float *X,
+                hipStream_t st);
+void spmv_block(Problem &P, int l, int s, const double *X, double *Y, hipStream_t st);
+void cg_block_mixed(Problem &P, int ordering, int s, const double *B, double *X, int max_iter,
+                    double tol, int recompute, int *niters, double *normr, double *normr0,
+                    double *times, hipStream_t st);
 void to_f64(int64_t n, const float *in, double *out, hipStream_t s);
 void cg_mixed(Problem &P, int ordering, const double *b, double *x,
               int max_iter, double tol, int recompute, int *niters,

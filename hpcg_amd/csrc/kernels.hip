@@ -151,10 +151,37 @@ __device__ inline void gs_row(int64_t i, int64_t n,
   x[i] = sum / d;
 }
 
-// 16-bit column ids (Level::col16).  The caller's wavefront lies in one tile,
-// so the tile id, its "32-bit only" flag and its slot bases are uniform.
+// Narrow column ids (Level::col16 / off8).  The caller's wavefront lies in
+// one tile, so the tile id, its class and its slot bases are uniform (read
+// through the scalar cache).  narrow_row() calls f(j, k, c) for every slot j
+// in order, with k = j*n + i and c the slot's column.
 __device__ inline int tile_of(int64_t i) {
   return __builtin_amdgcn_readfirstlane((int)(i / kTile));
+}
+
+template <bool STREAM, class F>
+__device__ inline void narrow_row(const Col16 &C, int t, int info, int64_t n, int64_t i,
+                                  F &&f) {
+  const int32_t *__restrict__ base = C.base + t;
+  if (info == kTile8 && C.use8) {
+#pragma unroll
+    for (int g = 0; g < kSlotWords; ++g) {
+      const uint32_t w = mload<STREAM>(C.off8 + (int64_t)g * n + i);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int j = 4 * g + b;
+        if (j < kSlots)
+          f(j, (int64_t)j * n + i,
+            base[(int64_t)j * C.ntiles] + (int32_t)((w >> (8 * b)) & 0xffu));
+      }
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const int64_t k = (int64_t)j * n + i;
+      f(j, k, base[(int64_t)j * C.ntiles] + (int32_t)mload<STREAM>(C.off + k));
+    }
+  }
 }
 
 template <bool STREAM>
@@ -163,15 +190,12 @@ __device__ inline double row_dot16(const Col16 &C, const int32_t *__restrict__ c
                                    const double *__restrict__ x, int64_t n,
                                    int64_t i) {
   const int t = tile_of(i);
-  if (C.info[t]) return row_dot<STREAM>(col, val, x, n, i);
-  const int32_t *__restrict__ base = C.base + t;
+  const int info = C.info[t];
+  if (info == kTile32) return row_dot<STREAM>(col, val, x, n, i);
   double sum = 0.0;
-#pragma unroll
-  for (int j = 0; j < kSlots; ++j) {
-    const int64_t k = (int64_t)j * n + i;
-    const int32_t c = base[(int64_t)j * C.ntiles] + (int32_t)mload<STREAM>(C.off + k);
+  narrow_row<STREAM>(C, t, info, n, i, [&](int, int64_t k, int32_t c) {
     sum += mload<STREAM>(val + k) * x[c];
-  }
+  });
   return sum;
 }
 
@@ -181,21 +205,18 @@ __device__ inline void gs_row16(const Col16 &C, int64_t i, int64_t n,
                                 const double *__restrict__ val,
                                 const double *__restrict__ r, double *x) {
   const int t = tile_of(i);
-  if (C.info[t]) {
+  const int info = C.info[t];
+  if (info == kTile32) {
     gs_row<STREAM>(i, n, col, val, r, x);
     return;
   }
-  const int32_t *__restrict__ base = C.base + t;
   double sum = r[i];
   double d = 0.0;
-#pragma unroll
-  for (int j = 0; j < kSlots; ++j) {
-    const int64_t k = (int64_t)j * n + i;
+  narrow_row<STREAM>(C, t, info, n, i, [&](int, int64_t k, int32_t c) {
     const double v = mload<STREAM>(val + k);
-    const int32_t c = base[(int64_t)j * C.ntiles] + (int32_t)mload<STREAM>(C.off + k);
     sum -= v * x[c];
     if (c == i && v != 0.0) d = v;
-  }
+  });
   sum += x[i] * d;
   x[i] = sum / d;
 }
@@ -518,11 +539,13 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(
   grid_finish(s, partials, ticket, scalars, nullptr, kFinishAlpha);
 }
 
-// x += alpha p; r -= alpha Ap; r.r (CG.cpp:121-124).
+// x += alpha p; r -= alpha Ap; r.r (CG.cpp:121-124).  R32: also writes r
+// rounded to FP32, the input of the mixed-precision V-cycle.
+template <bool R32>
 __global__ __launch_bounds__(kBlock) void k_update_xr(
     int64_t n, double *__restrict__ x, double *__restrict__ r,
     const double *__restrict__ p, const double *__restrict__ Ap,
-    double *partials, unsigned *ticket, double *scalars) {
+    float *__restrict__ r32, double *partials, unsigned *ticket, double *scalars) {
   const double alpha = scalars[kAlpha];
   const double malpha = -alpha;
   double s = 0.0;
@@ -531,10 +554,30 @@ __global__ __launch_bounds__(kBlock) void k_update_xr(
     x[i] = x[i] + alpha * p[i];
     const double ri = r[i] + malpha * Ap[i];
     r[i] = ri;
+    if (R32) r32[i] = (float)ri;
     s += ri * ri;
   }
   s = block_sum(s);
   grid_finish(s, partials, ticket, scalars, &scalars[kNormr2], kFinishStore);
+}
+
+// r.z with z the FP32 V-cycle output, widened exactly (CG.cpp:110, 113).
+__global__ __launch_bounds__(kBlock) void k_dot32(int64_t n, const double *x, const float *y,
+                                                   double *partials, unsigned *ticket,
+                                                   double *scalars, double *out, int finish) {
+  double s = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = gtid(); i < n; i += stride) s += x[i] * (double)y[i];
+  s = block_sum(s);
+  grid_finish(s, partials, ticket, scalars, out, finish);
+}
+
+// p = z + beta p with z in FP32, widened exactly (CG.cpp:115).
+__global__ __launch_bounds__(kBlock) void k_update_p32(int64_t n, const float *__restrict__ z,
+                                                        double *__restrict__ p,
+                                                        const double *__restrict__ scalars) {
+  const int64_t i = gtid();
+  if (i < n) p[i] = (double)z[i] + scalars[kBeta] * p[i];
 }
 
 // r = b - Ap and r.r (CG.cpp:88-89)
@@ -588,7 +631,7 @@ void launch_gather(const int32_t *idx, const double *src, double *dst,
 void launch_spmv(const Problem &P, const Level &L, const double *x, double *y,
                  hipStream_t s) {
   const unsigned g = (unsigned)L.nchunks;
-  const Col16 C = col16_of(L);
+  const Col16 C = col16_of(P, L);
   const bool c16 = use_col16(P, L, 0);
   if (L.stream && c16)
     k_spmv<true, true><<<g, kBlock, 0, s>>>(L.chunks, L.n, C, L.col, L.val, x, y, L.xcd);
@@ -605,7 +648,7 @@ void launch_symgs_rows(const Problem &P, const Level &L, int64_t begin, int64_t 
                        const double *r, double *x, hipStream_t s) {
   if (end <= begin) return;
   if (use_col16(P, L, begin)) {
-    const Col16 C = col16_of(L);
+    const Col16 C = col16_of(P, L);
     const unsigned g = grid_for(end - begin);
     if (L.finest)
       k_gs_range16<true, true><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col, L.val, r, x);
@@ -694,7 +737,7 @@ void launch_spmv_dot(Problem &P, const Level &L, const double *p, double *Ap,
                      hipStream_t s) {
   int64_t g = std::min<int64_t>(L.nchunks, std::min(P.spmv_dot_blocks, kMaxPartials));
   g = std::max<int64_t>(8, (g + 7) / 8 * 8);
-  const Col16 C = col16_of(L);
+  const Col16 C = col16_of(P, L);
   const bool c16 = use_col16(P, L, 0);
   const unsigned G = (unsigned)g;
   if (L.stream && c16)
@@ -713,9 +756,26 @@ void launch_spmv_dot(Problem &P, const Level &L, const double *p, double *Ap,
 }
 
 void launch_update_xr(Problem &P, int64_t n, double *x, double *r,
-                      const double *p, const double *Ap, hipStream_t s) {
-  k_update_xr<<<reduce_grid(n), kBlock, 0, s>>>(n, x, r, p, Ap, P.partials,
-                                                P.ticket, P.scalars);
+                      const double *p, const double *Ap, hipStream_t s, float *r32) {
+  if (r32)
+    k_update_xr<true><<<reduce_grid(n), kBlock, 0, s>>>(n, x, r, p, Ap, r32, P.partials,
+                                                        P.ticket, P.scalars);
+  else
+    k_update_xr<false><<<reduce_grid(n), kBlock, 0, s>>>(n, x, r, p, Ap, nullptr, P.partials,
+                                                         P.ticket, P.scalars);
+  HPCGX_LAUNCHED();
+}
+
+void launch_dot32(Problem &P, int64_t n, const double *x, const float *y, double *out,
+                  int finish, hipStream_t s) {
+  k_dot32<<<reduce_grid(n), kBlock, 0, s>>>(n, x, y, P.partials, P.ticket, P.scalars, out,
+                                            finish);
+  HPCGX_LAUNCHED();
+}
+
+void launch_update_p32(int64_t n, const float *z, double *p, const double *scalars,
+                       hipStream_t s) {
+  k_update_p32<<<grid_for(n), kBlock, 0, s>>>(n, z, p, scalars);
   HPCGX_LAUNCHED();
 }
 

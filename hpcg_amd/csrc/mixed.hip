@@ -124,6 +124,60 @@ __device__ inline half_t vc_val(uint32_t w) {
   return __builtin_bit_cast(half_t, (uint16_t)(w & 0xffffu));
 }
 
+// kTile8 tiles in the FP32 V-cycle: slots in groups of four, each group one
+// 12-byte record per row (Level::q4): two words of FP16 values (slots
+// 4g..4g+3) and one word of 8-bit column offsets.  Seven loads per row
+// instead of 27: the FP32 colour step is bound by the number of vector
+// memory instructions it issues.
+struct Q4 {
+  uint32_t h01, h23, off;
+};
+
+template <bool STREAM>
+__device__ inline float sum8_32(const Col16 &C, int t, const uint32_t *__restrict__ q4,
+                                int64_t i, int64_t n, const float *__restrict__ x,
+                                float s, bool subtract) {
+  const int32_t *__restrict__ base = C.base + t;
+#pragma unroll
+  for (int g = 0; g < kSlotWords; ++g) {
+    const uint32_t *rec = q4 + 3 * ((int64_t)g * n + i);
+    const uint32_t h01 = mload<STREAM>(rec), h23 = mload<STREAM>(rec + 1),
+                   off = mload<STREAM>(rec + 2);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int j = 4 * g + b;
+      if (j < kSlots) {
+        const int32_t c = base[(int64_t)j * C.ntiles] + (int32_t)((off >> (8 * b)) & 0xffu);
+        const uint32_t hw = b < 2 ? h01 : h23;
+        const half_t h = __builtin_bit_cast(half_t, (uint16_t)((hw >> (16 * (b & 1))) & 0xffffu));
+        const float p = (float)h * x[c];
+        if (subtract)
+          s -= p;
+        else
+          s += p;
+      }
+    }
+  }
+  return s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_pack_q4(int64_t n, const half_t *__restrict__ val16,
+                                                    const uint32_t *__restrict__ off8,
+                                                    uint32_t *__restrict__ q4) {
+  const int64_t k = gtid();
+  if (k >= (int64_t)kSlotWords * n) return;
+  const int64_t g = k / n, i = k % n;
+  uint32_t h[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const int64_t j = 4 * g + b;
+    h[b] = j < kSlots ? (uint32_t)__builtin_bit_cast(uint16_t, val16[j * n + i]) : 0u;
+  }
+  q4[3 * k] = h[0] | (h[1] << 16);
+  q4[3 * k + 1] = h[2] | (h[3] << 16);
+  q4[3 * k + 2] = off8[k];
+}
+
 template <bool STREAM>
 __device__ inline void gs32_row16(const Col16 &C, const uint32_t *__restrict__ vc,
                                   int64_t i, int64_t n, const int32_t *__restrict__ col,
@@ -131,8 +185,13 @@ __device__ inline void gs32_row16(const Col16 &C, const uint32_t *__restrict__ v
                                   const float *__restrict__ diag32,
                                   const float *__restrict__ r, float *x) {
   const int t = tile_of(i);
-  if (C.info[t]) {
+  const int info = C.info[t];
+  if (info == kTile32) {
     gs32_row<STREAM>(i, n, col, val16, diag32, r, x);
+    return;
+  }
+  if (info == kTile8 && C.use8) {
+    x[i] = sum8_32<STREAM>(C, t, C.q4, i, n, x, r[i], true) / diag32[i];
     return;
   }
   const int32_t *__restrict__ base = C.base + t;
@@ -153,7 +212,10 @@ __device__ inline float row_dot32_16(const Col16 &C, const uint32_t *__restrict_
                                      const float *__restrict__ diag32,
                                      const float *__restrict__ x) {
   const int t = tile_of(i);
-  if (C.info[t]) return row_dot32<STREAM>(i, n, col, val16, diag32, x);
+  const int info = C.info[t];
+  if (info == kTile32) return row_dot32<STREAM>(i, n, col, val16, diag32, x);
+  if (info == kTile8 && C.use8)
+    return sum8_32<STREAM>(C, t, C.q4, i, n, x, 0.0f, false) + diag32[i] * x[i];
   const int32_t *__restrict__ base = C.base + t;
   float s = 0.0f;
 #pragma unroll
@@ -248,7 +310,7 @@ void gs32_step(const Problem &P, const Level &L, int64_t begin, int64_t end,
   if (end <= begin) return;
   const unsigned g = grid_for(end - begin);
   if (use_col16(P, L, begin)) {
-    const Col16 C = col16_of(L);
+    const Col16 C = col16_of(P, L);
     if (L.finest)
       k_gs32_range16<true, true><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.vc, L.col, L.val16,
                                                       L.diag32, r, x);
@@ -290,6 +352,10 @@ void prepare_mixed(Problem &P) {
       const int64_t m = (int64_t)kSlots * L.n;
       L.vc = P.arena.alloc<uint32_t>((size_t)m);
       k_pack_vc<<<grid_for(m), kBlock>>>(m, L.col16, L.val16, L.vc);
+      HPCGX_LAUNCHED();
+      const int64_t mg = (int64_t)kSlotWords * L.n;
+      L.q4 = P.arena.alloc<uint32_t>((size_t)3 * mg);
+      k_pack_q4<<<grid_for(mg), kBlock>>>(L.n, L.val16, L.off8, L.q4);
       HPCGX_LAUNCHED();
     }
     if (L.nc) {
@@ -334,7 +400,7 @@ void symgs32(Problem &P, int l, int ordering, const float *r, float *x,
 
 void spmv32(const Problem &P, const Level &L, const float *x, float *y, hipStream_t s) {
   if (use_col16(P, L, 0)) {
-    const Col16 C = col16_of(L);
+    const Col16 C = col16_of(P, L);
     if (L.stream)
       k_spmv32_16<true><<<(unsigned)L.nchunks, kBlock, 0, s>>>(L.chunks, L.n, C, L.vc,
                                                                L.col, L.val16, L.diag32, x, y);
@@ -411,21 +477,24 @@ void cg_mixed(Problem &P, int ordering, const double *b, double *x,
   double nr = std::sqrt(read_scalar(P, kNormr2, s));
   *normr0 = nr;
   const bool check_each = tol > 0.0;
+  // r32 = r rounded to FP32 is kept up to date by the r update itself; z
+  // stays FP32 (z32) and is widened inside the dot and the p update, which
+  // gives the bits of widening it first.
+  to_f32(n, P.r, P.r32, s);
   for (int k = 1; k <= max_iter && nr / *normr0 > tol; ++k) {
-    to_f32(n, P.r, P.r32, s);
     mg32(P, 0, ordering, P.r32, P.z32, s);
-    to_f64(n, P.z32, P.z, s);
-    launch_dot(P, n, P.r, P.z, nullptr, kFinishBeta, s);
+    launch_dot32(P, n, P.r, P.z32, nullptr, kFinishBeta, s);
     if (k == 1)
-      launch_copy(P.z, P.p, n, s);
+      to_f64(n, P.z32, P.p, s);
     else
-      launch_update_p(n, P.z, P.p, P.scalars, s);
+      launch_update_p32(n, P.z32, P.p, P.scalars, s);
     launch_spmv_dot(P, A, P.p, P.Ap, s);
-    launch_update_xr(P, n, x, P.r, P.p, P.Ap, s);
+    launch_update_xr(P, n, x, P.r, P.p, P.Ap, s, P.r32);
     if (recompute > 0 && k % recompute == 0) {
       // r = b - A x in FP64 (replaces the recursively updated r)
       launch_spmv(P, A, x, P.Ap, s);
       launch_residual0(P, n, b, P.Ap, P.r, s);
+      to_f32(n, P.r, P.r32, s);
     }
     *niters = k;
     if (check_each || k == max_iter) nr = std::sqrt(read_scalar(P, kNormr2, s));

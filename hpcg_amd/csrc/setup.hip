@@ -256,12 +256,13 @@ __device__ inline int wave_max_i32(int v) {
 //  2. Entries move to their slots (highest slot first, so nothing is
 //     overwritten before it is read); a row without an entry in a slot gets
 //     a column already in that slot's range and the value 0.  The slot's base
-//     is its smallest column; a slot spanning more than 65535 columns, or more
-//     than 27 distinct offsets in the tile, marks the tile 32-bit only.
+//     is its smallest column.  The tile's class: 8-bit offsets if every slot
+//     spans at most 255 columns, 16-bit if at most 65535, else (or with more
+//     than 27 distinct offsets) 32-bit ids only.
 __global__ __launch_bounds__(kBlock) void k_tile_layout(
     int64_t n, int64_t ntiles, const int32_t *__restrict__ iperm, int32_t *col,
     double *val, const uint8_t *__restrict__ nnz_row, uint8_t *dslot,
-    uint16_t *col16, int32_t *cbase, int32_t *tinfo) {
+    uint16_t *col16, uint32_t *off8, int32_t *cbase, int32_t *tinfo) {
   const int64_t t = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
   if (t >= ntiles) return;  // whole wavefronts
   const int lane = threadIdx.x & 63;
@@ -289,12 +290,13 @@ __global__ __launch_bounds__(kBlock) void k_tile_layout(
     ++nu;
   }
   if (!fits) {
-    if (lane == 0) tinfo[t] = 1;
+    if (lane == 0) tinfo[t] = kTile32;
     return;
   }
 
-  bool narrow = true;
+  bool narrow = true, narrow8 = true;
   int ds = 0;
+  uint32_t o8[kSlotWords] = {};
 #pragma unroll
   for (int u = kSlots - 1; u >= 0; --u) {
     const bool has = (mask >> u) & 1u;
@@ -309,18 +311,24 @@ __global__ __launch_bounds__(kBlock) void k_tile_layout(
     const int hi = wave_max_i32(has ? c : INT_MIN);
     const bool used = lo != INT_MAX;
     if (used && hi - lo > 65535) narrow = false;
+    if (used && hi - lo > 255) narrow8 = false;
     const int32_t base = used ? lo : (int32_t)(t * kTile);
     if (row) {
       const int32_t cc = has ? c : (used ? lo : (int32_t)i);
       col[(int64_t)u * n + i] = cc;
       val[(int64_t)u * n + i] = has ? v : 0.0;
       col16[(int64_t)u * n + i] = (uint16_t)(cc - base);
+      o8[u / 4] |= ((uint32_t)(cc - base) & 0xffu) << (8 * (u % 4));
       if (has && c == i) ds = u;
     }
     if (lane == 0) cbase[(int64_t)u * ntiles + t] = base;
   }
-  if (row) dslot[i] = (uint8_t)ds;
-  if (lane == 0) tinfo[t] = narrow ? 0 : 1;
+  if (row) {
+    dslot[i] = (uint8_t)ds;
+#pragma unroll
+    for (int g = 0; g < kSlotWords; ++g) off8[(int64_t)g * n + i] = o8[g];
+  }
+  if (lane == 0) tinfo[t] = !narrow ? kTile32 : narrow8 ? kTile8 : kTile16;
 }
 
 // Rebuild the ELL in device order: device row k is natural row iperm[k];
@@ -655,15 +663,17 @@ void finalize_level(Problem &P, int l, NaturalLevel &N, int flags,
     L.col16 = P.arena.alloc<uint16_t>((size_t)kSlots * n);
     L.cbase = P.arena.alloc<int32_t>((size_t)kSlots * T);
     L.tinfo = P.arena.alloc<int32_t>(T);
+    L.off8 = P.arena.alloc<uint32_t>((size_t)kSlotWords * n);
     const int64_t wpb = kBlock / 64;
     k_tile_layout<<<(unsigned)((T + wpb - 1) / wpb), kBlock, 0, s>>>(
-        n, T, L.iperm, L.col, L.val, L.nnz_row, L.dslot, L.col16, L.cbase, L.tinfo);
+        n, T, L.iperm, L.col, L.val, L.nnz_row, L.dslot, L.col16, L.off8, L.cbase, L.tinfo);
     HPCGX_LAUNCHED();
     std::vector<int32_t> info(T);
     HPCGX_HIP(hipMemcpyAsync(info.data(), L.tinfo, sizeof(int32_t) * T,
                              hipMemcpyDeviceToHost, s));
     HPCGX_HIP(hipStreamSynchronize(s));
-    L.wide_tiles = std::count(info.begin(), info.end(), 1);
+    L.wide_tiles = std::count(info.begin(), info.end(), (int32_t)kTile32);
+    L.byte_tiles = std::count(info.begin(), info.end(), (int32_t)kTile8);
     L.c16 = true;
   }
   HPCGX_HIP(hipStreamSynchronize(s));

@@ -141,10 +141,53 @@ void check_status(Problem &P) {
   }
 }
 
+// One CG iteration after the first (CG.cpp:102-124), enqueued on s.
+void cg_iteration(Problem &P, int ordering, bool precond, double *x, hipStream_t s) {
+  const Level &A = P.L[0];
+  const int64_t n = A.n;
+  if (precond)
+    mg(P, 0, ordering, P.r, P.z, s);
+  else
+    launch_copy(P.r, P.z, n, s);
+  launch_dot(P, n, P.r, P.z, nullptr, kFinishBeta, s);
+  launch_update_p(n, P.z, P.p, P.scalars, s);
+  launch_spmv_dot(P, A, P.p, P.Ap, s);
+  launch_update_xr(P, n, x, P.r, P.p, P.Ap, s);
+}
+
+namespace {
+// The captured body of iterations 2.. (rebuilt when its key changes).
+hipGraphExec_t iteration_graph(Problem &P, int ordering, bool precond, double *x) {
+  auto &G = P.graph;
+  if (G.exec && G.x == x && G.ordering == ordering && G.precond == precond) return G.exec;
+  G.reset();
+  if (!G.cap) HPCGX_HIP(hipStreamCreateWithFlags(&G.cap, hipStreamNonBlocking));
+  HPCGX_HIP(hipStreamBeginCapture(G.cap, hipStreamCaptureModeThreadLocal));
+  hipGraph_t graph = nullptr;
+  try {
+    cg_iteration(P, ordering, precond, x, G.cap);
+  } catch (...) {
+    (void)hipStreamEndCapture(G.cap, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  HPCGX_HIP(hipStreamEndCapture(G.cap, &graph));
+  const hipError_t e = hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  HPCGX_HIP(e);
+  G.x = x;
+  G.ordering = ordering;
+  G.precond = precond;
+  return G.exec;
+}
+}  // namespace
+
 // CG.cpp:59-143.  With the reductions fused into their producers:
 //   r.z            -> dot kernel that also forms beta = rtz/oldrtz (:112-114)
 //   Ap = A p, p.Ap -> one SpMV kernel that also forms alpha (:118-120)
 //   x, r updates, r.r -> one kernel (:121-124)
+// With HPCGX_CG_GRAPH (and no tolerance test, timers or wavefront sweeps)
+// iterations 2.. replay one captured graph of cg_iteration.
 void cg(Problem &P, int ordering, const double *b, double *x, int max_iter,
         double tol, bool precond, int flags, int *niters, double *normr,
         double *normr0, double *times, hipStream_t s) {
@@ -165,8 +208,17 @@ void cg(Problem &P, int ordering, const double *b, double *x, int max_iter,
   double nr = std::sqrt(read_scalar(P, kNormr2, s));
   *normr0 = nr;
   const bool check_each = tol > 0.0;
+  bool wave = false;
+  for (int l = 0; l < P.nlevels; ++l) wave = wave || (P.L[l].wave && P.use_wave);
+  const bool graphed = (flags & HPCGX_CG_GRAPH) && !check_each && !T.on && !P.klog.on && !wave;
   int k = 1;
   for (; k <= max_iter && nr / *normr0 > tol; ++k) {     // :100
+    if (graphed && k >= 2) {
+      HPCGX_HIP(hipGraphLaunch(iteration_graph(P, ordering, precond, x), s));
+      *niters = k;
+      if (k == max_iter) nr = std::sqrt(read_scalar(P, kNormr2, s));
+      continue;
+    }
     T.tick();
     if (precond)
       mg(P, 0, ordering, P.r, P.z, s);                   // :103

@@ -28,9 +28,11 @@ int DeviceCG(const SparseMatrix &A, const Vector &b, Vector &x, int max_iter,
   if (!rc) rc = Upload(*L, x, dx);
   if (rc) return rc;
   double t[6] = {0, 0, 0, 0, 0, 0};
+  // timed sets (tolerance 0) replay a captured graph of the iteration
   rc = hpcgx_cg(L->ctx->problem, ordering, db, dx, max_iter, tolerance,
-                doPreconditioning ? 1 : 0, cg_phase_timers ? HPCGX_CG_TIMERS : 0,
-                &niters, &normr, &normr0, t, nullptr);
+                doPreconditioning ? 1 : 0,
+                (cg_phase_timers ? HPCGX_CG_TIMERS : 0) | HPCGX_CG_GRAPH, &niters, &normr,
+                &normr0, t, nullptr);
   if (rc) return Report("CG", rc);
   rc = Download(*L, dx, x);
   if (times) {

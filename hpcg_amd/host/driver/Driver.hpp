@@ -5,6 +5,8 @@
 #ifndef HPCG_AMD_DRIVER_HPP
 #define HPCG_AMD_DRIVER_HPP
 
+#include <string>
+
 #include "../CGData.hpp"
 #include "../Geometry.hpp"
 #include "../SparseMatrix.hpp"
@@ -76,6 +78,12 @@ struct RunSummary {
   bool quickPath;
   const char *ordering;
 };
+// Prints the run summary and writes the result file (ReportFile).
 void ReportResults(const SparseMatrix &A, const RunSummary &s);
+// The result file in the reference's key=value format with its section and
+// key names (ReportResults.cpp:211-405, OutputFile.cpp:92-129), written as
+// HPCG-Benchmark_3.1_<date>.txt in the working directory; returns the text
+// and the file name through *path (empty if it could not be written).
+std::string ReportFile(const SparseMatrix &A, const RunSummary &s, std::string *path);
 
 #endif

@@ -14,7 +14,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import (CG_TIMERS, KEEP_SCHEDULE, ORDER_LEXICOGRAPHIC, ORDER_MULTICOLOR,
+from ._lib import (CG_GRAPH, CG_TIMERS, KEEP_SCHEDULE, ORDER_LEXICOGRAPHIC, ORDER_MULTICOLOR,
                    HostLevel, LevelInfo, call)
 
 ORDERS = {"lexicographic": ORDER_LEXICOGRAPHIC, "level": ORDER_LEXICOGRAPHIC,
@@ -134,7 +134,8 @@ class Problem:
         call("hpcgx_problem_info", self._h, level, ctypes.byref(i))
         return dict(nx=i.nx, ny=i.ny, nz=i.nz, nrow=i.nrow, nnz=i.nnz, ncolors=i.ncolors,
                     nlevels_sched=i.nlevels_sched, ncoarse=i.ncoarse, wave=bool(i.wave),
-                    nslabs=i.nslabs, col16=bool(i.col16), wide_tiles=i.wide_tiles)
+                    nslabs=i.nslabs, col16=bool(i.col16), wide_tiles=i.wide_tiles,
+                    byte_tiles=i.byte_tiles)
 
     def nrow(self, level=0):
         return self.levels[level]["nrow"]
@@ -273,6 +274,43 @@ class Problem:
         return call("hpcgx_mg32", self._h, level, _order(ordering),
                     _ptr(r, n, "r", "float32"), _ptr(x, n, "x", "float32"), _stream())
 
+    # ---- block PCG over s right-hand sides ([n][s] multivectors) ----------
+    def to_device_order_multi(self, natural, level=0, dtype="float64"):
+        """natural-order [n][s] array -> device-order [n][s] tensor."""
+        torch = _torch()
+        nat = np.asarray(natural, np.float64)
+        cols = [self.to_device_order(nat[:, k], level) for k in range(nat.shape[1])]
+        return torch.stack(cols, dim=1).to(getattr(torch, dtype)).contiguous()
+
+    def to_numpy_multi(self, dev, level=0):
+        """device-order [n][s] tensor -> natural-order [n][s] numpy array."""
+        d = dev.double()
+        return np.stack([self.to_numpy(d[:, k].contiguous(), level)
+                         for k in range(d.shape[1])], axis=1)
+
+    def spmv_block(self, X, Y, level=0):
+        n, s = X.shape
+        return call("hpcgx_spmv_block", self._h, level, s, _ptr(X, n * s, "X"),
+                    _ptr(Y, n * s, "Y"), _stream())
+
+    def mg32_block(self, R, X, level=0, ordering="multicolor"):
+        n, s = R.shape
+        return call("hpcgx_mg32_block", self._h, level, _order(ordering), s,
+                    _ptr(R, n * s, "R", "float32"), _ptr(X, n * s, "X", "float32"), _stream())
+
+    def cg_block_mixed(self, B, X, max_iter=50, tolerance=0.0, recompute=10,
+                       ordering="multicolor"):
+        """hpcgx_cg_block_mixed: s systems at once (B, X are [n][s], device
+        order).  Returns niters and per-system normr, normr0."""
+        n, s = B.shape
+        ni = ctypes.c_int()
+        nr, nr0 = (ctypes.c_double * s)(), (ctypes.c_double * s)()
+        t = (ctypes.c_double * 6)()
+        call("hpcgx_cg_block_mixed", self._h, _order(ordering), s, _ptr(B, n * s, "B"),
+             _ptr(X, n * s, "X"), int(max_iter), float(tolerance), int(recompute),
+             ctypes.byref(ni), nr, nr0, t, _stream())
+        return dict(niters=ni.value, normr=list(nr), normr0=list(nr0), times=list(t))
+
     def cg_mixed(self, b, x, max_iter=50, tolerance=0.0, recompute=10, ordering="multicolor"):
         """FP64 PCG with the FP32 V-cycle (FP16 values) as preconditioner and
         r = b - A x recomputed in FP64 every `recompute` iterations."""
@@ -285,7 +323,7 @@ class Problem:
         return dict(niters=ni.value, normr=nr.value, normr0=nr0.value, times=list(t))
 
     def tune(self, key, value):
-        """Set a tuning knob ("col16", "wave", "wave_min_rows",
+        """Set a tuning knob ("col16", "col8", "spmv_dot_blocks", "wave", "wave_min_rows",
         "wave_slab_factor", "wave_rows", "xcd_remap", "pair_rows",
         "stream_min_bytes"); see include/hpcg_amd.h."""
         return call("hpcgx_tune", self._h, key.encode(), float(value))
@@ -306,14 +344,16 @@ class Problem:
         return ms.value, n.value
 
     def cg(self, b, x, max_iter=50, tolerance=0.0, precondition=True,
-           ordering="multicolor", timers=False, times=None):
+           ordering="multicolor", timers=False, times=None, graph=False):
         """CG (CG.cpp:59-143).  x is updated in place; returns niters, normr,
-        normr0 and the times[] slots accumulated by the call."""
+        normr0 and the times[] slots accumulated by the call.  graph=True
+        replays a captured hipGraph of the iteration (HPCGX_CG_GRAPH)."""
         n = self.nrow(0)
         ni, nr, nr0 = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
         t = (ctypes.c_double * 6)(*(times if times is not None else [0.0] * 6))
         call("hpcgx_cg", self._h, _order(ordering), _ptr(b, n, "b"), _ptr(x, n, "x"),
              int(max_iter), float(tolerance), int(bool(precondition)),
-             CG_TIMERS if timers else 0, ctypes.byref(ni), ctypes.byref(nr),
+             (CG_TIMERS if timers else 0) | (CG_GRAPH if graph else 0),
+             ctypes.byref(ni), ctypes.byref(nr),
              ctypes.byref(nr0), t, _stream())
         return dict(niters=ni.value, normr=nr.value, normr0=nr0.value, times=list(t))

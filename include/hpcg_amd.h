@@ -78,8 +78,9 @@ typedef struct {
   int64_t ncoarse;       /* rows of the next level (0 on coarsest) */
   int wave;              /* 1: multicolour sweeps run as a wavefront */
   int nslabs;            /* slabs of the wavefront schedule */
-  int col16;             /* 1: 16-bit column ids (tile base + offset) built */
+  int col16;             /* 1: narrow column ids (tile base + offset) built */
   int64_t wide_tiles;    /* tiles of 64 rows that keep 32-bit column ids */
+  int64_t byte_tiles;    /* tiles of 64 rows with 8-bit column offsets */
 } hpcgx_level_info;
 
 const char *hpcgx_last_error(void);
@@ -204,7 +205,15 @@ int hpcgx_waxpby(int64_t n, double alpha, const double *x, double beta,
  * accumulated like the reference (total, dot, waxpby, spmv, allreduce = 0,
  * MG) when times != NULL; times[1..5] are measured with HIP events only if
  * HPCGX_CG_TIMERS is set in `flags`.  Synchronous. */
-enum { HPCGX_CG_TIMERS = 1 };
+enum {
+  HPCGX_CG_TIMERS = 1,
+  /* run iterations 2.. as replays of one captured hipGraph of the
+   * iteration (tolerance 0 only; falls back to eager launches when a
+   * tolerance, timers, the kernel-timing log or wavefront sweeps are on).
+   * The graph is kept with the problem and rebuilt when x, the ordering,
+   * the preconditioning switch or a tuning knob changes. */
+  HPCGX_CG_GRAPH = 2
+};
 int hpcgx_cg(hpcgx_problem *p, int ordering, const double *b, double *x,
              int max_iter, double tolerance, int precondition, int flags,
              int *niters, double *normr, double *normr0, double *times,
@@ -238,6 +247,26 @@ int hpcgx_cg_mixed(hpcgx_problem *p, int ordering, const double *b, double *x,
                    int max_iter, double tolerance, int recompute, int *niters,
                    double *normr, double *normr0, double *times, void *stream);
 
+/* ---- block PCG over s right-hand sides ----------------------------------------
+ * No counterpart in the reference (CG.cpp solves one system).  s independent
+ * systems with the same matrix; every kernel reads a matrix row once for all
+ * s vectors.  Multivectors are [n][s] row-major (the s values of a row
+ * adjacent) in the level's device row order; s must divide 64. */
+
+/* Y = A X (FP64), per right-hand side ComputeSPMV_ref.cpp:62-69's order. */
+int hpcgx_spmv_block(hpcgx_problem *p, int level, int s, const double *X, double *Y,
+                     void *stream);
+/* The FP32 V-cycle of hpcgx_mg32 for s right-hand sides at once. */
+int hpcgx_mg32_block(hpcgx_problem *p, int level, int ordering, int s, const float *R,
+                     float *X, void *stream);
+/* hpcgx_cg_mixed for s right-hand sides: each keeps its own alpha, beta and
+ * residual; iterations continue until every system has
+ * normr[k]/normr0[k] <= tolerance (or max_iter).  normr and normr0 are host
+ * arrays of s doubles.  Synchronous. */
+int hpcgx_cg_block_mixed(hpcgx_problem *p, int ordering, int s, const double *B, double *X,
+                         int max_iter, double tolerance, int recompute, int *niters,
+                         double *normr, double *normr0, double *times, void *stream);
+
 /* ---- tuning ----------------------------------------------------------------- */
 
 /* No counterpart in the reference (tuning knobs of this implementation):
@@ -254,8 +283,11 @@ int hpcgx_cg_mixed(hpcgx_problem *p, int ordering, const double *b, double *x,
  *                      (16-byte matrix loads)
  *   "stream_min_bytes" matrices larger than this are read with non-temporal
  *                      loads (default 512e6)
- *   "col16"            1/0: read 16-bit column ids (a per-tile base plus a
- *                      16-bit offset) where a level has them (default 1)
+ *   "col16"            1/0: read narrow column ids (a per-tile base plus an
+ *                      8- or 16-bit offset) where a level has them
+ *                      (default 1)
+ *   "col8"             1/0: use the 8-bit offsets of tiles that have them
+ *                      (else their 16-bit ones; default 1)
  *   "spmv_dot_blocks"  workgroups of the persistent SpMV + dot kernel
  *                      (8..8192, rounded down to a multiple of 8; default
  *                      4096).  Changes the dot's summation order.

@@ -26,6 +26,18 @@ def load_golden(name):
     return d
 
 
+def splitmix_uniform(seed, n):
+    """splitmix64 stream mapped to uniform [-1, 1) (53-bit mantissa), the
+    same stream oracle/ref_harness.cpp draws; vectorised with wrapping
+    uint64 arithmetic."""
+    with np.errstate(over="ignore"):
+        s = np.uint64(seed) + np.uint64(0x9E3779B97F4A7C15) * np.arange(1, n + 1, dtype=np.uint64)
+        z = (s ^ (s >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (2.0 / 9007199254740992.0) - 1.0
+
+
 def golden_level(g, l):
     return dict(nnz_row=g["L%d_nnz_row" % l], cols=g["L%d_cols" % l], vals=g["L%d_vals" % l],
                 diag_slot=g["L%d_diag_slot" % l])

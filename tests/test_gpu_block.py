@@ -1,0 +1,127 @@
+"""Block PCG over s right-hand sides (hpcgx_*_block) against the oracle.
+
+Every right-hand side gets the single-vector arithmetic, so per column:
+  * FP64 SpMV and the FP32 V-cycle (both orderings) are bit-exact against
+    the oracle and against the single-vector GPU kernels;
+  * the CG trajectory matches the oracle's mixed-precision CG on that column
+    within 1e-9 relative up to the first FP64 residual recompute and 1e-7
+    after it (the dots' reassociation, amplified by the recompute's
+    cancellation, as in test_gpu_mixed.py).
+Right-hand sides: splitmix64 uniform(-1, 1) from seed 42, one stream read
+row-major into [n][s].
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import splitmix_uniform
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def hp():
+    import torch
+    assert torch.cuda.is_available()
+    import hpcg_amd
+    return hpcg_amd
+
+
+def rhs(n, s, seed=42):
+    return splitmix_uniform(seed, n * s).reshape(n, s)
+
+
+@pytest.mark.parametrize("dims,s", [((16, 16, 16), 8), ((32, 48, 64), 4), ((32, 48, 64), 16)])
+def test_spmv_block_bitexact(hp, dims, s):
+    say("spmv block", dims, s)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    h = O.Hierarchy(*dims)
+    for l in (0, 1):
+        n = h.nrow(l)
+        Xn = rhs(n, s, 7 + l)
+        Y = P.to_device_order_multi(Xn, l) * 0
+        P.spmv_block(P.to_device_order_multi(Xn, l), Y, level=l)
+        got = P.to_numpy_multi(Y, l)
+        for k in range(s):
+            assert np.array_equal(got[:, k], O.spmv(h.level(l), Xn[:, k])), (l, k)
+    del P
+
+
+@pytest.mark.parametrize("ordering", ["multicolor", "lexicographic"])
+@pytest.mark.parametrize("dims,s", [((16, 16, 16), 8), ((32, 48, 64), 8)])
+def test_mg32_block_bitexact(hp, dims, s, ordering):
+    import torch
+    say("mg32 block", dims, s, ordering)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    h = O.Hierarchy(*dims)
+    if ordering == "multicolor":
+        h.use_color_order()
+    n = h.nrow(0)
+    Rn = rhs(n, s, 11).astype(np.float32)
+    R = P.to_device_order_multi(Rn, 0, "float32")
+    X = torch.zeros_like(R)
+    P.mg32_block(R, X, ordering=ordering)
+    got = P.to_numpy_multi(X).astype(np.float32)
+    for k in range(s):
+        want = h.mg32(Rn[:, k])
+        assert np.array_equal(got[:, k], want), k
+        # and the single-vector GPU V-cycle on the same column
+        z = P.zeros32()
+        P.mg32(R[:, k].contiguous(), z, ordering=ordering)
+        assert torch.equal(z, X[:, k].contiguous()), k
+    del P
+
+
+def test_cg_block_trajectories(hp):
+    import torch
+    dims, s = (32, 48, 64), 4
+    say("cg block trajectories", dims, s)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    h = O.Hierarchy(*dims)
+    h.use_color_order()
+    n = h.nrow(0)
+    Bn = rhs(n, s)
+    B = P.to_device_order_multi(Bn)
+    want = [h.cg_mixed(b=Bn[:, k], max_iter=20, tol=0.0, recompute=10)["traj"] for k in range(s)]
+    for it in (1, 5, 10, 11, 20):
+        X = torch.zeros_like(B)
+        r = P.cg_block_mixed(B, X, max_iter=it, tolerance=0.0, recompute=10)
+        assert r["niters"] == it
+        tol = 1e-9 if it <= 10 else 1e-7
+        for k in range(s):
+            assert r["normr"][k] / r["normr0"][k] == pytest.approx(want[k][it], rel=tol), (it, k)
+    # the block solution equals s single-vector solves to rounding
+    X = torch.zeros_like(B)
+    P.cg_block_mixed(B, X, max_iter=20, tolerance=0.0, recompute=10)
+    for k in range(s):
+        x = P.zeros()
+        P.cg_mixed(B[:, k].contiguous(), x, max_iter=20, tolerance=0.0, recompute=10)
+        d = float((x - X[:, k]).abs().max()) / float(x.abs().max())
+        assert d <= 1e-7, (k, d)
+    del P
+
+
+def test_cg_block_reaches_tolerance(hp):
+    import torch
+    dims, s = (32, 48, 64), 8
+    say("cg block to 1e-9", dims, s)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    B = P.to_device_order_multi(rhs(P.nrow(0), s))
+    X = torch.zeros_like(B)
+    r = P.cg_block_mixed(B, X, max_iter=500, tolerance=1e-9, recompute=10)
+    assert r["niters"] < 500
+    assert all(a / b <= 1e-9 for a, b in zip(r["normr"], r["normr0"]))
+    # the true residual of every system, recomputed in FP64
+    Y = torch.zeros_like(B)
+    P.spmv_block(X, Y)
+    for k in range(s):
+        res = float((B[:, k] - Y[:, k]).norm()) / float(B[:, k].norm())
+        assert res <= 2e-9, (k, res)
+    del P

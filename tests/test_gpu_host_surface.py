@@ -173,6 +173,58 @@ def test_driver_valid_run_matches_reference_tolerance():
     assert rating > 0.0
 
 
+def _kv(path):
+    out = []
+    with open(path) as f:
+        for line in f.read().splitlines()[2:]:
+            k, _, v = line.partition("=")
+            out.append((k, v))
+    return out
+
+
+@pytest.mark.gpu
+def test_driver_result_file_matches_reference(tmp_path):
+    """xhpcg_amd's result file has the reference's sections and keys in the
+    reference's order (tests/golden/ref_report_16x16x16.txt, written by the
+    compiled reference at 16^3), and the same values wherever they do not
+    depend on timing or on the implementation."""
+    assert os.path.exists(DRIVER), "xhpcg_amd not built (run __graft_entry__.build())"
+    say("driver result file 16^3")
+    r = subprocess.run([DRIVER, "16", "16", "16"], cwd=str(tmp_path), capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    files = [f for f in os.listdir(tmp_path) if f.startswith("HPCG-Benchmark_3.1_")]
+    assert len(files) == 1, files
+    with open(os.path.join(tmp_path, files[0])) as f:
+        head = f.read().splitlines()[:2]
+    assert head == ["HPCG-Benchmark", "version=3.1"]
+    ours = _kv(os.path.join(tmp_path, files[0]))
+    ref = _kv(os.path.join(ROOT, "tests", "golden", "ref_report_16x16x16.txt"))
+    # implementation lines: the reference's release date and its "reference
+    # version of ... used" warnings, this driver's implementation line
+    ref = [(k, v) for k, v in ref
+           if k != "Release date" and not k.startswith("Final Summary::Reference version")]
+    ours = [(k, v) for k, v in ours if k != "Implementation"]
+    assert [k for k, _ in ours] == [k for k, _ in ref]
+    same = ("Global Problem Dimensions", "Processor Dimensions", "Local Domain Dimensions",
+            "Linear System Information", "Multigrid Information",
+            "Memory Use Information::Memory used for linear system and CG",
+            "Memory Use Information::Coarse Grids", "Spectral Convergence Tests",
+            "Departure from Symmetry |x'Ay-y'Ax|/(2*||x||*||A||*||y||)/epsilon::Result",
+            "Iteration Count Information::Result",
+            "Iteration Count Information::Reference CG iterations per set",
+            "Iteration Count Information::Total number of reference iterations",
+            "Reproducibility Information::Result", "Final Summary::You have selected",
+            "Final Summary::After confirmation")
+    for (k, v), (_, w) in zip(ours, ref):
+        if k.startswith(same):
+            assert v == w, (k, v, w)
+    d = dict(ours)
+    assert int(d["Iteration Count Information::Optimized CG iterations per set"]) >= 50
+    assert float(d["Final Summary::HPCG result is VALID with a GFLOP/s rating of"]) > 0
+    assert float(d["Memory Use Information::Memory used for OptimizeProblem data (Gbytes)"]) > 0
+
+
 @pytest.mark.gpu
 def test_driver_lexicographic_ordering():
     assert os.path.exists(DRIVER), "xhpcg_amd not built (run __graft_entry__.build())"

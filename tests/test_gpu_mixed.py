@@ -94,15 +94,16 @@ def test_symgs32_bitexact(hp, dims, ordering):
 
 
 @pytest.mark.parametrize("dims", [(32, 48, 64), (64, 64, 64), (8, 8, 24)])
-@pytest.mark.parametrize("c16", [1, 0])
-def test_symgs32_col16(hp, dims, c16):
-    """Multicolour FP32 sweeps with the packed 16-bit column / FP16 value
-    words (the default where a level has 16-bit ids) and with 32-bit ids:
+@pytest.mark.parametrize("c16,c8", [(1, 1), (1, 0), (0, 1)])
+def test_symgs32_col16(hp, dims, c16, c8):
+    """Multicolour FP32 sweeps and FP32 SpMV with 8-bit column offsets (the
+    default), the packed 16-bit column / FP16 value words, and 32-bit ids:
     the oracle's bits on every level, including colour blocks whose length
     is not a multiple of 128 rows."""
-    say("symgs32 col16", dims, c16)
+    say("symgs32 col16", dims, c16, c8)
     P = hp.Problem.generate(*dims, nlevels=4)
     P.tune("col16", c16)
+    P.tune("col8", c8)
     h = hier(dims, False)
     for l in range(4):
         m = h.level(l)
@@ -110,7 +111,10 @@ def test_symgs32_col16(hp, dims, c16):
         want = O.symgs32(m, r, x0, O.color_order(O.colors(m)[0]))
         x = dev32(P, x0, l)
         P.symgs32(dev32(P, r, l), x, level=l, ordering="multicolor")
-        assert np.array_equal(host32(P, x, l), want), (c16, l)
+        assert np.array_equal(host32(P, x, l), want), (c16, c8, l)
+        y = P.zeros32(l)
+        P.spmv32(dev32(P, x0, l), y, level=l)
+        assert np.array_equal(host32(P, y, l), O.spmv32(m, x0)), (c16, c8, l)
     del P
 
 

@@ -192,11 +192,10 @@ def test_pair_rows_bitexact(hp, dims):
 
 @pytest.mark.parametrize("dims", SMALL + [(64, 64, 64)])
 def test_col16_layout(hp, dims):
-    """Tile-aligned slots with 16-bit column ids: built wherever the colour
-    blocks start on 64-row tiles, no tile of the 27-point problem needs
-    32-bit ids, and SpMV, both SYMGS orderings and the V-cycle give the
-    oracle's bits with the 16-bit ids and with the 32-bit ids of the same
-    layout."""
+    """Tile-aligned slots with narrow column ids: built wherever the colour
+    blocks start on 64-row tiles, every tile of the 27-point problem gets
+    8-bit offsets, and SpMV, SYMGS and the V-cycle give the oracle's bits
+    with the 8-bit, the 16-bit and the 32-bit ids of the same layout."""
     say("col16 layout", dims)
     P = hp.Problem.generate(*dims, nlevels=4)
     h = hier(dims, color=True)
@@ -209,24 +208,26 @@ def test_col16_layout(hp, dims):
         assert inf["col16"] == aligned, (l, inf)
         if aligned:
             assert inf["wide_tiles"] == 0, (l, inf)
+            assert inf["byte_tiles"] == (n + 63) // 64, (l, inf)  # every tile: 8-bit
     r0 = rnd(301, h.nrow(0))
     want_mg = h.mg(r0)
-    for c16 in (1, 0):
+    for c16, c8 in ((1, 1), (1, 0), (0, 1)):
         P.tune("col16", c16)
+        P.tune("col8", c8)
         for l in range(4):
             m = h.level(l)
             n = h.nrow(l)
             xv, r, x0 = rnd(310 + l, n), rnd(320 + l, n), rnd(330 + l, n)
             y = P.empty(l)
             P.spmv(P.to_device_order(xv, l), y, level=l)
-            assert np.array_equal(P.to_numpy(y, l), O.spmv(m, xv)), (c16, l)
+            assert np.array_equal(P.to_numpy(y, l), O.spmv(m, xv)), (c16, c8, l)
             want = O.symgs(m, r, x0, O.color_order(O.colors(m)[0]))
             x = P.to_device_order(x0, l)
             P.symgs(P.to_device_order(r, l), x, level=l, ordering="multicolor")
-            assert np.array_equal(P.to_numpy(x, l), want), (c16, l)
+            assert np.array_equal(P.to_numpy(x, l), want), (c16, c8, l)
         z = P.zeros()
         P.mg(P.to_device_order(r0), z, ordering="multicolor")
-        assert np.array_equal(P.to_numpy(z), want_mg), c16
+        assert np.array_equal(P.to_numpy(z), want_mg), (c16, c8)
     del P
 
 
@@ -393,6 +394,40 @@ def test_cg_trajectory(hp, torch, dims, its, ordering):
         assert r["normr"] / r["normr0"] == pytest.approx(want[k], rel=1e-9), k
 
 
+@pytest.mark.parametrize("ordering,precond", [("multicolor", True), ("lexicographic", True),
+                                              ("multicolor", False)])
+def test_cg_graph_replays(hp, torch, ordering, precond):
+    """Iterations replayed from the captured hipGraph give the eager
+    launches' bits, on every call, with eager work on the same problem
+    between the calls and a recapture after a tuning change."""
+    dims = (32, 48, 64)
+    say("cg graph", ordering, precond)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    b, x, _ = P.vectors()
+    want = {}
+    for c16 in (1, 0):
+        P.tune("col16", c16)
+        x.zero_()
+        want[c16] = (P.cg(b, x, max_iter=17, tolerance=0.0, ordering=ordering,
+                          precondition=precond), x.clone())
+    for rep in range(3):
+        for c16 in (1, 0):
+            P.tune("col16", c16)  # drops the captured graph
+            for again in range(2):
+                x.zero_()
+                r = P.cg(b, x, max_iter=17, tolerance=0.0, ordering=ordering,
+                         precondition=precond, graph=True)
+                w, xw = want[c16]
+                assert r["niters"] == 17 and r["normr"] == w["normr"], (rep, c16, again)
+                assert torch.equal(x, xw), (rep, c16, again)
+                # eager work on the same problem and vectors between replays
+                y = P.zeros()
+                P.mg(b, y, ordering=ordering)
+                x.zero_()
+                P.cg(b, x, max_iter=3, tolerance=0.0, ordering=ordering)
+    del P
+
+
 def test_cg_tolerance_stop(hp, torch):
     say("cg tolerance stop")
     P = problem(hp, (32, 48, 64))
@@ -465,6 +500,46 @@ def test_128_bitexact(hp, torch):
     P.mg(P.to_device_order(x), z, ordering="lexicographic")
     assert np.array_equal(P.to_numpy(z), h.mg(x))
     del P
+
+
+def test_256_kernels_vs_oracle(hp):
+    """256^3, kernel by kernel against the oracle on the three input vectors
+    x = 1, x_i = (i mod 7)/7 and splitmix64 uniform(-1, 1) from seed 0:
+    SpMV, the level-scheduled (natural order) and the 8-colour SYMGS, and
+    the V-cycle in both orderings -- all bit for bit."""
+    from conftest import splitmix_uniform
+    dims = (256, 256, 256)
+    say("256^3 kernels: oracle hierarchy", dims)
+    h = O.Hierarchy(*dims)
+    n = h.nrow(0)
+    inputs = {"ones": np.ones(n), "mod7": (np.arange(n) % 7) / 7.0,
+              "splitmix0": splitmix_uniform(0, n)}
+    P = hp.Problem.generate(*dims, nlevels=4)
+    L0 = h.level(0)
+    b = np.array(h.b)
+    for name, u in inputs.items():
+        say("spmv", name)
+        y = P.empty()
+        P.spmv(P.to_device_order(u), y)
+        assert np.array_equal(P.to_numpy(y), O.spmv(L0, u)), name
+    col, _ = O.colors(L0)
+    corder = O.color_order(col)
+    for ordering, order in (("lexicographic", None), ("multicolor", corder)):
+        for name, u in inputs.items():
+            say("symgs", ordering, name)
+            x = P.to_device_order(u)
+            P.symgs(P.to_device_order(b), x, ordering=ordering)   # r = b, x0 = u
+            assert np.array_equal(P.to_numpy(x), O.symgs(L0, b, u, order)), (ordering, name)
+    for ordering in ("lexicographic", "multicolor"):
+        if ordering == "multicolor":
+            h.use_color_order()
+        for name in ("mod7", "splitmix0"):
+            say("mg", ordering, name)
+            u = inputs[name]
+            z = P.zeros()
+            P.mg(P.to_device_order(u), z, ordering=ordering)
+            assert np.array_equal(P.to_numpy(z), h.mg(u)), (ordering, name)
+    del P, h
 
 
 def test_256_properties(hp, torch):

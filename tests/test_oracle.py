@@ -60,6 +60,12 @@ def uniform(seed, n):
     return out
 
 
+def test_splitmix_vectorised_matches_scalar():
+    from conftest import splitmix_uniform
+    for seed in (0, 42, 7):
+        assert np.array_equal(splitmix_uniform(seed, 1000), uniform(seed, 1000))
+
+
 # --------------------------------------------------------------------------
 # generator and f2c
 # --------------------------------------------------------------------------
