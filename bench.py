@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--block", type=int, nargs="*", default=[],
                     help="also solve s right-hand sides at once (block PCG, FP32 V-cycle) "
                          "for each s given, to --tol-compare")
+    ap.add_argument("--autotune", type=int, default=1, choices=[0, 1],
+                    help="pick the tuning knobs by timing (hpcgx_autotune, before the warm-up)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device (the known-size kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -216,6 +218,10 @@ def main():
     if a.stream_min_bytes:
         P.tune("stream_min_bytes", a.stream_min_bytes)
     log("setup %.2f s, device memory %.1f GB" % (setup_s, P.memory_use() / 1e9))
+    tuned = None
+    if a.autotune:
+        tuned = P.autotune(ordering=a.ordering)
+        log("autotune: " + "; ".join(l for l in tuned if l.startswith("chosen")))
 
     if a.warmup > 0:
         P.cg(b, x, max_iter=a.warmup, tolerance=0.0, ordering=a.ordering, graph=bool(a.graph))
@@ -309,6 +315,8 @@ def main():
             "launches": ks_n,
         },
     }
+    if tuned is not None:
+        out["autotune"] = [l for l in tuned if l.startswith("chosen")]
     if a.tol_compare > 0:
         out["convergence"] = converge(P, b, lv, a, log)
     if a.block:

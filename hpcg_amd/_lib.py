@@ -78,6 +78,7 @@ SIGNATURES = {
     "hpcgx_mg32": (I, [P, I, I, P, P, P]),
     "hpcgx_cg_mixed": (I, [P, I, P, P, I, D, I, P, P, P, P, P]),
     "hpcgx_spmv_block": (I, [P, I, I, P, P, P]),
+    "hpcgx_autotune": (I, [P, I, I, ctypes.c_char_p, I]),
     "hpcgx_mg32_block": (I, [P, I, I, I, P, P, P]),
     "hpcgx_cg_block_mixed": (I, [P, I, I, P, P, I, D, I, P, P, P, P, P]),
     "hpcgx_tune": (I, [P, ctypes.c_char_p, D]),

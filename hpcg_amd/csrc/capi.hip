@@ -7,6 +7,9 @@
 #include <string>
 #include <vector>
 
+#include <chrono>
+#include <cstdio>
+
 #include "internal.hpp"
 
 struct hpcgx_problem {
@@ -441,6 +444,65 @@ int hpcgx_tune(hpcgx_problem *p, const char *key, double value) {
       }
     }
     HPCGX_HIP(hipDeviceSynchronize());
+  });
+}
+
+int hpcgx_autotune(hpcgx_problem *p, int ordering, int iterations, char *report,
+                   int report_len) {
+  return guard([&] {
+    if (!p) throw Error(HPCGX_EINVAL, "NULL problem");
+    check_order(ordering);
+    if (iterations < 1) throw Error(HPCGX_EINVAL, "iterations must be >= 1");
+    hpcgx::Problem &P = p->P;
+    const int64_t n = P.L[0].n;
+    hpcgx::TempBufD bx(sizeof(double) * 2 * n);
+    double *b = static_cast<double *>(bx.p), *x = b + n;
+    hpcgx::launch_fill(b, n, 1.0, nullptr);
+    // one timed solve of `iterations` PCG iterations (after one warm-up)
+    auto timed = [&]() {
+      int it;
+      double nr, nr0;
+      hpcgx::launch_fill(x, n, 0.0, nullptr);
+      hpcgx::cg(P, ordering, b, x, 1, 0.0, true, 0, &it, &nr, &nr0, nullptr, nullptr);
+      hpcgx::launch_fill(x, n, 0.0, nullptr);
+      HPCGX_HIP(hipDeviceSynchronize());
+      const auto t0 = std::chrono::steady_clock::now();
+      hpcgx::cg(P, ordering, b, x, iterations, 0.0, true, 0, &it, &nr, &nr0, nullptr, nullptr);
+      HPCGX_HIP(hipDeviceSynchronize());
+      return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() /
+             iterations;
+    };
+    struct Knob {
+      const char *key;
+      std::vector<double> values;
+    };
+    bool any_c16 = false;
+    for (int l = 0; l < P.nlevels; ++l) any_c16 = any_c16 || P.L[l].c16;
+    std::vector<Knob> knobs;
+    if (any_c16) knobs.push_back({"col8", {1, 0}});
+    knobs.push_back({"spmv_dot_blocks", {1024, 2048, 4096, 8192}});
+    knobs.push_back({"xcd_remap", {0, 1}});
+    std::string log;
+    char line[160];
+    for (const Knob &k : knobs) {  // one knob at a time, the others at their best
+      double best_t = 1e300, best_v = k.values[0];
+      for (double v : k.values) {
+        HPCGX_REQUIRE(hpcgx_tune(p, k.key, v) == 0, "autotune: tuning call failed");
+        const double t = timed();
+        std::snprintf(line, sizeof line, "%s=%g: %.4f ms/iteration\n", k.key, v, t * 1e3);
+        log += line;
+        if (t < best_t) {
+          best_t = t;
+          best_v = v;
+        }
+      }
+      HPCGX_REQUIRE(hpcgx_tune(p, k.key, best_v) == 0, "autotune: tuning call failed");
+      std::snprintf(line, sizeof line, "chosen %s=%g\n", k.key, best_v);
+      log += line;
+    }
+    if (report && report_len > 0) {
+      std::snprintf(report, (size_t)report_len, "%s", log.c_str());
+    }
   });
 }
 

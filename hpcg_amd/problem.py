@@ -328,6 +328,13 @@ class Problem:
         "stream_min_bytes"); see include/hpcg_amd.h."""
         return call("hpcgx_tune", self._h, key.encode(), float(value))
 
+    def autotune(self, ordering="multicolor", iterations=3):
+        """hpcgx_autotune: time the candidate knob values and keep the
+        fastest; returns the report lines."""
+        buf = ctypes.create_string_buffer(4096)
+        call("hpcgx_autotune", self._h, _order(ordering), int(iterations), buf, len(buf))
+        return buf.value.decode().splitlines()
+
     def check(self):
         """Wait for the stream; raise if a wavefront sweep gave up waiting."""
         return call("hpcgx_check_status", self._h, _stream())

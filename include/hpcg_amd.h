@@ -294,6 +294,15 @@ int hpcgx_cg_block_mixed(hpcgx_problem *p, int ordering, int s, const double *B,
  * Synchronous.  Unknown keys return HPCGX_EINVAL. */
 int hpcgx_tune(hpcgx_problem *p, const char *key, double value);
 
+/* No counterpart in the reference: picks the tuning knobs for this problem
+ * and machine by timing `iterations` PCG iterations (plus one warm-up) per
+ * candidate, one knob at a time ("col8", "spmv_dot_blocks", "xcd_remap"),
+ * keeping the fastest value of each.  Writes one line per timing and per
+ * choice into `report` (may be NULL).  Uses the default stream; synchronous.
+ * Changing "spmv_dot_blocks" changes the dots' summation order. */
+int hpcgx_autotune(hpcgx_problem *p, int ordering, int iterations, char *report,
+                   int report_len);
+
 /* No counterpart in the reference: waits for `stream` and reports a
  * wavefront sweep whose dependency wait gave up (HPCGX_EHIP).  hpcgx_cg
  * performs the same check before returning. */

@@ -428,6 +428,30 @@ def test_cg_graph_replays(hp, torch, ordering, precond):
     del P
 
 
+def test_autotune(hp, torch):
+    """The autotuner times every candidate, keeps one value per knob, and the
+    tuned problem still solves: bit-exact V-cycle, CG within the dots'
+    reassociation of the untuned run."""
+    dims = (64, 64, 64)
+    say("autotune", dims)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    b, x, _ = P.vectors()
+    r0 = P.cg(b, x, max_iter=10, tolerance=0.0)
+    rep = P.autotune(iterations=2)
+    chosen = [l for l in rep if l.startswith("chosen ")]
+    assert {l.split()[1].split("=")[0] for l in chosen} == {"col8", "spmv_dot_blocks", "xcd_remap"}
+    assert len([l for l in rep if "ms/iteration" in l]) == 2 + 4 + 2
+    x.zero_()
+    r1 = P.cg(b, x, max_iter=10, tolerance=0.0)
+    assert r1["normr"] == pytest.approx(r0["normr"], rel=1e-10)
+    h = hier(dims, color=True)
+    rr = rnd(77, P.nrow(0))
+    z = P.zeros()
+    P.mg(P.to_device_order(rr), z, ordering="multicolor")
+    assert np.array_equal(P.to_numpy(z), h.mg(rr))
+    del P
+
+
 def test_cg_tolerance_stop(hp, torch):
     say("cg tolerance stop")
     P = problem(hp, (32, 48, 64))
