@@ -298,6 +298,9 @@ double read_block_scalars(const Problem &P, int s, std::vector<double> &norm2, h
 
 void prepare_block(Problem &P, int s) {
   HPCGX_REQUIRE(s >= 1 && s <= 64 && 64 % s == 0, "s must divide 64");
+  // one work-item per (row, right-hand side): a launch holds < 2^32 of them
+  HPCGX_REQUIRE(P.L[0].n * s < ((int64_t)1 << 32) - kBlock,
+                "n * s exceeds the 2^32 work-items of one launch");
   prepare_mixed(P);
   auto &B = P.blk;
   if (B.s >= s) return;

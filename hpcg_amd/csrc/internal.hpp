@@ -17,6 +17,9 @@ constexpr int kSlots = 27;        // ELL width: the 27-point stencil's maximum
 constexpr int kMaxLevels = 8;
 constexpr int kMaxColors = 32;    // first-fit colours of a <=27-entry row: <=27
 constexpr int kBlock = 256;       // threads per workgroup for streaming kernels
+// A launch holds fewer than 2^32 work-items (the dispatch packet's grid
+// size is 32 bits): kernels over all n*27 ELL entries loop over the slots
+// inside one work-item per row.
 constexpr int kReduceBlocks = 4096;  // workgroups of a grid-stride reduction
 constexpr int kMaxPartials = 8192;   // partial-sum slots (largest reducing grid)
 constexpr double kStreamBytes = 512e6;  // matrices above this are streamed
@@ -125,9 +128,6 @@ struct Level {
   // zeroed, the FP32 diagonal, FP32 coarse-level vectors
   _Float16 *val16 = nullptr;
   uint32_t *vc = nullptr;  // c16 levels: col16 << 16 | FP16 bits, one word per slot
-  // c16 levels: per row and group g of 4 slots, 12 bytes: FP16 values of
-  // slots 4g, 4g+1 | 4g+2, 4g+3 and the 8-bit offsets (kTile8 tiles)
-  uint32_t *q4 = nullptr;
   float *diag32 = nullptr;
   float *rc32 = nullptr, *xc32 = nullptr;
   // Narrow column ids.  Tile t = device rows [kTile t, kTile t + kTile).  The
@@ -225,14 +225,13 @@ struct Problem {
 struct Col16 {
   const uint16_t *off;
   const uint32_t *off8;
-  const uint32_t *q4;  // FP32 V-cycle records of kTile8 tiles (Level::q4)
   const int32_t *base;
   const int32_t *info;
   int64_t ntiles;
   int use8;  // read 8-bit offsets in kTile8 tiles (else their 16-bit ids)
 };
 inline Col16 col16_of(const Problem &P, const Level &L) {
-  return Col16{L.col16, L.off8, L.q4, L.cbase, L.tinfo, L.ntiles, P.use_c8 ? 1 : 0};
+  return Col16{L.col16, L.off8, L.cbase, L.tinfo, L.ntiles, P.use_c8 ? 1 : 0};
 }
 // 16-bit kernels apply to rows [begin, ...) of L when this holds.
 inline bool use_col16(const Problem &P, const Level &L, int64_t begin) {
@@ -267,6 +266,10 @@ void launch_dot(Problem &P, int64_t n, const double *x, const double *y,
                 double *out, int finish, hipStream_t s);
 void launch_spmv_dot(Problem &P, const Level &L, const double *p, double *Ap,
                      hipStream_t s);
+// the mixed path's outer Ap = A p, p.Ap from the FP16 off-diagonal values
+// and the FP64 diagonal (the FP64 result while those values are exact)
+void launch_spmv_dot_half(Problem &P, const Level &L, const double *p, double *Ap,
+                          hipStream_t s);
 void launch_update_xr(Problem &P, int64_t n, double *x, double *r,
                       const double *p, const double *Ap, hipStream_t s,
                       float *r32 = nullptr);

@@ -539,6 +539,51 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(
   grid_finish(s, partials, ticket, scalars, nullptr, kFinishAlpha);
 }
 
+// Ap = A p and p.Ap for the mixed-precision outer iteration, reading the
+// FP32 V-cycle's packed words (16-bit column offset above the FP16 value)
+// for the off-diagonal values and the FP64 diagonal array for the diagonal,
+// in slot order with FP64 products and sums: the FP64 SpMV whenever the
+// off-diagonal values are exact in FP16 (they are -1 here), at 4 instead of
+// 9 matrix bytes per entry.  32-bit tiles read the FP64 ELL.
+template <bool STREAM>
+__global__ __launch_bounds__(kBlock) void k_spmv_dot_h(
+    int64_t nchunks, const int64_t *__restrict__ chunks, int64_t n, Col16 C,
+    const uint32_t *__restrict__ vc, const uint8_t *__restrict__ dslot,
+    const double *__restrict__ diag, const int32_t *__restrict__ col,
+    const double *__restrict__ val, const double *__restrict__ p, double *__restrict__ Ap,
+    double *partials, unsigned *ticket, double *scalars) {
+  const int64_t xq = blockIdx.x % 8, slot = blockIdx.x / 8, per = gridDim.x / 8;
+  const int64_t lo = xq * nchunks / 8, hi = (xq + 1) * nchunks / 8;
+  double s = 0.0;
+  for (int64_t w = lo + slot; w < hi; w += per) {
+    const int64_t i = chunks[2 * w] + threadIdx.x;
+    if (i < chunks[2 * w + 1]) {
+      const int t = tile_of(i);
+      double y;
+      if (C.info[t] == kTile32) {
+        y = row_dot<STREAM>(col, val, p, n, i);
+      } else {
+        const int32_t *__restrict__ base = C.base + t;
+        const int ds = dslot[i];
+        const double di = diag[i];
+        y = 0.0;
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) {
+          const uint32_t wd = mload<STREAM>(vc + (int64_t)j * n + i);
+          const int32_t c = base[(int64_t)j * C.ntiles] + (int32_t)(wd >> 16);
+          const double a = j == ds ? di
+                                   : (double)__builtin_bit_cast(_Float16, (uint16_t)(wd & 0xffffu));
+          y += a * p[c];
+        }
+      }
+      Ap[i] = y;
+      s += p[i] * y;
+    }
+  }
+  s = block_sum(s);
+  grid_finish(s, partials, ticket, scalars, nullptr, kFinishAlpha);
+}
+
 // x += alpha p; r -= alpha Ap; r.r (CG.cpp:121-124).  R32: also writes r
 // rounded to FP32, the input of the mixed-precision V-cycle.
 template <bool R32>
@@ -752,6 +797,26 @@ void launch_spmv_dot(Problem &P, const Level &L, const double *p, double *Ap,
   else
     k_spmv_dot<false, false><<<G, kBlock, 0, s>>>(L.nchunks, L.chunks, L.n, C, L.col, L.val,
                                                   p, Ap, P.partials, P.ticket, P.scalars);
+  HPCGX_LAUNCHED();
+}
+
+void launch_spmv_dot_half(Problem &P, const Level &L, const double *p, double *Ap,
+                          hipStream_t s) {
+  if (!L.vc || !use_col16(P, L, 0)) {
+    launch_spmv_dot(P, L, p, Ap, s);
+    return;
+  }
+  int64_t g = std::min<int64_t>(L.nchunks, std::min(P.spmv_dot_blocks, kMaxPartials));
+  g = std::max<int64_t>(8, (g + 7) / 8 * 8);
+  const Col16 C = col16_of(P, L);
+  if (L.stream)
+    k_spmv_dot_h<true><<<(unsigned)g, kBlock, 0, s>>>(L.nchunks, L.chunks, L.n, C, L.vc,
+                                                      L.dslot, L.diag, L.col, L.val, p, Ap,
+                                                      P.partials, P.ticket, P.scalars);
+  else
+    k_spmv_dot_h<false><<<(unsigned)g, kBlock, 0, s>>>(L.nchunks, L.chunks, L.n, C, L.vc,
+                                                       L.dslot, L.diag, L.col, L.val, p, Ap,
+                                                       P.partials, P.ticket, P.scalars);
   HPCGX_LAUNCHED();
 }
 

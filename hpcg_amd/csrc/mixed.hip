@@ -12,9 +12,12 @@
 //   Gauss-Seidel  sum = r_i; sum -= h_ij * x_j over the slots (the diagonal
 //                 and unused slots hold 0); x_i = sum / d_i
 //   A*x           s = sum_j h_ij * x_j over the slots; s += d_i * x_i
-// The outer PCG (CG.cpp:59-143 of the reference) stays in FP64: SpMV with the
-// FP64 matrix, dot products, alpha, beta, x and r; every `recompute`-th
-// iteration r = b - A x is recomputed in FP64 instead of updated.
+// The outer PCG (CG.cpp:59-143 of the reference) stays in FP64: dot
+// products, alpha, beta, x and r, and A p with FP64 products and sums (its
+// off-diagonal values read from the FP16 copy, the diagonal from the FP64
+// array: the FP64 SpMV while those values are exact in FP16, as -1 is);
+// every `recompute`-th iteration r = b - A x is recomputed with the FP64
+// matrix instead of updated.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -124,60 +127,6 @@ __device__ inline half_t vc_val(uint32_t w) {
   return __builtin_bit_cast(half_t, (uint16_t)(w & 0xffffu));
 }
 
-// kTile8 tiles in the FP32 V-cycle: slots in groups of four, each group one
-// 12-byte record per row (Level::q4): two words of FP16 values (slots
-// 4g..4g+3) and one word of 8-bit column offsets.  Seven loads per row
-// instead of 27: the FP32 colour step is bound by the number of vector
-// memory instructions it issues.
-struct Q4 {
-  uint32_t h01, h23, off;
-};
-
-template <bool STREAM>
-__device__ inline float sum8_32(const Col16 &C, int t, const uint32_t *__restrict__ q4,
-                                int64_t i, int64_t n, const float *__restrict__ x,
-                                float s, bool subtract) {
-  const int32_t *__restrict__ base = C.base + t;
-#pragma unroll
-  for (int g = 0; g < kSlotWords; ++g) {
-    const uint32_t *rec = q4 + 3 * ((int64_t)g * n + i);
-    const uint32_t h01 = mload<STREAM>(rec), h23 = mload<STREAM>(rec + 1),
-                   off = mload<STREAM>(rec + 2);
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int j = 4 * g + b;
-      if (j < kSlots) {
-        const int32_t c = base[(int64_t)j * C.ntiles] + (int32_t)((off >> (8 * b)) & 0xffu);
-        const uint32_t hw = b < 2 ? h01 : h23;
-        const half_t h = __builtin_bit_cast(half_t, (uint16_t)((hw >> (16 * (b & 1))) & 0xffffu));
-        const float p = (float)h * x[c];
-        if (subtract)
-          s -= p;
-        else
-          s += p;
-      }
-    }
-  }
-  return s;
-}
-
-__global__ __launch_bounds__(kBlock) void k_pack_q4(int64_t n, const half_t *__restrict__ val16,
-                                                    const uint32_t *__restrict__ off8,
-                                                    uint32_t *__restrict__ q4) {
-  const int64_t k = gtid();
-  if (k >= (int64_t)kSlotWords * n) return;
-  const int64_t g = k / n, i = k % n;
-  uint32_t h[4];
-#pragma unroll
-  for (int b = 0; b < 4; ++b) {
-    const int64_t j = 4 * g + b;
-    h[b] = j < kSlots ? (uint32_t)__builtin_bit_cast(uint16_t, val16[j * n + i]) : 0u;
-  }
-  q4[3 * k] = h[0] | (h[1] << 16);
-  q4[3 * k + 1] = h[2] | (h[3] << 16);
-  q4[3 * k + 2] = off8[k];
-}
-
 template <bool STREAM>
 __device__ inline void gs32_row16(const Col16 &C, const uint32_t *__restrict__ vc,
                                   int64_t i, int64_t n, const int32_t *__restrict__ col,
@@ -188,10 +137,6 @@ __device__ inline void gs32_row16(const Col16 &C, const uint32_t *__restrict__ v
   const int info = C.info[t];
   if (info == kTile32) {
     gs32_row<STREAM>(i, n, col, val16, diag32, r, x);
-    return;
-  }
-  if (info == kTile8 && C.use8) {
-    x[i] = sum8_32<STREAM>(C, t, C.q4, i, n, x, r[i], true) / diag32[i];
     return;
   }
   const int32_t *__restrict__ base = C.base + t;
@@ -214,8 +159,6 @@ __device__ inline float row_dot32_16(const Col16 &C, const uint32_t *__restrict_
   const int t = tile_of(i);
   const int info = C.info[t];
   if (info == kTile32) return row_dot32<STREAM>(i, n, col, val16, diag32, x);
-  if (info == kTile8 && C.use8)
-    return sum8_32<STREAM>(C, t, C.q4, i, n, x, 0.0f, false) + diag32[i] * x[i];
   const int32_t *__restrict__ base = C.base + t;
   float s = 0.0f;
 #pragma unroll
@@ -228,12 +171,18 @@ __device__ inline float row_dot32_16(const Col16 &C, const uint32_t *__restrict_
   return s;
 }
 
-// vc[k] = col16[k] << 16 | bits of val16[k]
-__global__ __launch_bounds__(kBlock) void k_pack_vc(int64_t m, const uint16_t *__restrict__ off,
+// vc[k] = col16[k] << 16 | bits of val16[k], one thread per row (a launch
+// holds fewer than 2^32 work-items; 27 n can exceed that).
+__global__ __launch_bounds__(kBlock) void k_pack_vc(int64_t n, const uint16_t *__restrict__ off,
                                                     const half_t *__restrict__ val16,
                                                     uint32_t *__restrict__ vc) {
-  const int64_t k = gtid();
-  if (k < m) vc[k] = ((uint32_t)off[k] << 16) | __builtin_bit_cast(uint16_t, val16[k]);
+  const int64_t i = gtid();
+  if (i >= n) return;
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int64_t k = (int64_t)j * n + i;
+    vc[k] = ((uint32_t)off[k] << 16) | __builtin_bit_cast(uint16_t, val16[k]);
+  }
 }
 
 template <bool STREAM, bool FINEST>
@@ -351,11 +300,7 @@ void prepare_mixed(Problem &P) {
     if (L.c16) {
       const int64_t m = (int64_t)kSlots * L.n;
       L.vc = P.arena.alloc<uint32_t>((size_t)m);
-      k_pack_vc<<<grid_for(m), kBlock>>>(m, L.col16, L.val16, L.vc);
-      HPCGX_LAUNCHED();
-      const int64_t mg = (int64_t)kSlotWords * L.n;
-      L.q4 = P.arena.alloc<uint32_t>((size_t)3 * mg);
-      k_pack_q4<<<grid_for(mg), kBlock>>>(L.n, L.val16, L.off8, L.q4);
+      k_pack_vc<<<grid_for(L.n), kBlock>>>(L.n, L.col16, L.val16, L.vc);
       HPCGX_LAUNCHED();
     }
     if (L.nc) {
@@ -488,7 +433,7 @@ void cg_mixed(Problem &P, int ordering, const double *b, double *x,
       to_f64(n, P.z32, P.p, s);
     else
       launch_update_p32(n, P.z32, P.p, P.scalars, s);
-    launch_spmv_dot(P, A, P.p, P.Ap, s);
+    launch_spmv_dot_half(P, A, P.p, P.Ap, s);
     launch_update_xr(P, n, x, P.r, P.p, P.Ap, s, P.r32);
     if (recompute > 0 && k % recompute == 0) {
       // r = b - A x in FP64 (replaces the recursively updated r)

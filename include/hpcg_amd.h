@@ -240,9 +240,11 @@ int hpcgx_symgs32(hpcgx_problem *p, int level, int ordering, const float *r,
 int hpcgx_mg32(hpcgx_problem *p, int level, int ordering, const float *r,
                float *x, void *stream);
 /* CG.cpp:59-143 in FP64 with the FP32 V-cycle as preconditioner (r rounded
- * to FP32 on the way in, z widened on the way out); after every
- * `recompute`-th iteration r = b - A x is recomputed in FP64 (0 = never).
- * times[0] accumulates the wall time.  Synchronous. */
+ * to FP32 on the way in, z widened on the way out).  A p uses FP64 products
+ * and sums over the FP16 off-diagonal values and the FP64 diagonal (equal
+ * to the FP64 SpMV while the off-diagonal values are exact in FP16); after
+ * every `recompute`-th iteration r = b - A x is recomputed with the FP64
+ * matrix (0 = never).  times[0] accumulates the wall time.  Synchronous. */
 int hpcgx_cg_mixed(hpcgx_problem *p, int ordering, const double *b, double *x,
                    int max_iter, double tolerance, int recompute, int *niters,
                    double *normr, double *normr0, double *times, void *stream);

@@ -130,6 +130,39 @@ def test_mg32_bitexact(hp, dims, ordering):
     assert np.array_equal(host32(P, z), h.mg32(r))
 
 
+def test_512_packed_words(hp):
+    """512x512x640: 27 n = 4.5e9 ELL entries, more than one launch's 2^32
+    work-items.  The packed FP16/column words must equal the 32-bit path:
+    FP32 SpMV and V-cycle bit for bit, and the mixed CG's first iterations
+    match the FP64-matrix ones (regression: the words were built by a
+    launch over all entries)."""
+    import torch
+    dims = (512, 512, 640)
+    say("512 packed words", dims)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    P.prepare_mixed()
+    n = P.nrow(0)
+    g = torch.Generator(device="cuda").manual_seed(3)
+    x = torch.rand(n, device="cuda", generator=g, dtype=torch.float32) * 2 - 1
+    out = {}
+    for c16 in (1, 0):
+        P.tune("col16", c16)
+        y, z = P.zeros32(), P.zeros32()
+        P.spmv32(x, y)
+        P.mg32(x, z, ordering="multicolor")
+        out[c16] = (y, z)
+    assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
+    b, xx, _ = P.vectors()
+    res = {}
+    for c16 in (1, 0):
+        P.tune("col16", c16)
+        xx.zero_()
+        res[c16] = P.cg_mixed(b, xx, max_iter=3, tolerance=0.0, recompute=10)
+    assert res[1]["normr"] == pytest.approx(res[0]["normr"], rel=1e-12)
+    assert res[1]["normr"] / res[1]["normr0"] < 0.2
+    del P
+
+
 def test_cg_mixed_trajectory(hp):
     dims = (32, 48, 64)
     say("cg mixed trajectory", dims)
