@@ -254,14 +254,17 @@ def main():
     # (two symmetric sweeps per iteration: pre- and post-smoother)
     info0 = P.info(0)
     c16 = bool(a.col16 and info0["col16"])
+    ntiles0 = (lv[0]["n"] + 63) // 64
+    c8 = c16 and info0["byte_tiles"] == ntiles0 and not any(
+        l.strip() == "chosen col8=0" for l in (tuned or []))
     ksym = ("k_gs_wave<true,true>" if info0["wave"] else
             "k_gs_range16<true,true>" if c16 else "k_gs_range<true,true>")
     kname = ksym + (" (finest-level wavefront sweep, one direction)" if info0["wave"]
                     else " (finest-level colour step)")
     per_launch = K * 2 * cb["symgs_L0"] / max(ks_n, 1)
     # the same sweep in the stored format: every ELL slot (padding included)
-    # with 8-byte values and 2- (16-bit ids) or 4-byte column ids
-    a_st = 8 + (2 if c16 else 4)
+    # with 8-byte values and 1- (8-bit offsets), 2- (16-bit) or 4-byte ids
+    a_st = 8 + (1 if c8 else 2 if c16 else 4)
     stored = K * 2 * (2 * a_st * 27 * lv[0]["n"] + 6 * 8 * lv[0]["n"]) / max(ks_n, 1)
     launch_ms = ks_ms / max(ks_n, 1)
     achieved = per_launch / (launch_ms * 1e-3) * 1e-9 if ks_n else None
@@ -309,7 +312,7 @@ def main():
             "traffic": round(traffic) if traffic else None,
             "bytes_per_launch": per_launch,
             "stored_format_bytes_per_launch": stored,
-            "column_ids": "16-bit" if c16 else "32-bit",
+            "column_ids": "8-bit" if c8 else "16-bit" if c16 else "32-bit",
             "mean_launch_ms": round(launch_ms, 5),
             "launch_timing": "eager pass after the timed region" if a.graph else "timed region",
             "launches": ks_n,
