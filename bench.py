@@ -58,6 +58,13 @@ def parse():
     ap.add_argument("--block", type=int, nargs="*", default=[],
                     help="also solve s right-hand sides at once (block PCG, FP32 V-cycle) "
                          "for each s given, to --tol-compare")
+    ap.add_argument("--fleet", type=int, default=0,
+                    help="also run this many independent problems (grids drawn from "
+                         "{32,48,64,96,128} per axis by splitmix64 seed 7) as one fleet, "
+                         "--steps iterations, against the same problems one at a time "
+                         "(0 = skip)")
+    ap.add_argument("--fleet-only", action="store_true",
+                    help="run only the fleet measurement and print its JSON")
     ap.add_argument("--autotune", type=int, default=1, choices=[0, 1],
                     help="pick the tuning knobs by timing (hpcgx_autotune, before the warm-up)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -183,6 +190,70 @@ def block_solve(P, s, a, log):
             "speedup_vs_single_rhs": round(seq / dt, 4)}
 
 
+def fleet_solve(count, a, log):
+    """`count` independent HPCG problems (hpcg_amd.fleet_dims(count, seed=7))
+    as one fleet: setup, autotune, then a timed set of --steps PCG iterations
+    of every problem (tolerance 0, captured iteration), against the same
+    problems solved one after another with the single-problem CG (captured
+    iteration as well)."""
+    import torch
+    import hpcg_amd
+    from hpcg_amd import model
+    dims = hpcg_amd.fleet_dims(count, seed=7)
+    log("fleet of %d problems, %d rows" % (count, sum(x * y * z for x, y, z in dims)))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    F = hpcg_amd.Problem.generate_fleet(dims, nlevels=4)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    tuned = F.autotune(ordering=a.ordering) if a.autotune else None
+    b, x, _ = F.vectors()
+    F.cg_fleet(b, x, max_iter=3, ordering=a.ordering, graph=True)  # warm, capture
+    x.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = F.cg_fleet(b, x, max_iter=a.steps, tolerance=0.0, ordering=a.ordering, graph=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    K = a.steps
+    flops, canon = 0.0, 0.0
+    for d in dims:
+        lv = model.grid_levels(*d)
+        flops += model.hpcg_model([l["n"] for l in lv], [l["nnz"] for l in lv], 1, K, K)["total"]
+        canon += model.canonical_bytes(lv)["iteration"] * K
+    log("fleet: %d iterations in %.3f s" % (K, dt))
+    F.close()
+    seq, worst = 0.0, 0.0
+    for q, d in enumerate(dims):
+        P = hpcg_amd.Problem.generate(*d, nlevels=4)
+        bq, xq, _ = P.vectors()
+        P.cg(bq, xq, max_iter=3, ordering=a.ordering, graph=True)
+        xq.zero_()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        s1 = P.cg(bq, xq, max_iter=K, tolerance=0.0, ordering=a.ordering, graph=True)
+        torch.cuda.synchronize()
+        seq += time.perf_counter() - t1
+        rel_f = r["normr"][q] / r["normr0"][q]
+        rel_s = s1["normr"] / s1["normr0"]
+        worst = max(worst, abs(rel_f - rel_s) / rel_s)
+        P.close()
+    log("one at a time: %.3f s" % seq)
+    out = {"problems": count, "grids": [list(d) for d in dims],
+           "rows": sum(x * y * z for x, y, z in dims),
+           "iterations": K, "seconds": round(dt, 4), "ms_per_iteration": round(dt / K * 1e3, 4),
+           "hpcg_gflops": round(flops / dt * 1e-9, 3),
+           "iteration_gbs_canonical": round(canon / dt * 1e-9, 1),
+           "iteration_hbm_frac": round(canon / dt * 1e-9 / HBM_PEAK_GBS, 4),
+           "setup_s": round(setup, 3),
+           "one_at_a_time_seconds": round(seq, 4),
+           "speedup_vs_one_at_a_time": round(seq / dt, 4),
+           "max_rel_diff_normr_vs_single": worst}
+    if tuned is not None:
+        out["autotune"] = [l for l in tuned if l.startswith("chosen")]
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -204,6 +275,9 @@ def main():
         print("[bench]", *m, file=sys.stderr, flush=True)
 
     torch.cuda.init()
+    if a.fleet_only:
+        print(json.dumps({"fleet": fleet_solve(max(a.fleet, 1), a, log)}), flush=True)
+        return
     log("generate %dx%dx%d on device" % (a.nx, a.ny, a.nz))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -324,6 +398,8 @@ def main():
         out["convergence"] = converge(P, b, lv, a, log)
     if a.block:
         out["block_pcg"] = [block_solve(P, s, a, log) for s in a.block]
+    if a.fleet:
+        out["fleet"] = fleet_solve(a.fleet, a, log)
     if not a.no_cpu_baseline:
         log("cpu baseline")
         try:

@@ -10,6 +10,7 @@ from ._lib import (CG_GRAPH, CG_TIMERS, KEEP_SCHEDULE, ORDER_LEXICOGRAPHIC, ORDE
                    exported_symbols, lib)
 from .problem import Problem
 from . import model
+from .fleet import fleet_dims
 
 __all__ = ["Problem", "HpcgxError", "ORDER_LEXICOGRAPHIC", "ORDER_MULTICOLOR", "model", "lib",
-           "exported_symbols", "KEEP_SCHEDULE", "CG_TIMERS", "CG_GRAPH"]
+           "exported_symbols", "KEEP_SCHEDULE", "CG_TIMERS", "CG_GRAPH", "fleet_dims"]

@@ -81,6 +81,9 @@ SIGNATURES = {
     "hpcgx_autotune": (I, [P, I, I, ctypes.c_char_p, I]),
     "hpcgx_mg32_block": (I, [P, I, I, I, P, P, P]),
     "hpcgx_cg_block_mixed": (I, [P, I, I, P, P, I, D, I, P, P, P, P, P]),
+    "hpcgx_generate_fleet": (I, [I, P, P, P, I, I, PP]),
+    "hpcgx_fleet_info": (I, [P, I, P, P]),
+    "hpcgx_cg_fleet": (I, [P, I, P, P, I, D, I, I, P, P, P, P, P]),
     "hpcgx_tune": (I, [P, ctypes.c_char_p, D]),
     "hpcgx_check_status": (I, [P, P]),
     "hpcgx_kernel_timing": (I, [P, I]),

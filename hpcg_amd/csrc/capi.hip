@@ -1,6 +1,7 @@
 // capi.hip -- the extern "C" entry points of libhpcg_amd.so (include/hpcg_amd.h).
 // Exceptions never cross this boundary: each entry point maps them to an
 // HPCGX_E* code and records the message for hpcgx_last_error().
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -393,12 +394,18 @@ int hpcgx_tune(hpcgx_problem *p, const char *key, double value) {
     const std::string k(key);
     hpcgx::Problem &P = p->P;
     P.graph.reset();  // kernel choices may change: recapture on next use
+    P.fleet.graph.reset();
     if (k == "col16") {
       P.use_c16 = value != 0.0;
       return;
     }
     if (k == "col8") {
       P.use_c8 = value != 0.0;
+      return;
+    }
+    if (k == "fleet_blocks") {
+      if (value < 1 || value > 65536) throw Error(HPCGX_EINVAL, "fleet_blocks must be in [1, 65536]");
+      P.fleet.grid = (int)value;
       return;
     }
     if (k == "spmv_dot_blocks") {
@@ -462,12 +469,23 @@ int hpcgx_autotune(hpcgx_problem *p, int ordering, int iterations, char *report,
     auto timed = [&]() {
       int it;
       double nr, nr0;
+      // a fleet is timed with its own CG (per-problem scalars)
+      const int Q = std::max(1, P.fleet.count);
+      std::vector<int> fit(Q);
+      std::vector<double> fnr(Q), fnr0(Q);
+      auto solve = [&](int iters) {
+        if (P.fleet.count)
+          hpcgx::cg_fleet(P, ordering, b, x, iters, 0.0, true, HPCGX_CG_GRAPH, fit.data(),
+                          fnr.data(), fnr0.data(), nullptr, nullptr);
+        else
+          hpcgx::cg(P, ordering, b, x, iters, 0.0, true, 0, &it, &nr, &nr0, nullptr, nullptr);
+      };
       hpcgx::launch_fill(x, n, 0.0, nullptr);
-      hpcgx::cg(P, ordering, b, x, 1, 0.0, true, 0, &it, &nr, &nr0, nullptr, nullptr);
+      solve(1);
       hpcgx::launch_fill(x, n, 0.0, nullptr);
       HPCGX_HIP(hipDeviceSynchronize());
       const auto t0 = std::chrono::steady_clock::now();
-      hpcgx::cg(P, ordering, b, x, iterations, 0.0, true, 0, &it, &nr, &nr0, nullptr, nullptr);
+      solve(iterations);
       HPCGX_HIP(hipDeviceSynchronize());
       return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() /
              iterations;
@@ -480,7 +498,10 @@ int hpcgx_autotune(hpcgx_problem *p, int ordering, int iterations, char *report,
     for (int l = 0; l < P.nlevels; ++l) any_c16 = any_c16 || P.L[l].c16;
     std::vector<Knob> knobs;
     if (any_c16) knobs.push_back({"col8", {1, 0}});
-    knobs.push_back({"spmv_dot_blocks", {1024, 2048, 4096, 8192}});
+    if (P.fleet.count)
+      knobs.push_back({"fleet_blocks", {512, 1024, 2048, 4096}});
+    else
+      knobs.push_back({"spmv_dot_blocks", {1024, 2048, 4096, 8192}});
     knobs.push_back({"xcd_remap", {0, 1}});
     std::string log;
     char line[160];
@@ -594,6 +615,87 @@ int hpcgx_cg(hpcgx_problem *p, int ordering, const double *b, double *x, int max
     if (max_iter < 0) throw Error(HPCGX_EINVAL, "negative max_iter");
     hpcgx::cg(p->P, ordering, b, x, max_iter, tolerance, precondition != 0, flags, niters,
               normr, normr0, times, S(stream));
+  });
+}
+
+int hpcgx_generate_fleet(int count, const int *nx, const int *ny, const int *nz, int nlevels,
+                         int flags, hpcgx_problem **out) {
+  return guard([&] {
+    if (!out || !nx || !ny || !nz) throw Error(HPCGX_EINVAL, "NULL argument");
+    *out = nullptr;
+    if (count < 1 || count > hpcgx::kMaxFleet)
+      throw Error(HPCGX_EINVAL, "fleet size must be in [1, 1024]");
+    if (nlevels < 1 || nlevels > hpcgx::kMaxLevels) throw Error(HPCGX_EINVAL, "nlevels out of range");
+    const int div = 1 << (nlevels - 1);
+    std::vector<int> dims(3 * (size_t)count);
+    int64_t total = 0;
+    for (int q = 0; q < count; ++q) {
+      if (nx[q] < 1 || ny[q] < 1 || nz[q] < 1)
+        throw Error(HPCGX_EINVAL, "grid dimensions must be >= 1");
+      if (nx[q] % div || ny[q] % div || nz[q] % div)  // GenerateCoarseProblem.cpp:50
+        throw Error(HPCGX_EINVAL, "grid dimensions must be divisible by 2^(nlevels-1)");
+      dims[3 * q] = nx[q];
+      dims[3 * q + 1] = ny[q];
+      dims[3 * q + 2] = nz[q];
+      total += (int64_t)nx[q] * ny[q] * nz[q];
+    }
+    if (total >= ((int64_t)1 << 31))
+      throw Error(HPCGX_EINVAL, "fleet rows must be < 2^31 (int32 row ids)");
+    auto h = std::make_unique<hpcgx_problem>();
+    hpcgx::Problem &P = h->P;
+    HPCGX_HIP(hipGetDevice(&P.device));
+    P.nlevels = nlevels;
+    P.generated = true;
+    P.L[0].finest = true;
+    std::vector<NaturalOwner> nat(nlevels);
+    hipStream_t s = nullptr;
+    for (int l = 0; l < nlevels; ++l) {
+      int64_t n = 0;
+      for (int q = 0; q < count; ++q)
+        n += (int64_t)(dims[3 * q] >> l) * (dims[3 * q + 1] >> l) * (dims[3 * q + 2] >> l);
+      nat[l].alloc(n, (l + 1 < nlevels) ? n / 8 : 0);
+      int64_t off = 0, offc = 0;
+      for (int q = 0; q < count; ++q) {
+        const int x = dims[3 * q] >> l, y = dims[3 * q + 1] >> l, z = dims[3 * q + 2] >> l;
+        hpcgx::generate_natural_part(nat[l].N, x, y, z, off, offc, s);
+        off += (int64_t)x * y * z;
+        offc += (int64_t)x * y * z / 8;
+      }
+      // not one grid: no band order, level dimensions reported as 0
+      hpcgx::finalize_level(P, l, nat[l].N, flags, s);
+      nat[l].drop_matrix();
+    }
+    for (int l = 0; l + 1 < nlevels; ++l) hpcgx::link_levels(P, l, nat[l].N, s);
+    HPCGX_HIP(hipStreamSynchronize(s));
+    alloc_solver_state(P);
+    hpcgx::prepare_fleet(P, dims, s);
+    HPCGX_HIP(hipStreamSynchronize(s));
+    *out = h.release();
+  });
+}
+
+int hpcgx_fleet_info(const hpcgx_problem *p, int level, int *count, int64_t *row_offsets) {
+  return guard([&] {
+    if (!p || !count) throw Error(HPCGX_EINVAL, "NULL argument");
+    const hpcgx::Problem &P = p->P;
+    *count = P.fleet.count;
+    if (P.fleet.count == 0) return;
+    if (level < 0 || level >= P.nlevels) throw Error(HPCGX_EINVAL, "level out of range");
+    if (row_offsets)
+      std::copy(P.fleet.off[level].begin(), P.fleet.off[level].end(), row_offsets);
+  });
+}
+
+int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, int max_iter,
+                   double tolerance, int precondition, int flags, int *niters, double *normr,
+                   double *normr0, double *times, void *stream) {
+  return guard([&] {
+    if (!p || !b || !x || !niters || !normr || !normr0) throw Error(HPCGX_EINVAL, "NULL argument");
+    check_order(ordering);
+    if (max_iter < 0) throw Error(HPCGX_EINVAL, "negative max_iter");
+    if (p->P.fleet.count == 0) throw Error(HPCGX_EINVAL, "not a fleet (hpcgx_generate_fleet)");
+    hpcgx::cg_fleet(p->P, ordering, b, x, max_iter, tolerance, precondition != 0, flags, niters,
+                    normr, normr0, times, S(stream));
   });
 }
 

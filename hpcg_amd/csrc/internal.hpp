@@ -208,6 +208,31 @@ struct Problem {
       if (cap) (void)hipStreamDestroy(cap);
     }
   } graph;
+  // A fleet: the block-diagonal union of `count` independent problems
+  // (fleet.hip).  Level l holds every problem's level l, natural rows
+  // concatenated problem by problem (problem q: rows [off[l][q], off[l][q+1])).
+  // Setup, SpMV and the V-cycle treat it as one matrix; CG keeps its scalars
+  // per problem.  The fine level's device rows are cut into pieces of at most
+  // kPieceRows rows inside one (problem, colour) range, listed problem by
+  // problem: pieces[3k..3k+2] = begin, end, problem; problem q owns pieces
+  // [pptr[q], pptr[q+1]).
+  struct FleetState {
+    int count = 0;  // 0: not a fleet
+    std::vector<int> dims;  // nx, ny, nz of each problem
+    std::vector<int64_t> off[kMaxLevels];
+    int64_t npieces = 0;
+    int32_t *pieces = nullptr;
+    int32_t *pptr = nullptr;
+    double *partials = nullptr;  // [npieces]
+    unsigned *ticket = nullptr;
+    double *S = nullptr;         // per-problem scalars, slot-major: S[slot * count + q]
+    int *live = nullptr;         // problem still iterating
+    int *niters = nullptr;
+    int *nlive = nullptr;
+    int grid = 2048;             // persistent grid of the piece kernels
+    IterGraph graph;             // captured fleet iteration
+    double graph_tol = -1.0;
+  } fleet;
   // event log of the finest level's multicolour Gauss-Seidel launches
   struct KernelLog {
     bool on = false;
@@ -298,6 +323,9 @@ struct NaturalLevel {  // device buffers of one level in natural row order
   int64_t nc = 0;
 };
 void generate_natural(NaturalLevel &N, hipStream_t s);
+// one nx*ny*nz grid at natural rows [off, ...) of N (coarse rows at offc)
+void generate_natural_part(NaturalLevel &N, int nx, int ny, int nz, int64_t off,
+                           int64_t offc, hipStream_t s);
 void finalize_level(Problem &P, int l, NaturalLevel &N, int flags,
                     hipStream_t s);
 void link_levels(Problem &P, int l, const NaturalLevel &Nf, hipStream_t s);
@@ -329,6 +357,13 @@ void cg_block_mixed(Problem &P, int ordering, int s, const double *B, double *X,
                     double tol, int recompute, int *niters, double *normr, double *normr0,
                     double *times, hipStream_t st);
 void to_f64(int64_t n, const float *in, double *out, hipStream_t s);
+// ---- fleets of independent problems (fleet.hip) ----
+constexpr int kMaxFleet = 1024;  // problems per fleet
+constexpr int kPieceRows = 4096; // rows per piece of the fleet's reductions
+void prepare_fleet(Problem &P, const std::vector<int> &dims, hipStream_t s);
+void cg_fleet(Problem &P, int ordering, const double *b, double *x, int max_iter,
+              double tol, bool precond, int flags, int *niters, double *normr,
+              double *normr0, double *times, hipStream_t s);
 void cg_mixed(Problem &P, int ordering, const double *b, double *x,
               int max_iter, double tol, int recompute, int *niters,
               double *normr, double *normr0, double *times, hipStream_t s);

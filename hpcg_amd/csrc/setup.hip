@@ -47,9 +47,12 @@ struct TempBuf {  // a temporary device buffer freed on scope exit
 };
 
 // GenerateProblem_ref.cpp:131-183 (single rank): natural order, neighbours
-// in sz, sy, sx order, 26 on the diagonal and -1 elsewhere.
+// in sz, sy, sx order, 26 on the diagonal and -1 elsewhere.  The grid's rows
+// are rows [off, off + nx*ny*nz) of a level of ntot rows (a fleet level
+// holds several grids one after the other; a single problem has off = 0).
 __global__ __launch_bounds__(kBlock) void k_generate(int nx, int ny, int nz,
-                                                      int64_t n, int32_t *col,
+                                                      int64_t n, int64_t off,
+                                                      int64_t ntot, int32_t *col,
                                                       double *val,
                                                       uint8_t *nnz_row) {
   const int64_t i = gtid();
@@ -59,6 +62,7 @@ __global__ __launch_bounds__(kBlock) void k_generate(int nx, int ny, int nz,
   const int iy = (int)(t % ny);
   const int iz = (int)(t / ny);
   const int64_t nxy = (int64_t)nx * ny;
+  const int64_t g = off + i;
   int cnt = 0;
   for (int sz = -1; sz <= 1; ++sz) {
     if (iz + sz < 0 || iz + sz >= nz) continue;
@@ -66,32 +70,33 @@ __global__ __launch_bounds__(kBlock) void k_generate(int nx, int ny, int nz,
       if (iy + sy < 0 || iy + sy >= ny) continue;
       for (int sx = -1; sx <= 1; ++sx) {
         if (ix + sx < 0 || ix + sx >= nx) continue;
-        const int64_t c = i + sz * nxy + (int64_t)sy * nx + sx;
-        col[(int64_t)cnt * n + i] = (int32_t)c;
-        val[(int64_t)cnt * n + i] = (c == i) ? 26.0 : -1.0;
+        const int64_t c = g + sz * nxy + (int64_t)sy * nx + sx;
+        col[(int64_t)cnt * ntot + g] = (int32_t)c;
+        val[(int64_t)cnt * ntot + g] = (c == g) ? 26.0 : -1.0;
         ++cnt;
       }
     }
   }
   for (int j = cnt; j < kSlots; ++j) {
-    col[(int64_t)j * n + i] = (int32_t)i;
-    val[(int64_t)j * n + i] = 0.0;
+    col[(int64_t)j * ntot + g] = (int32_t)g;
+    val[(int64_t)j * ntot + g] = 0.0;
   }
-  nnz_row[i] = (uint8_t)cnt;
+  nnz_row[g] = (uint8_t)cnt;
 }
 
-// GenerateCoarseProblem.cpp:71-82
+// GenerateCoarseProblem.cpp:71-82, for a grid whose fine rows start at offf
+// and whose coarse rows start at offc.
 __global__ __launch_bounds__(kBlock) void k_f2c(int nxf, int nyf, int64_t nc,
-                                                 int nxc, int nyc,
-                                                 int32_t *f2c) {
+                                                 int nxc, int nyc, int64_t offf,
+                                                 int64_t offc, int32_t *f2c) {
   const int64_t c = gtid();
   if (c >= nc) return;
   const int ixc = (int)(c % nxc);
   const int64_t t = c / nxc;
   const int iyc = (int)(t % nyc);
   const int izc = (int)(t / nyc);
-  f2c[c] = (int32_t)((int64_t)(2 * izc) * nxf * nyf + (int64_t)(2 * iyc) * nxf +
-                     2 * ixc);
+  f2c[offc + c] = (int32_t)(offf + (int64_t)(2 * izc) * nxf * nyf +
+                            (int64_t)(2 * iyc) * nxf + 2 * ixc);
 }
 
 __global__ __launch_bounds__(kBlock) void k_count_nnz(int64_t n,
@@ -496,13 +501,20 @@ void build_chunks(Problem &P, Level &L) {
 }  // namespace
 
 void generate_natural(NaturalLevel &N, hipStream_t s) {
-  const int64_t n = N.n;
-  k_generate<<<grid_for(n), kBlock, 0, s>>>(N.nx, N.ny, N.nz, n, N.col, N.val,
+  generate_natural_part(N, N.nx, N.ny, N.nz, 0, 0, s);
+}
+
+void generate_natural_part(NaturalLevel &N, int nx, int ny, int nz, int64_t off,
+                           int64_t offc, hipStream_t s) {
+  const int64_t n = (int64_t)nx * ny * nz;
+  HPCGX_REQUIRE(off >= 0 && off + n <= N.n, "grid rows outside the level");
+  k_generate<<<grid_for(n), kBlock, 0, s>>>(nx, ny, nz, n, off, N.n, N.col, N.val,
                                             N.nnz_row);
   HPCGX_LAUNCHED();
   if (N.f2c) {
-    k_f2c<<<grid_for(N.nc), kBlock, 0, s>>>(N.nx, N.ny, N.nc, N.nx / 2,
-                                            N.ny / 2, N.f2c);
+    const int64_t nc = n / 8;
+    HPCGX_REQUIRE(offc >= 0 && offc + nc <= N.nc, "coarse rows outside the next level");
+    k_f2c<<<grid_for(nc), kBlock, 0, s>>>(nx, ny, nc, nx / 2, ny / 2, off, offc, N.f2c);
     HPCGX_LAUNCHED();
   }
 }

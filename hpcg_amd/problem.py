@@ -117,6 +117,48 @@ class Problem:
         del keep
         return cls(h)
 
+    @classmethod
+    def generate_fleet(cls, dims, nlevels=4, keep_schedule=False):
+        """hpcgx_generate_fleet: independent problems on the grids in `dims`
+        ([(nx, ny, nz)]) as one handle.  Level l is the block-diagonal union
+        of the problems' level l (natural rows problem by problem); every
+        level method treats it as one matrix, cg_fleet solves each problem
+        with its own scalars."""
+        q = len(dims)
+        nx = np.ascontiguousarray([d[0] for d in dims], np.int32)
+        ny = np.ascontiguousarray([d[1] for d in dims], np.int32)
+        nz = np.ascontiguousarray([d[2] for d in dims], np.int32)
+        h = ctypes.c_void_p()
+        call("hpcgx_generate_fleet", q, nx.ctypes.data, ny.ctypes.data, nz.ctypes.data,
+             nlevels, _flags(keep_schedule, 0), ctypes.byref(h))
+        return cls(h)
+
+    def fleet_offsets(self, level=0):
+        """Natural row offsets (count + 1) of the fleet's problems on `level`,
+        or None for a single problem."""
+        c = ctypes.c_int()
+        call("hpcgx_fleet_info", self._h, level, ctypes.byref(c), None)
+        if c.value == 0:
+            return None
+        off = np.zeros(c.value + 1, np.int64)
+        call("hpcgx_fleet_info", self._h, level, ctypes.byref(c), off.ctypes.data)
+        return off
+
+    def cg_fleet(self, b, x, max_iter=50, tolerance=0.0, precondition=True,
+                 ordering="multicolor", graph=False):
+        """hpcgx_cg_fleet: CG on every problem of the fleet, each with its own
+        scalars and stopping test.  Returns per-problem niters, normr, normr0
+        (lists) and the times[] slots."""
+        n = self.nrow(0)
+        q = len(self.fleet_offsets()) - 1
+        ni = (ctypes.c_int * q)()
+        nr, nr0 = (ctypes.c_double * q)(), (ctypes.c_double * q)()
+        t = (ctypes.c_double * 6)()
+        call("hpcgx_cg_fleet", self._h, _order(ordering), _ptr(b, n, "b"), _ptr(x, n, "x"),
+             int(max_iter), float(tolerance), int(bool(precondition)),
+             CG_GRAPH if graph else 0, ni, nr, nr0, t, _stream())
+        return dict(niters=list(ni), normr=list(nr), normr0=list(nr0), times=list(t))
+
     def close(self):
         if getattr(self, "_h", None):
             _lib.lib().hpcgx_problem_destroy(self._h)
@@ -323,7 +365,7 @@ class Problem:
         return dict(niters=ni.value, normr=nr.value, normr0=nr0.value, times=list(t))
 
     def tune(self, key, value):
-        """Set a tuning knob ("col16", "col8", "spmv_dot_blocks", "wave", "wave_min_rows",
+        """Set a tuning knob ("col16", "col8", "spmv_dot_blocks", "fleet_blocks", "wave", "wave_min_rows",
         "wave_slab_factor", "wave_rows", "xcd_remap", "pair_rows",
         "stream_min_bytes"); see include/hpcg_amd.h."""
         return call("hpcgx_tune", self._h, key.encode(), float(value))

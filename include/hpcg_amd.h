@@ -269,6 +269,34 @@ int hpcgx_cg_block_mixed(hpcgx_problem *p, int ordering, int s, const double *B,
                          int max_iter, double tolerance, int recompute, int *niters,
                          double *normr, double *normr0, double *times, void *stream);
 
+/* ---- fleets of independent problems ----------------------------------------
+ * No counterpart in the reference (main.cpp runs one problem per rank).  A
+ * fleet holds `count` independent HPCG problems with grids of their own
+ * sizes as one handle: level l is the block-diagonal union of the problems'
+ * level l, natural rows concatenated problem by problem, so every level
+ * entry point above (SpMV, SYMGS, MG, vectors, to/from device order) treats
+ * the fleet as one matrix and each launch advances all problems.  Each row's
+ * arithmetic is that of its own problem. */
+
+/* Generates problem q on an nx[q] x ny[q] x nz[q] grid (each divisible by
+ * 2^(nlevels-1)), q = 0..count-1 (count <= 1024), and sets everything up
+ * as hpcgx_generate_problem does.  Synchronous. */
+int hpcgx_generate_fleet(int count, const int *nx, const int *ny, const int *nz, int nlevels,
+                         int flags, hpcgx_problem **out);
+/* *count = problems in the fleet (0 for a single problem).  When
+ * row_offsets is not NULL it receives count + 1 entries: problem q owns the
+ * natural rows [row_offsets[q], row_offsets[q+1]) of `level`. */
+int hpcgx_fleet_info(const hpcgx_problem *p, int level, int *count, int64_t *row_offsets);
+/* CG.cpp:59-143 for every problem of a fleet at once, each with its own
+ * scalars: problem q iterates while normr[q]/normr0[q] > tolerance, up to
+ * max_iter, and then keeps its x.  b and x are fleet vectors (device order).
+ * niters, normr and normr0 are host arrays of `count` entries.  flags:
+ * HPCGX_CG_GRAPH replays iterations 2.. from one captured graph.  times[0]
+ * accumulates the wall time.  Synchronous. */
+int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, int max_iter,
+                   double tolerance, int precondition, int flags, int *niters, double *normr,
+                   double *normr0, double *times, void *stream);
+
 /* ---- tuning ----------------------------------------------------------------- */
 
 /* No counterpart in the reference (tuning knobs of this implementation):
@@ -293,13 +321,16 @@ int hpcgx_cg_block_mixed(hpcgx_problem *p, int ordering, int s, const double *B,
  *   "spmv_dot_blocks"  workgroups of the persistent SpMV + dot kernel
  *                      (8..8192, rounded down to a multiple of 8; default
  *                      4096).  Changes the dot's summation order.
+ *   "fleet_blocks"     workgroups of a fleet's persistent CG vector kernels
+ *                      (1..65536, default 2048; fleets only)
  * Synchronous.  Unknown keys return HPCGX_EINVAL. */
 int hpcgx_tune(hpcgx_problem *p, const char *key, double value);
 
 /* No counterpart in the reference: picks the tuning knobs for this problem
  * and machine by timing `iterations` PCG iterations (plus one warm-up) per
- * candidate, one knob at a time ("col8", "spmv_dot_blocks", "xcd_remap"),
- * keeping the fastest value of each.  Writes one line per timing and per
+ * candidate, one knob at a time ("col8", "spmv_dot_blocks", "xcd_remap";
+ * a fleet times hpcgx_cg_fleet and tunes "fleet_blocks" in place of
+ * "spmv_dot_blocks"), keeping the fastest value of each.  Writes one line per timing and per
  * choice into `report` (may be NULL).  Uses the default stream; synchronous.
  * Changing "spmv_dot_blocks" changes the dots' summation order. */
 int hpcgx_autotune(hpcgx_problem *p, int ordering, int iterations, char *report,
