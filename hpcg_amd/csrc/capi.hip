@@ -346,7 +346,7 @@ int hpcgx_residual_restriction(const hpcgx_problem *p, int level, const double *
   return guard([&] {
     const hpcgx::Level &L = level_of(p, level);
     if (L.nc == 0) throw Error(HPCGX_EINVAL, "no coarser level");
-    hpcgx::launch_residual_restriction(L, rf, xf, rc, S(stream));
+    hpcgx::launch_residual_restriction(p->P, L, rf, xf, rc, S(stream));
   });
 }
 

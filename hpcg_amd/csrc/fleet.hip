@@ -146,7 +146,10 @@ __device__ void fleet_finish(const FleetView &F, int finish) {
 }
 
 // The piece kernels: each workgroup takes pieces blockIdx.x, blockIdx.x +
-// gridDim.x, ...; piece k covers device rows [beg, end) of problem q.
+// gridDim.x, ...; piece k covers device rows [beg, end) of problem q.  A
+// thread visits rows beg + threadIdx.x + kBlock*j, j = 0, 1, ... in order;
+// the main loops load four of them before using any (four rows' loads in
+// flight per wavefront; the sums are those of one row at a time).
 struct Piece {
   int beg, end, q;
 };
@@ -161,7 +164,22 @@ __global__ __launch_bounds__(kBlock) void k_fresidual0(FleetView F, const double
   for (int64_t k = blockIdx.x; k < F.npieces; k += gridDim.x) {
     const Piece w = piece_at(F, k);
     double s = 0.0;
-    for (int64_t i = w.beg + threadIdx.x; i < w.end; i += kBlock) {
+    int64_t i = w.beg + threadIdx.x;
+    for (; i + 3 * kBlock < w.end; i += 4 * kBlock) {
+      double bv[4], av[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        bv[u] = b[i + u * kBlock];
+        av[u] = Ap[i + u * kBlock];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double ri = bv[u] + -1.0 * av[u];
+        r[i + u * kBlock] = ri;
+        s += ri * ri;
+      }
+    }
+    for (; i < w.end; i += kBlock) {
       const double ri = b[i] + -1.0 * Ap[i];
       r[i] = ri;
       s += ri * ri;
@@ -178,8 +196,20 @@ __global__ __launch_bounds__(kBlock) void k_fdot(FleetView F, const double *__re
   for (int64_t k = blockIdx.x; k < F.npieces; k += gridDim.x) {
     const Piece w = piece_at(F, k);
     double s = 0.0;
-    if (F.live[w.q])
-      for (int64_t i = w.beg + threadIdx.x; i < w.end; i += kBlock) s += x[i] * y[i];
+    if (F.live[w.q]) {
+      int64_t i = w.beg + threadIdx.x;
+      for (; i + 3 * kBlock < w.end; i += 4 * kBlock) {
+        double a[4], c[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          a[u] = x[i + u * kBlock];
+          c[u] = y[i + u * kBlock];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) s += a[u] * c[u];
+      }
+      for (; i < w.end; i += kBlock) s += x[i] * y[i];
+    }
     s = fblock_sum(s);
     if (threadIdx.x == 0) F.partials[k] = s;
   }
@@ -209,7 +239,27 @@ __global__ __launch_bounds__(kBlock) void k_fupdate_xr(FleetView F, double *__re
     if (F.live[w.q]) {
       const double alpha = F.S[fAlpha * F.count + w.q];
       const double malpha = -alpha;
-      for (int64_t i = w.beg + threadIdx.x; i < w.end; i += kBlock) {
+      int64_t i = w.beg + threadIdx.x;
+      for (; i + 3 * kBlock < w.end; i += 4 * kBlock) {
+        double xv[4], pv[4], rv[4], av[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int64_t j = i + u * kBlock;
+          xv[u] = x[j];
+          pv[u] = p[j];
+          rv[u] = r[j];
+          av[u] = Ap[j];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int64_t j = i + u * kBlock;
+          x[j] = xv[u] + alpha * pv[u];
+          const double ri = rv[u] + malpha * av[u];
+          r[j] = ri;
+          s += ri * ri;
+        }
+      }
+      for (; i < w.end; i += kBlock) {
         x[i] = x[i] + alpha * p[i];
         const double ri = r[i] + malpha * Ap[i];
         r[i] = ri;

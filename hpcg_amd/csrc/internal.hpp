@@ -154,6 +154,13 @@ struct Level {
   // transfer to the next level: f2c[coarse device row] = fine device row
   int64_t nc = 0;
   int32_t *f2c = nullptr;
+  // When f2c maps the coarse rows one-to-one onto the fine device rows
+  // [rf0, rf0 + nc) and rf0 is a multiple of kTile (the injected points of
+  // the 27-point grid are one parity class, the first colour block), the
+  // transfers run in fine order: thread t owns fine row rf0 + t and coarse
+  // row rperm[t] (contiguous matrix rows, whole tiles per wavefront).
+  int64_t rf0 = 0;
+  int32_t *rperm = nullptr;
   double *rc = nullptr, *xc = nullptr;  // next level's residual and correction
 };
 
@@ -279,7 +286,7 @@ void launch_symgs_list(const Level &L, const int32_t *rows, int64_t count,
                        const double *r, double *x, hipStream_t s);
 void launch_restriction(const Level &L, const double *rf, const double *Axf,
                         double *rc, hipStream_t s);
-void launch_residual_restriction(const Level &L, const double *rf,
+void launch_residual_restriction(const Problem &P, const Level &L, const double *rf,
                                  const double *xf, double *rc, hipStream_t s);
 void launch_prolongation(const Level &L, const double *xc, double *xf,
                          hipStream_t s);

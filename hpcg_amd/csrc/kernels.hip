@@ -419,7 +419,31 @@ __global__ __launch_bounds__(kBlock) void k_resid_restrict(
   }
 }
 
+// The same in fine order (Level::rperm): thread t computes fine row
+// rf0 + t, so a wavefront reads whole tiles of the matrix (narrow column ids
+// when C16), and stores the coarse row rperm[t].
+template <bool STREAM, bool C16>
+__global__ __launch_bounds__(kBlock) void k_resid_restrict_f(
+    int64_t nc, int64_t rf0, const int32_t *__restrict__ rperm, int64_t n, Col16 C,
+    const int32_t *__restrict__ col, const double *__restrict__ val,
+    const double *__restrict__ rf, const double *__restrict__ xf, double *__restrict__ rc) {
+  const int64_t t = gtid();
+  if (t >= nc) return;
+  const int64_t f = rf0 + t;
+  const double ax = C16 ? row_dot16<STREAM>(C, col, val, xf, n, f)
+                        : row_dot<STREAM>(col, val, xf, n, f);
+  rc[rperm[t]] = rf[f] - ax;
+}
+
 // ComputeProlongation_ref.cpp:49 (f2c is injective: no write conflicts)
+__global__ __launch_bounds__(kBlock) void k_prolong_f(int64_t nc, int64_t rf0,
+                                                      const int32_t *__restrict__ rperm,
+                                                      const double *__restrict__ xc,
+                                                      double *__restrict__ xf) {
+  const int64_t t = gtid();
+  if (t < nc) xf[rf0 + t] += xc[rperm[t]];
+}
+
 __global__ __launch_bounds__(kBlock) void k_prolong(
     int64_t nc, const int32_t *__restrict__ f2c, const double *__restrict__ xc,
     double *__restrict__ xf) {
@@ -500,6 +524,10 @@ inline unsigned reduce_grid(int64_t n) {
   return (unsigned)(g < kReduceBlocks ? (g < 1 ? 1 : g) : kReduceBlocks);
 }
 
+// The grid-stride loops below visit i = gtid(), gtid() + stride, ... in that
+// order and add into one running sum in that order; the main loop loads
+// four of those rows before it uses any, so four rows' loads are in flight
+// per wavefront (the sums are the same as one row at a time).
 __global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const double *x,
                                                  const double *y,
                                                  double *partials,
@@ -508,7 +536,18 @@ __global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const double *x,
                                                  int finish) {
   double s = 0.0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = gtid(); i < n; i += stride) s += x[i] * y[i];
+  int64_t i = gtid();
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    double a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      a[u] = x[i + u * stride];
+      b[u] = y[i + u * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) s += a[u] * b[u];
+  }
+  for (; i < n; i += stride) s += x[i] * y[i];
   s = block_sum(s);
   grid_finish(s, partials, ticket, scalars, out, finish);
 }
@@ -595,7 +634,28 @@ __global__ __launch_bounds__(kBlock) void k_update_xr(
   const double malpha = -alpha;
   double s = 0.0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = gtid(); i < n; i += stride) {
+  int64_t i = gtid();
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    double xv[4], pv[4], rv[4], av[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t k = i + u * stride;
+      xv[u] = x[k];
+      pv[u] = p[k];
+      rv[u] = r[k];
+      av[u] = Ap[k];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t k = i + u * stride;
+      x[k] = xv[u] + alpha * pv[u];
+      const double ri = rv[u] + malpha * av[u];
+      r[k] = ri;
+      if (R32) r32[k] = (float)ri;
+      s += ri * ri;
+    }
+  }
+  for (; i < n; i += stride) {
     x[i] = x[i] + alpha * p[i];
     const double ri = r[i] + malpha * Ap[i];
     r[i] = ri;
@@ -612,7 +672,19 @@ __global__ __launch_bounds__(kBlock) void k_dot32(int64_t n, const double *x, co
                                                    double *scalars, double *out, int finish) {
   double s = 0.0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = gtid(); i < n; i += stride) s += x[i] * (double)y[i];
+  int64_t i = gtid();
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    double a[4];
+    float b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      a[u] = x[i + u * stride];
+      b[u] = y[i + u * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) s += a[u] * (double)b[u];
+  }
+  for (; i < n; i += stride) s += x[i] * (double)y[i];
   s = block_sum(s);
   grid_finish(s, partials, ticket, scalars, out, finish);
 }
@@ -632,7 +704,22 @@ __global__ __launch_bounds__(kBlock) void k_residual0(
     double *scalars) {
   double s = 0.0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = gtid(); i < n; i += stride) {
+  int64_t i = gtid();
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    double bv[4], av[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      bv[u] = b[i + u * stride];
+      av[u] = Ap[i + u * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double ri = bv[u] + -1.0 * av[u];
+      r[i + u * stride] = ri;
+      s += ri * ri;
+    }
+  }
+  for (; i < n; i += stride) {
     const double ri = b[i] + -1.0 * Ap[i];
     r[i] = ri;
     s += ri * ri;
@@ -742,8 +829,27 @@ void launch_restriction(const Level &L, const double *rf, const double *Axf,
   HPCGX_LAUNCHED();
 }
 
-void launch_residual_restriction(const Level &L, const double *rf,
+void launch_residual_restriction(const Problem &P, const Level &L, const double *rf,
                                  const double *xf, double *rc, hipStream_t s) {
+  if (L.rperm) {
+    const Col16 C = col16_of(P, L);
+    const bool c16 = use_col16(P, L, L.rf0);
+    const unsigned g = grid_for(L.nc);
+    if (L.stream && c16)
+      k_resid_restrict_f<true, true><<<g, kBlock, 0, s>>>(L.nc, L.rf0, L.rperm, L.n, C, L.col,
+                                                          L.val, rf, xf, rc);
+    else if (L.stream)
+      k_resid_restrict_f<true, false><<<g, kBlock, 0, s>>>(L.nc, L.rf0, L.rperm, L.n, C, L.col,
+                                                           L.val, rf, xf, rc);
+    else if (c16)
+      k_resid_restrict_f<false, true><<<g, kBlock, 0, s>>>(L.nc, L.rf0, L.rperm, L.n, C, L.col,
+                                                           L.val, rf, xf, rc);
+    else
+      k_resid_restrict_f<false, false><<<g, kBlock, 0, s>>>(L.nc, L.rf0, L.rperm, L.n, C,
+                                                            L.col, L.val, rf, xf, rc);
+    HPCGX_LAUNCHED();
+    return;
+  }
   if (L.stream)
     k_resid_restrict<true><<<grid_for(L.nc), kBlock, 0, s>>>(
         L.nc, L.n, L.f2c, L.col, L.val, rf, xf, rc, L.xcd);
@@ -755,6 +861,11 @@ void launch_residual_restriction(const Level &L, const double *rf,
 
 void launch_prolongation(const Level &L, const double *xc, double *xf,
                          hipStream_t s) {
+  if (L.rperm) {
+    k_prolong_f<<<grid_for(L.nc), kBlock, 0, s>>>(L.nc, L.rf0, L.rperm, xc, xf);
+    HPCGX_LAUNCHED();
+    return;
+  }
   k_prolong<<<grid_for(L.nc), kBlock, 0, s>>>(L.nc, L.f2c, xc, xf);
   HPCGX_LAUNCHED();
 }

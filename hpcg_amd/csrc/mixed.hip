@@ -254,6 +254,30 @@ __global__ __launch_bounds__(kBlock) void k_prolong32(
   if (c < nc) xf[f2c[c]] += xc[c];
 }
 
+// The transfers in fine order (Level::rperm): thread t owns fine row rf0 + t
+// (whole tiles per wavefront, packed words when C16) and coarse row rperm[t].
+template <bool STREAM, bool C16>
+__global__ __launch_bounds__(kBlock) void k_resid_restrict32_f(
+    int64_t nc, int64_t rf0, const int32_t *__restrict__ rperm, int64_t n, Col16 C,
+    const uint32_t *__restrict__ vc, const int32_t *__restrict__ col,
+    const half_t *__restrict__ val16, const float *__restrict__ diag32,
+    const float *__restrict__ rf, const float *__restrict__ xf, float *__restrict__ rc) {
+  const int64_t t = gtid();
+  if (t >= nc) return;
+  const int64_t f = rf0 + t;
+  const float ax = C16 ? row_dot32_16<STREAM>(C, vc, f, n, col, val16, diag32, xf)
+                       : row_dot32<STREAM>(f, n, col, val16, diag32, xf);
+  rc[rperm[t]] = rf[f] - ax;
+}
+
+__global__ __launch_bounds__(kBlock) void k_prolong32_f(int64_t nc, int64_t rf0,
+                                                        const int32_t *__restrict__ rperm,
+                                                        const float *__restrict__ xc,
+                                                        float *__restrict__ xf) {
+  const int64_t t = gtid();
+  if (t < nc) xf[rf0 + t] += xc[rperm[t]];
+}
+
 void gs32_step(const Problem &P, const Level &L, int64_t begin, int64_t end,
                const float *r, float *x, hipStream_t s) {
   if (end <= begin) return;
@@ -372,15 +396,35 @@ void mg32(Problem &P, int l, int ordering, const float *r, float *x,
   HPCGX_LAUNCHED();
   symgs32(P, l, ordering, r, x, s);
   if (l + 1 < P.nlevels) {
-    if (L.stream)
-      k_resid_restrict32<true><<<grid_for(L.nc), kBlock, 0, s>>>(
-          L.nc, L.n, L.f2c, L.col, L.val16, L.diag32, r, x, L.rc32);
-    else
-      k_resid_restrict32<false><<<grid_for(L.nc), kBlock, 0, s>>>(
-          L.nc, L.n, L.f2c, L.col, L.val16, L.diag32, r, x, L.rc32);
+    const unsigned g = grid_for(L.nc);
+    if (L.rperm) {
+      const Col16 C = col16_of(P, L);
+      const bool c16 = L.vc && use_col16(P, L, L.rf0);
+      if (L.stream && c16)
+        k_resid_restrict32_f<true, true><<<g, kBlock, 0, s>>>(
+            L.nc, L.rf0, L.rperm, L.n, C, L.vc, L.col, L.val16, L.diag32, r, x, L.rc32);
+      else if (L.stream)
+        k_resid_restrict32_f<true, false><<<g, kBlock, 0, s>>>(
+            L.nc, L.rf0, L.rperm, L.n, C, L.vc, L.col, L.val16, L.diag32, r, x, L.rc32);
+      else if (c16)
+        k_resid_restrict32_f<false, true><<<g, kBlock, 0, s>>>(
+            L.nc, L.rf0, L.rperm, L.n, C, L.vc, L.col, L.val16, L.diag32, r, x, L.rc32);
+      else
+        k_resid_restrict32_f<false, false><<<g, kBlock, 0, s>>>(
+            L.nc, L.rf0, L.rperm, L.n, C, L.vc, L.col, L.val16, L.diag32, r, x, L.rc32);
+    } else if (L.stream) {
+      k_resid_restrict32<true><<<g, kBlock, 0, s>>>(L.nc, L.n, L.f2c, L.col, L.val16,
+                                                    L.diag32, r, x, L.rc32);
+    } else {
+      k_resid_restrict32<false><<<g, kBlock, 0, s>>>(L.nc, L.n, L.f2c, L.col, L.val16,
+                                                     L.diag32, r, x, L.rc32);
+    }
     HPCGX_LAUNCHED();
     mg32(P, l + 1, ordering, L.rc32, L.xc32, s);
-    k_prolong32<<<grid_for(L.nc), kBlock, 0, s>>>(L.nc, L.f2c, L.xc32, x);
+    if (L.rperm)
+      k_prolong32_f<<<g, kBlock, 0, s>>>(L.nc, L.rf0, L.rperm, L.xc32, x);
+    else
+      k_prolong32<<<g, kBlock, 0, s>>>(L.nc, L.f2c, L.xc32, x);
     HPCGX_LAUNCHED();
     symgs32(P, l, ordering, r, x, s);
   }

@@ -774,6 +774,26 @@ void link_levels(Problem &P, int l, const NaturalLevel &Nf, hipStream_t s) {
   F.xc = P.arena.alloc<double>(C.n);
   k_link<<<grid_for(C.n), kBlock, 0, s>>>(C.n, C.iperm, Nf.f2c, F.perm, F.f2c);
   HPCGX_LAUNCHED();
+  // fine-order transfers (Level::rperm) when f2c is onto a tile-aligned range
+  std::vector<int32_t> h(C.n);
+  HPCGX_HIP(hipMemcpyAsync(h.data(), F.f2c, sizeof(int32_t) * C.n, hipMemcpyDeviceToHost, s));
+  HPCGX_HIP(hipStreamSynchronize(s));
+  const auto mm = std::minmax_element(h.begin(), h.end());
+  const int64_t lo = *mm.first, hi = *mm.second;
+  if (hi - lo + 1 == C.n && lo % kTile == 0) {
+    std::vector<int32_t> rp(C.n, -1);
+    bool onto = true;
+    for (int64_t kc = 0; kc < C.n && onto; ++kc) {
+      int32_t &slot = rp[h[kc] - lo];
+      onto = slot < 0;
+      slot = (int32_t)kc;
+    }
+    if (onto) {
+      F.rf0 = lo;
+      F.rperm = P.arena.alloc<int32_t>(C.n);
+      HPCGX_HIP(hipMemcpy(F.rperm, rp.data(), sizeof(int32_t) * C.n, hipMemcpyHostToDevice));
+    }
+  }
 }
 
 void problem_vectors(const Problem &P, double *b, double *x, double *xexact,

@@ -85,7 +85,7 @@ void mg(Problem &P, int l, int ordering, const double *r, double *x,
   launch_fill(x, L.n, 0.0, s);                          // :42
   if (l + 1 < P.nlevels) {
     symgs(P, l, ordering, r, x, s);                     // :47
-    launch_residual_restriction(L, r, x, L.rc, s);      // :49-51
+    launch_residual_restriction(P, L, r, x, L.rc, s);   // :49-51
     mg(P, l + 1, ordering, L.rc, L.xc, s);              // :52
     launch_prolongation(L, L.xc, x, s);                 // :53
     symgs(P, l, ordering, r, x, s);                     // :55
