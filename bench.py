@@ -55,16 +55,21 @@ def parse():
                     help="replay a captured hipGraph of the CG iteration in the timed "
                          "region (the colour-step launches are then timed in a separate "
                          "eager pass of the same length)")
-    ap.add_argument("--block", type=int, nargs="*", default=[],
+    ap.add_argument("--block", type=int, nargs="*", default=[8, 16],
                     help="also solve s right-hand sides at once (block PCG, FP32 V-cycle) "
-                         "for each s given, to --tol-compare")
-    ap.add_argument("--fleet", type=int, default=0,
+                         "for each s given, to --tol-compare (none given = skip)")
+    ap.add_argument("--block-grid", type=int, default=256,
+                    help="grid edge of the block PCG problem (its own problem unless it "
+                         "equals the main grid)")
+    ap.add_argument("--fleet", type=int, default=32,
                     help="also run this many independent problems (grids drawn from "
                          "{32,48,64,96,128} per axis by splitmix64 seed 7) as one fleet, "
                          "--steps iterations, against the same problems one at a time "
                          "(0 = skip)")
     ap.add_argument("--fleet-only", action="store_true",
                     help="run only the fleet measurement and print its JSON")
+    ap.add_argument("--fleet-compare", type=int, default=1, choices=[0, 1],
+                    help="also solve the fleet's problems one at a time (0 = skip)")
     ap.add_argument("--autotune", type=int, default=1, choices=[0, 1],
                     help="pick the tuning knobs by timing (hpcgx_autotune, before the warm-up)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -224,7 +229,7 @@ def fleet_solve(count, a, log):
     log("fleet: %d iterations in %.3f s" % (K, dt))
     F.close()
     seq, worst = 0.0, 0.0
-    for q, d in enumerate(dims):
+    for q, d in enumerate(dims if a.fleet_compare else []):
         P = hpcg_amd.Problem.generate(*d, nlevels=4)
         bq, xq, _ = P.vectors()
         P.cg(bq, xq, max_iter=3, ordering=a.ordering, graph=True)
@@ -245,10 +250,11 @@ def fleet_solve(count, a, log):
            "hpcg_gflops": round(flops / dt * 1e-9, 3),
            "iteration_gbs_canonical": round(canon / dt * 1e-9, 1),
            "iteration_hbm_frac": round(canon / dt * 1e-9 / HBM_PEAK_GBS, 4),
-           "setup_s": round(setup, 3),
-           "one_at_a_time_seconds": round(seq, 4),
-           "speedup_vs_one_at_a_time": round(seq / dt, 4),
-           "max_rel_diff_normr_vs_single": worst}
+           "setup_s": round(setup, 3)}
+    if a.fleet_compare:
+        out.update({"one_at_a_time_seconds": round(seq, 4),
+                    "speedup_vs_one_at_a_time": round(seq / dt, 4),
+                    "max_rel_diff_normr_vs_single": worst})
     if tuned is not None:
         out["autotune"] = [l for l in tuned if l.startswith("chosen")]
     return out
@@ -397,7 +403,15 @@ def main():
     if a.tol_compare > 0:
         out["convergence"] = converge(P, b, lv, a, log)
     if a.block:
-        out["block_pcg"] = [block_solve(P, s, a, log) for s in a.block]
+        g = a.block_grid
+        if (g, g, g) == (a.nx, a.ny, a.nz):
+            out["block_pcg"] = [block_solve(P, s, a, log) for s in a.block]
+        else:
+            log("block PCG problem %d^3" % g)
+            PB = hpcg_amd.Problem.generate(g, g, g, nlevels=4)
+            out["block_pcg"] = [dict(block_solve(PB, s, a, log), grid=[g, g, g])
+                                for s in a.block]
+            PB.close()
     if a.fleet:
         out["fleet"] = fleet_solve(a.fleet, a, log)
     if not a.no_cpu_baseline:

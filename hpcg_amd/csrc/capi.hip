@@ -427,6 +427,8 @@ int hpcgx_tune(hpcgx_problem *p, const char *key, double value) {
       hpcgx::Level &L = P.L[l];
       if (k == "xcd_remap") {
         L.xcd = value != 0.0;
+      } else if (k == "spmv_order") {
+        if (L.chunks_interleaved != (value == 0.0)) hpcgx::build_chunks(P, L, value == 0.0);
       } else if (k == "pair_rows") {
         L.pair_rows = value != 0.0;
       } else if (k == "stream_min_bytes") {
@@ -502,6 +504,7 @@ int hpcgx_autotune(hpcgx_problem *p, int ordering, int iterations, char *report,
       knobs.push_back({"fleet_blocks", {512, 1024, 2048, 4096}});
     else
       knobs.push_back({"spmv_dot_blocks", {1024, 2048, 4096, 8192}});
+    knobs.push_back({"spmv_order", {0, 1}});
     knobs.push_back({"xcd_remap", {0, 1}});
     std::string log;
     char line[160];

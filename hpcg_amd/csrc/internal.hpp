@@ -111,6 +111,7 @@ struct Level {
   // row chunks (<= kBlock rows, [begin, end) pairs) interleaving the colours
   int64_t nchunks = 0;
   int64_t *chunks = nullptr;
+  bool chunks_interleaved = true;  // else colour block by colour block
   // wavefront multicolour sweeps: tile (c, s) = device rows of colour c in
   // natural-row slab s = [tile_ptr[c*(nslabs+1)+s], tile_ptr[c*(nslabs+1)+s+1])
   int64_t bandwidth = 0;  // max |col - row| in natural order
@@ -337,6 +338,7 @@ void finalize_level(Problem &P, int l, NaturalLevel &N, int flags,
                     hipStream_t s);
 void link_levels(Problem &P, int l, const NaturalLevel &Nf, hipStream_t s);
 void build_wave(Problem &P, Level &L, int slab_factor, int rows, hipStream_t s);
+void build_chunks(Problem &P, Level &L, bool interleave);
 void drop_wave(Problem &P, Level &L);
 void problem_vectors(const Problem &P, double *b, double *x, double *xexact,
                      hipStream_t s);

@@ -466,11 +466,18 @@ __global__ __launch_bounds__(kBlock) void k_tiles(int C, int S, int64_t W,
   tile_ptr[k] = lo;
 }
 
-// Row chunks of at most kBlock rows, ordered by their relative position in
-// their colour block and then by colour: chunk k of every colour comes
-// before chunk k+1 of any colour.  A sweep over this list reads each colour
-// block's x values at about the same relative position at the same time.
-void build_chunks(Problem &P, Level &L) {
+}  // namespace
+
+// Row chunks of at most kBlock rows.  Interleaved: ordered by their relative
+// position in their colour block and then by colour (chunk k of every colour
+// before chunk k+1 of any), so a sweep reads each colour block's x values
+// at about the same relative position at the same time.  Otherwise colour
+// block by colour block: the persistent SpMV's XCD x (which takes the x-th
+// eighth of the list) then streams about one colour block's matrix rows in
+// order, as a colour step does.
+void build_chunks(Problem &P, Level &L, bool interleave) {
+  if (L.chunks) P.arena.release(L.chunks, sizeof(int64_t) * 2 * L.nchunks);
+  L.chunks = nullptr;
   struct Chunk {
     double key;
     int colour;
@@ -484,9 +491,11 @@ void build_chunks(Problem &P, Level &L) {
       v.push_back({(k + 0.5) / (double)nch, c, b0 + k * kBlock,
                    std::min<int64_t>(b0 + (k + 1) * kBlock, e0)});
   }
-  std::stable_sort(v.begin(), v.end(), [](const Chunk &a, const Chunk &b) {
-    return a.key < b.key || (a.key == b.key && a.colour < b.colour);
-  });
+  if (interleave)
+    std::stable_sort(v.begin(), v.end(), [](const Chunk &a, const Chunk &b) {
+      return a.key < b.key || (a.key == b.key && a.colour < b.colour);
+    });
+  L.chunks_interleaved = interleave;
   std::vector<int64_t> h(2 * v.size());
   for (size_t k = 0; k < v.size(); ++k) {
     h[2 * k] = v[k].begin;
@@ -497,8 +506,6 @@ void build_chunks(Problem &P, Level &L) {
   HPCGX_HIP(hipMemcpy(L.chunks, h.data(), sizeof(int64_t) * h.size(),
                       hipMemcpyHostToDevice));
 }
-
-}  // namespace
 
 void generate_natural(NaturalLevel &N, hipStream_t s) {
   generate_natural_part(N, N.nx, N.ny, N.nz, 0, 0, s);
@@ -636,7 +643,7 @@ void finalize_level(Problem &P, int l, NaturalLevel &N, int flags,
   }
   k_invert<<<grid_for(n), kBlock, 0, s>>>(n, L.iperm, L.perm);
   HPCGX_LAUNCHED();
-  build_chunks(P, L);
+  build_chunks(P, L, true);
   L.stream = (double)n * kSlots * (sizeof(double) + sizeof(int32_t)) > kStreamBytes;
   HPCGX_LAUNCHED();
 

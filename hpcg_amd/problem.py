@@ -365,7 +365,8 @@ class Problem:
         return dict(niters=ni.value, normr=nr.value, normr0=nr0.value, times=list(t))
 
     def tune(self, key, value):
-        """Set a tuning knob ("col16", "col8", "spmv_dot_blocks", "fleet_blocks", "wave", "wave_min_rows",
+        """Set a tuning knob ("col16", "col8", "spmv_dot_blocks", "spmv_order", "fleet_blocks",
+        "wave", "wave_min_rows",
         "wave_slab_factor", "wave_rows", "xcd_remap", "pair_rows",
         "stream_min_bytes"); see include/hpcg_amd.h."""
         return call("hpcgx_tune", self._h, key.encode(), float(value))

@@ -439,8 +439,9 @@ def test_autotune(hp, torch):
     r0 = P.cg(b, x, max_iter=10, tolerance=0.0)
     rep = P.autotune(iterations=2)
     chosen = [l for l in rep if l.startswith("chosen ")]
-    assert {l.split()[1].split("=")[0] for l in chosen} == {"col8", "spmv_dot_blocks", "xcd_remap"}
-    assert len([l for l in rep if "ms/iteration" in l]) == 2 + 4 + 2
+    assert {l.split()[1].split("=")[0] for l in chosen} == {"col8", "spmv_dot_blocks",
+                                                            "spmv_order", "xcd_remap"}
+    assert len([l for l in rep if "ms/iteration" in l]) == 2 + 4 + 2 + 2
     x.zero_()
     r1 = P.cg(b, x, max_iter=10, tolerance=0.0)
     assert r1["normr"] == pytest.approx(r0["normr"], rel=1e-10)
@@ -449,6 +450,27 @@ def test_autotune(hp, torch):
     z = P.zeros()
     P.mg(P.to_device_order(rr), z, ordering="multicolor")
     assert np.array_equal(P.to_numpy(z), h.mg(rr))
+    del P
+
+
+@pytest.mark.parametrize("order", [1, 0])
+def test_spmv_order(hp, torch, order):
+    """Either chunk order gives the oracle's SpMV bits on every level and
+    the oracle's CG trajectory (only the dots' summation order changes)."""
+    dims = (32, 48, 64)
+    say("spmv order", order)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    P.tune("spmv_order", order)
+    h = hier(dims, color=True)
+    for l in range(4):
+        xv = rnd(31 + l, P.nrow(l))
+        y = P.zeros(l)
+        P.spmv(P.to_device_order(xv, l), y, level=l)
+        assert np.array_equal(P.to_numpy(y, l), O.spmv(h.level(l), xv)), l
+    want = h.cg(max_iter=12, tol=0.0)["traj"]
+    b, x, _ = P.vectors()
+    r = P.cg(b, x, max_iter=12, tolerance=0.0, graph=True)
+    assert r["normr"] / r["normr0"] == pytest.approx(want[12], rel=1e-9)
     del P
 
 
