@@ -36,22 +36,20 @@ __device__ inline T mload(const T *p) {
 // ---- per-right-hand-side reductions -------------------------------------
 // Thread t of a workgroup owns right-hand side t % s (the grid stride is a
 // multiple of s).  The workgroup's s sums go to partials[block * s + k]; the
-// last workgroup (ticket) adds them in block order for each k and applies
-// the finishing step to scalar slot k of each array in BlockScalars.
+// last workgroup (ticket) adds them in a fixed order for each k and applies
+// the finishing step to slot k of the scalar arrays (rtz, old rtz, alpha,
+// beta, |r|^2: s each).
 enum BlockFinish { kBStore = 0, kBBeta = 1, kBAlpha = 2 };
 
-__device__ inline void block_finish_s(double v, int s, double *partials, unsigned *ticket,
-                                      double *sc, int finish) {
-  __shared__ double red[kBlock];
+// The tail shared by both layouts: partials[blockIdx.x * s + k] are stored
+// (and the workgroup synchronised); the last workgroup reduces and finishes.
+// Its thread q sums the partials of column k = q % s from blocks q / s,
+// q / s + kBlock / s, ... (four loads in flight), then thread k adds the
+// kBlock / s sums of column k in thread order: a fixed order.
+__device__ inline void finish_tail_s(int s, double *partials, unsigned *ticket, double *sc,
+                                     int finish) {
   __shared__ int is_last;
-  red[threadIdx.x] = v;
-  __syncthreads();
-  if (threadIdx.x < (unsigned)s) {  // fixed order: rows of the block ascending
-    double t = 0.0;
-    for (int q = threadIdx.x; q < kBlock; q += s) t += red[q];
-    partials[(int64_t)blockIdx.x * s + threadIdx.x] = t;
-  }
-  __syncthreads();
+  __shared__ double red[kBlock];
   if (threadIdx.x == 0) {
     __threadfence();
     is_last = atomicAdd(ticket, 1u) == gridDim.x - 1;
@@ -59,12 +57,30 @@ __device__ inline void block_finish_s(double v, int s, double *partials, unsigne
   __syncthreads();
   if (!is_last) return;
   __threadfence();
+  {
+    const int k = threadIdx.x % s;
+    const unsigned step = kBlock / s, nb = gridDim.x;
+    double t = 0.0;
+    unsigned b = threadIdx.x / s;
+    for (; b + 3 * step < nb; b += 4 * step) {
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        v[u] = __hip_atomic_load(&partials[(int64_t)(b + u * step) * s + k], __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) t += v[u];
+    }
+    for (; b < nb; b += step)
+      t += __hip_atomic_load(&partials[(int64_t)b * s + k], __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    red[threadIdx.x] = t;
+  }
+  __syncthreads();
   if (threadIdx.x < (unsigned)s) {
     const int k = threadIdx.x;
     double t = 0.0;
-    for (unsigned b = 0; b < gridDim.x; ++b)
-      t += __hip_atomic_load(&partials[(int64_t)b * s + k], __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
+    for (int q = k; q < kBlock; q += s) t += red[q];
     double *rtz = sc, *oldrtz = sc + s, *alpha = sc + 2 * s, *beta = sc + 3 * s,
            *normr2 = sc + 4 * s;
     if (finish == kBStore) {
@@ -79,6 +95,241 @@ __device__ inline void block_finish_s(double v, int s, double *partials, unsigne
   }
   __syncthreads();
   if (threadIdx.x == 0) *ticket = 0u;
+}
+
+__device__ inline void block_finish_s(double v, int s, double *partials, unsigned *ticket,
+                                      double *sc, int finish) {
+  __shared__ double red[kBlock];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  if (threadIdx.x < (unsigned)s) {  // fixed order: rows of the block ascending
+    double t = 0.0;
+    for (int q = threadIdx.x; q < kBlock; q += s) t += red[q];
+    partials[(int64_t)blockIdx.x * s + threadIdx.x] = t;
+  }
+  __syncthreads();
+  finish_tail_s(s, partials, ticket, sc, finish);
+}
+
+// ---- W lanes per row, 16 bytes of right-hand sides per lane ---------------
+// For S in {4, 8, 16}: the W = S * (value bytes) / 16 lanes of a row each
+// hold V = 16 / (value bytes) of its S right-hand sides (4 FP32 or 2 FP64
+// values), so one load instruction of a wavefront reads 64/W whole rows of a
+// multivector, one contiguous kilobyte.  The row's ELL slots are read by its
+// W lanes at one address (the single-vector kernels' packed FP16 words and
+// narrow column ids where the level has them).  A wavefront's rows lie in
+// one tile.  Per column the arithmetic is the single-vector kernels'
+// (mixed.hip, kernels.hip), term for term.
+__device__ inline int tile_of(int64_t i) {
+  return __builtin_amdgcn_readfirstlane((int)(i / kTile));
+}
+__device__ inline half_t vc_val(uint32_t w) {
+  return __builtin_bit_cast(half_t, (uint16_t)(w & 0xffffu));
+}
+template <int S>
+__device__ inline void ldv(const float *p, float (&v)[S]) {
+#pragma unroll
+  for (int q = 0; q < S / 4; ++q) {
+    const float4 t = *reinterpret_cast<const float4 *>(p + 4 * q);
+    v[4 * q] = t.x;
+    v[4 * q + 1] = t.y;
+    v[4 * q + 2] = t.z;
+    v[4 * q + 3] = t.w;
+  }
+}
+template <int S>
+__device__ inline void stv(float *p, const float (&v)[S]) {
+#pragma unroll
+  for (int q = 0; q < S / 4; ++q)
+    *reinterpret_cast<float4 *>(p + 4 * q) = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2],
+                                                         v[4 * q + 3]);
+}
+template <int S>
+__device__ inline void ldv(const double *p, double (&v)[S]) {
+#pragma unroll
+  for (int q = 0; q < S / 2; ++q) {
+    const double2 t = *reinterpret_cast<const double2 *>(p + 2 * q);
+    v[2 * q] = t.x;
+    v[2 * q + 1] = t.y;
+  }
+}
+template <int S>
+__device__ inline void stv(double *p, const double (&v)[S]) {
+#pragma unroll
+  for (int q = 0; q < S / 2; ++q)
+    *reinterpret_cast<double2 *>(p + 2 * q) = make_double2(v[2 * q], v[2 * q + 1]);
+}
+
+// f(a, c) for every slot of row i in slot order: a the FP16 value (FP32),
+// c the column.  C16: packed words of a narrow tile (the caller's wavefront
+// lies in one tile), else the FP16 array and 32-bit ids.
+template <bool STREAM, bool C16, class F>
+__device__ inline void slots32(const Col16 &C, const uint32_t *__restrict__ vc,
+                               const int32_t *__restrict__ col,
+                               const half_t *__restrict__ val16, int64_t n, int64_t i, F &&f) {
+  const int t = C16 ? tile_of(i) : 0;
+  if (C16 && C.info[t] != kTile32) {
+    const int32_t *__restrict__ base = C.base + t;
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const uint32_t w = mload<STREAM>(vc + (int64_t)j * n + i);
+      f((float)vc_val(w), base[(int64_t)j * C.ntiles] + (int32_t)(w >> 16));
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const int64_t e = (int64_t)j * n + i;
+      f((float)mload<STREAM>(val16 + e), mload<STREAM>(col + e));
+    }
+  }
+}
+
+// One colour step, rows [begin, end) (begin a multiple of kTile when C16):
+// mixed.hip's gs32_row for each column.
+template <bool STREAM, int S, bool C16>
+__global__ __launch_bounds__(kBlock) void k_bgs32_rows(int64_t begin, int64_t end, int64_t n,
+                                                       Col16 C, const uint32_t *__restrict__ vc,
+                                                       const int32_t *__restrict__ col,
+                                                       const half_t *__restrict__ val16,
+                                                       const float *__restrict__ diag32,
+                                                       const float *__restrict__ R, float *X) {
+  constexpr int V = 4, W = S / V;
+  const int64_t g = gtid();
+  const int64_t i = begin + g / W;
+  const int o = (int)(g % W) * V;
+  if (i >= end) return;
+  float sum[V], xv[V];
+  ldv<V>(R + i * S + o, sum);
+  slots32<STREAM, C16>(C, vc, col, val16, n, i, [&](float a, int32_t c) {
+    ldv<V>(X + (int64_t)c * S + o, xv);
+#pragma unroll
+    for (int k = 0; k < V; ++k) sum[k] -= a * xv[k];
+  });
+  const float d = diag32[i];
+#pragma unroll
+  for (int k = 0; k < V; ++k) sum[k] = sum[k] / d;
+  stv<V>(X + i * S + o, sum);
+}
+
+// rc = rf[f2c] - (A xf)[f2c] in fine order (Level::rperm): mixed.hip's
+// row_dot32 for each column.
+template <bool STREAM, int S, bool C16>
+__global__ __launch_bounds__(kBlock) void k_bresid_restrict32_rows(
+    int64_t nc, int64_t rf0, const int32_t *__restrict__ rperm, int64_t n, Col16 C,
+    const uint32_t *__restrict__ vc, const int32_t *__restrict__ col,
+    const half_t *__restrict__ val16, const float *__restrict__ diag32,
+    const float *__restrict__ RF, const float *__restrict__ XF, float *__restrict__ RC) {
+  constexpr int V = 4, W = S / V;
+  const int64_t g = gtid();
+  const int64_t t = g / W;
+  const int o = (int)(g % W) * V;
+  if (t >= nc) return;
+  const int64_t f = rf0 + t;
+  float a[V], xv[V];
+#pragma unroll
+  for (int k = 0; k < V; ++k) a[k] = 0.0f;
+  slots32<STREAM, C16>(C, vc, col, val16, n, f, [&](float v, int32_t c) {
+    ldv<V>(XF + (int64_t)c * S + o, xv);
+#pragma unroll
+    for (int k = 0; k < V; ++k) a[k] += v * xv[k];
+  });
+  const float d = diag32[f];
+  ldv<V>(XF + f * S + o, xv);
+#pragma unroll
+  for (int k = 0; k < V; ++k) a[k] += d * xv[k];
+  float rf[V];
+  ldv<V>(RF + f * S + o, rf);
+#pragma unroll
+  for (int k = 0; k < V; ++k) rf[k] = rf[k] - a[k];
+  stv<V>(RC + (int64_t)rperm[t] * S + o, rf);
+}
+
+// xf[f2c] += xc in fine order
+template <int S>
+__global__ __launch_bounds__(kBlock) void k_bprolong32_rows(int64_t nc, int64_t rf0,
+                                                            const int32_t *__restrict__ rperm,
+                                                            const float *__restrict__ XC,
+                                                            float *__restrict__ XF) {
+  constexpr int V = 4, W = S / V;
+  const int64_t g = gtid();
+  const int64_t t = g / W;
+  const int o = (int)(g % W) * V;
+  if (t >= nc) return;
+  float a[V], b[V];
+  ldv<V>(XF + (rf0 + t) * S + o, a);
+  ldv<V>(XC + (int64_t)rperm[t] * S + o, b);
+#pragma unroll
+  for (int k = 0; k < V; ++k) a[k] += b[k];
+  stv<V>(XF + (rf0 + t) * S + o, a);
+}
+
+// Y = A X in FP64 (ComputeSPMV_ref.cpp:62-69's order for each column) and,
+// when `dot`, p_k . (A p)_k and alpha_k (CG.cpp:118-120).  H: the FP16
+// off-diagonal values and the FP64 diagonal (exactly the FP64 values when
+// the off-diagonal values are exact in FP16), else the FP64 ELL.  Work-items
+// gtid(), gtid() + stride, ... (stride a multiple of kTile), W per row.
+template <bool STREAM, int S, bool H>
+__global__ __launch_bounds__(kBlock) void k_bspmv_rows(
+    int64_t n, Col16 C, const uint32_t *__restrict__ vc, const uint8_t *__restrict__ dslot,
+    const double *__restrict__ diag, const int32_t *__restrict__ col,
+    const double *__restrict__ val, const double *__restrict__ X, double *__restrict__ Y,
+    bool dot, double *partials, unsigned *ticket, double *sc) {
+  constexpr int V = 2, W = S / V;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  double acc[V];
+#pragma unroll
+  for (int k = 0; k < V; ++k) acc[k] = 0.0;
+  const int o = (int)(threadIdx.x % W) * V;  // stride and blockDim are multiples of W
+  for (int64_t g = gtid(); g < n * W; g += stride) {
+    const int64_t i = g / W;
+    double y[V], xv[V];
+#pragma unroll
+    for (int k = 0; k < V; ++k) y[k] = 0.0;
+    const int t = H ? tile_of(i) : 0;
+    if (H && C.info[t] != kTile32) {
+      const int32_t *__restrict__ base = C.base + t;
+      const int ds = dslot[i];
+      const double di = diag[i];
+#pragma unroll
+      for (int j = 0; j < kSlots; ++j) {
+        const uint32_t w = mload<STREAM>(vc + (int64_t)j * n + i);
+        const int32_t c = base[(int64_t)j * C.ntiles] + (int32_t)(w >> 16);
+        const double a = j == ds ? di : (double)vc_val(w);
+        ldv<V>(X + (int64_t)c * S + o, xv);
+#pragma unroll
+        for (int k = 0; k < V; ++k) y[k] += a * xv[k];
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < kSlots; ++j) {
+        const int64_t e = (int64_t)j * n + i;
+        const double a = mload<STREAM>(val + e);
+        ldv<V>(X + (int64_t)mload<STREAM>(col + e) * S + o, xv);
+#pragma unroll
+        for (int k = 0; k < V; ++k) y[k] += a * xv[k];
+      }
+    }
+    stv<V>(Y + i * S + o, y);
+    if (dot) {
+      ldv<V>(X + i * S + o, xv);
+#pragma unroll
+      for (int k = 0; k < V; ++k) acc[k] += xv[k] * y[k];
+    }
+  }
+  if (!dot) return;
+  // column k = (lane's o) + v: the workgroup's threads holding it, in order
+  __shared__ double red[kBlock][V];
+#pragma unroll
+  for (int k = 0; k < V; ++k) red[threadIdx.x][k] = acc[k];
+  __syncthreads();
+  if (threadIdx.x < (unsigned)S) {
+    const int k = threadIdx.x, sub = k / V, v = k % V;
+    double t = 0.0;
+    for (int q = sub; q < kBlock; q += W) t += red[q][v];
+    partials[(int64_t)blockIdx.x * S + k] = t;
+  }
+  __syncthreads();
+  finish_tail_s(S, partials, ticket, sc, kBAlpha);
 }
 
 // ---- FP64 outer kernels ----------------------------------------------------
@@ -257,10 +508,119 @@ unsigned block_reduce_grid(int64_t total, int s) {
   return (unsigned)std::max<int64_t>(1, g);
 }
 
+// ---- launchers of the W-lanes-per-row kernels (S in {4, 8, 16}) ---------
+bool rows_kernels(int s) { return s == 4 || s == 8 || s == 16; }
+
+template <int S>
+void gs_rows_s(const Problem &P, const Level &L, int64_t b, int64_t e, const float *R, float *X,
+               hipStream_t st) {
+  const Col16 C = col16_of(P, L);
+  const bool c16 = L.vc && use_col16(P, L, b);
+  const unsigned g = grid_for((e - b) * (S / 4));
+  if (L.stream && c16)
+    k_bgs32_rows<true, S, true><<<g, kBlock, 0, st>>>(b, e, L.n, C, L.vc, L.col, L.val16,
+                                                      L.diag32, R, X);
+  else if (L.stream)
+    k_bgs32_rows<true, S, false><<<g, kBlock, 0, st>>>(b, e, L.n, C, L.vc, L.col, L.val16,
+                                                       L.diag32, R, X);
+  else if (c16)
+    k_bgs32_rows<false, S, true><<<g, kBlock, 0, st>>>(b, e, L.n, C, L.vc, L.col, L.val16,
+                                                       L.diag32, R, X);
+  else
+    k_bgs32_rows<false, S, false><<<g, kBlock, 0, st>>>(b, e, L.n, C, L.vc, L.col, L.val16,
+                                                        L.diag32, R, X);
+  HPCGX_LAUNCHED();
+}
+
+void gs_rows(const Problem &P, const Level &L, int s, int64_t b, int64_t e, const float *R,
+             float *X, hipStream_t st) {
+  if (s == 4) gs_rows_s<4>(P, L, b, e, R, X, st);
+  else if (s == 8) gs_rows_s<8>(P, L, b, e, R, X, st);
+  else gs_rows_s<16>(P, L, b, e, R, X, st);
+}
+
+template <int S>
+void restrict_rows_s(const Problem &P, const Level &L, const float *RF, const float *XF,
+                     float *RC, hipStream_t st) {
+  const Col16 C = col16_of(P, L);
+  const bool c16 = L.vc && use_col16(P, L, L.rf0);
+  const unsigned g = grid_for(L.nc * (S / 4));
+  if (L.stream && c16)
+    k_bresid_restrict32_rows<true, S, true><<<g, kBlock, 0, st>>>(
+        L.nc, L.rf0, L.rperm, L.n, C, L.vc, L.col, L.val16, L.diag32, RF, XF, RC);
+  else if (L.stream)
+    k_bresid_restrict32_rows<true, S, false><<<g, kBlock, 0, st>>>(
+        L.nc, L.rf0, L.rperm, L.n, C, L.vc, L.col, L.val16, L.diag32, RF, XF, RC);
+  else if (c16)
+    k_bresid_restrict32_rows<false, S, true><<<g, kBlock, 0, st>>>(
+        L.nc, L.rf0, L.rperm, L.n, C, L.vc, L.col, L.val16, L.diag32, RF, XF, RC);
+  else
+    k_bresid_restrict32_rows<false, S, false><<<g, kBlock, 0, st>>>(
+        L.nc, L.rf0, L.rperm, L.n, C, L.vc, L.col, L.val16, L.diag32, RF, XF, RC);
+  HPCGX_LAUNCHED();
+}
+
+void restrict_rows(const Problem &P, const Level &L, int s, const float *RF, const float *XF,
+                   float *RC, hipStream_t st) {
+  if (s == 4) restrict_rows_s<4>(P, L, RF, XF, RC, st);
+  else if (s == 8) restrict_rows_s<8>(P, L, RF, XF, RC, st);
+  else restrict_rows_s<16>(P, L, RF, XF, RC, st);
+}
+
+void prolong_rows(const Level &L, int s, const float *XC, float *XF, hipStream_t st) {
+  const unsigned g = grid_for(L.nc * (s / 4));
+  if (s == 4)
+    k_bprolong32_rows<4><<<g, kBlock, 0, st>>>(L.nc, L.rf0, L.rperm, XC, XF);
+  else if (s == 8)
+    k_bprolong32_rows<8><<<g, kBlock, 0, st>>>(L.nc, L.rf0, L.rperm, XC, XF);
+  else
+    k_bprolong32_rows<16><<<g, kBlock, 0, st>>>(L.nc, L.rf0, L.rperm, XC, XF);
+  HPCGX_LAUNCHED();
+}
+
+template <int S>
+void spmv_rows_s(const Problem &P, const Level &L, const double *X, double *Y, bool dot,
+                 unsigned g, hipStream_t st) {
+  const Col16 C = col16_of(P, L);
+  const bool h = L.vc && P.half_exact && use_col16(P, L, 0);
+  auto &B = P.blk;
+  if (L.stream && h)
+    k_bspmv_rows<true, S, true><<<g, kBlock, 0, st>>>(L.n, C, L.vc, L.dslot, L.diag, L.col,
+                                                      L.val, X, Y, dot, B.partials, B.ticket,
+                                                      B.scalars);
+  else if (L.stream)
+    k_bspmv_rows<true, S, false><<<g, kBlock, 0, st>>>(L.n, C, L.vc, L.dslot, L.diag, L.col,
+                                                       L.val, X, Y, dot, B.partials, B.ticket,
+                                                       B.scalars);
+  else if (h)
+    k_bspmv_rows<false, S, true><<<g, kBlock, 0, st>>>(L.n, C, L.vc, L.dslot, L.diag, L.col,
+                                                       L.val, X, Y, dot, B.partials, B.ticket,
+                                                       B.scalars);
+  else
+    k_bspmv_rows<false, S, false><<<g, kBlock, 0, st>>>(L.n, C, L.vc, L.dslot, L.diag, L.col,
+                                                        L.val, X, Y, dot, B.partials, B.ticket,
+                                                        B.scalars);
+  HPCGX_LAUNCHED();
+}
+
+void spmv_rows(const Problem &P, const Level &L, int s, const double *X, double *Y, bool dot,
+               unsigned g, hipStream_t st) {
+  if (s == 4) spmv_rows_s<4>(P, L, X, Y, dot, g, st);
+  else if (s == 8) spmv_rows_s<8>(P, L, X, Y, dot, g, st);
+  else spmv_rows_s<16>(P, L, X, Y, dot, g, st);
+}
+
 void bsymgs32(Problem &P, int l, int ordering, int s, const float *R, float *X,
               hipStream_t st) {
   const Level &L = P.L[l];
-  if (ordering == HPCGX_ORDER_MULTICOLOR) {
+  if (ordering == HPCGX_ORDER_MULTICOLOR && rows_kernels(s)) {
+    for (int pass = 0; pass < 2; ++pass)
+      for (int q = 0; q < L.ncolors; ++q) {
+        const int c = pass == 0 ? q : L.ncolors - 1 - q;
+        const int64_t b = L.color_ptr[c], e = L.color_ptr[c + 1];
+        if (e > b) gs_rows(P, L, s, b, e, R, X, st);
+      }
+  } else if (ordering == HPCGX_ORDER_MULTICOLOR) {
     for (int pass = 0; pass < 2; ++pass)
       for (int q = 0; q < L.ncolors; ++q) {
         const int c = pass == 0 ? q : L.ncolors - 1 - q;
@@ -315,7 +675,7 @@ void prepare_block(Problem &P, int s) {
     B.rc32[l] = P.arena.alloc<float>((size_t)P.L[l].nc * s);
     B.xc32[l] = P.arena.alloc<float>((size_t)P.L[l].nc * s);
   }
-  B.partials = P.arena.alloc<double>((size_t)kReduceBlocks * s);
+  B.partials = P.arena.alloc<double>((size_t)kMaxPartials * s);
   B.ticket = P.arena.alloc<unsigned>(1);
   HPCGX_HIP(hipMemset(B.ticket, 0, sizeof(unsigned)));
   B.scalars = P.arena.alloc<double>((size_t)5 * s);
@@ -335,7 +695,7 @@ void Problem::BlockState::release(Problem &P) {
     P.arena.release(rc32[l], sizeof(float) * P.L[l].nc * s);
     P.arena.release(xc32[l], sizeof(float) * P.L[l].nc * s);
   }
-  P.arena.release(partials, sizeof(double) * kReduceBlocks * s);
+  P.arena.release(partials, sizeof(double) * kMaxPartials * s);
   P.arena.release(ticket, sizeof(unsigned));
   P.arena.release(scalars, sizeof(double) * 5 * s);
   s = 0;
@@ -350,23 +710,36 @@ void mg32_block(Problem &P, int l, int ordering, int s, const float *R, float *X
   bsymgs32(P, l, ordering, s, R, X, st);
   if (l + 1 < P.nlevels) {
     float *rc = P.blk.rc32[l], *xc = P.blk.xc32[l];
+    const bool rows = rows_kernels(s) && L.rperm;
     const unsigned g = grid_for(L.nc * s);
-    if (L.stream)
-      k_bresid_restrict32<true><<<g, kBlock, 0, st>>>(L.nc, L.n, s, L.f2c, L.col, L.val16,
-                                                      L.diag32, R, X, rc);
-    else
-      k_bresid_restrict32<false><<<g, kBlock, 0, st>>>(L.nc, L.n, s, L.f2c, L.col, L.val16,
-                                                       L.diag32, R, X, rc);
-    HPCGX_LAUNCHED();
+    if (rows) {
+      restrict_rows(P, L, s, R, X, rc, st);
+    } else {
+      if (L.stream)
+        k_bresid_restrict32<true><<<g, kBlock, 0, st>>>(L.nc, L.n, s, L.f2c, L.col, L.val16,
+                                                        L.diag32, R, X, rc);
+      else
+        k_bresid_restrict32<false><<<g, kBlock, 0, st>>>(L.nc, L.n, s, L.f2c, L.col, L.val16,
+                                                         L.diag32, R, X, rc);
+      HPCGX_LAUNCHED();
+    }
     mg32_block(P, l + 1, ordering, s, rc, xc, st);
-    k_bprolong32<<<g, kBlock, 0, st>>>(L.nc, s, L.f2c, xc, X);
-    HPCGX_LAUNCHED();
+    if (rows) {
+      prolong_rows(L, s, xc, X, st);
+    } else {
+      k_bprolong32<<<g, kBlock, 0, st>>>(L.nc, s, L.f2c, xc, X);
+      HPCGX_LAUNCHED();
+    }
     bsymgs32(P, l, ordering, s, R, X, st);
   }
 }
 
 void spmv_block(Problem &P, int l, int s, const double *X, double *Y, hipStream_t st) {
   const Level &L = P.L[l];
+  if (rows_kernels(s)) {
+    spmv_rows(P, L, s, X, Y, false, grid_for(L.n * (s / 2)), st);
+    return;
+  }
   const unsigned g = grid_for(L.n * s);
   if (L.stream)
     k_bspmv_dot<true><<<g, kBlock, 0, st>>>(L.n, s, L.col, L.val, X, Y, false, nullptr,
@@ -389,7 +762,15 @@ void cg_block_mixed(Problem &P, int ordering, int s, const double *Bv, double *X
   const Level &A = P.L[0];
   const int64_t total = A.n * s;
   const unsigned gr = block_reduce_grid(total, s);
+  // W-lanes-per-row SpMV: a persistent grid when it reduces, one work-item
+  // per (row, lane) otherwise
+  const unsigned grow = (unsigned)std::max<int64_t>(
+      1, std::min<int64_t>((A.n * (s / 2) + kBlock - 1) / kBlock, kMaxPartials));
   auto spmv_dot = [&](const double *x, double *y, bool dot) {
+    if (rows_kernels(s)) {
+      spmv_rows(P, A, s, x, y, dot, dot ? grow : grid_for(A.n * (s / 2)), st);
+      return;
+    }
     const unsigned g = dot ? gr : grid_for(total);
     if (A.stream)
       k_bspmv_dot<true><<<g, kBlock, 0, st>>>(A.n, s, A.col, A.val, x, y, dot, B.partials,

@@ -184,6 +184,7 @@ struct Problem {
   bool use_c8 = true;         // ... 8-bit ones where a tile has them
   int spmv_dot_blocks = 4096; // persistent grid of the fused SpMV + dot
   bool mixed_ready = false;   // FP16/FP32 copies built
+  bool half_exact = false;    // every off-diagonal value is exact in FP16
   float *r32 = nullptr, *z32 = nullptr;  // FP32 input / output of the V-cycle
   // block PCG over s right-hand sides (block.hip), allocated on first use:
   // [n][s] multivectors of the fine level, FP32 coarse multivectors,

@@ -913,7 +913,7 @@ void launch_spmv_dot(Problem &P, const Level &L, const double *p, double *Ap,
 
 void launch_spmv_dot_half(Problem &P, const Level &L, const double *p, double *Ap,
                           hipStream_t s) {
-  if (!L.vc || !use_col16(P, L, 0)) {
+  if (!L.vc || !P.half_exact || !use_col16(P, L, 0)) {
     launch_spmv_dot(P, L, p, Ap, s);
     return;
   }

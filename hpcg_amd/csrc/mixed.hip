@@ -42,22 +42,28 @@ __device__ inline T mload(const T *p) {
   return *p;
 }
 
+// FP16 copy of the off-diagonal values (diagonal slot zeroed) and the FP32
+// diagonal.  flags |= 1: an off-diagonal value overflows FP16; flags |= 2: an
+// off-diagonal value is not exact in FP16 (the FP16-read outer SpMV then
+// falls back to the FP64 values).
 __global__ __launch_bounds__(kBlock) void k_make_half(
     int64_t n, const double *__restrict__ val, const uint8_t *__restrict__ dslot,
-    half_t *__restrict__ val16, float *__restrict__ diag32, int *overflow) {
+    half_t *__restrict__ val16, float *__restrict__ diag32, int *flags) {
   const int64_t i = gtid();
   if (i >= n) return;
   const int ds = dslot[i];
-  bool bad = false;
+  bool bad = false, inexact = false;
   for (int j = 0; j < kSlots; ++j) {
     const int64_t k = (int64_t)j * n + i;
-    const float f = (float)val[k];
+    const double v = val[k];
+    const float f = (float)v;
     const half_t h = (j == ds) ? (half_t)0.0f : (half_t)f;
     bad |= (j != ds) && __builtin_isinf((float)h) && !__builtin_isinf(f);
+    inexact |= (j != ds) && (double)(float)h != v;
     val16[k] = h;
   }
   diag32[i] = (float)val[(int64_t)ds * n + i];
-  if (bad) atomicOr(overflow, 1);
+  if (bad || inexact) atomicOr(flags, (bad ? 1 : 0) | (inexact ? 2 : 0));
 }
 
 __global__ __launch_bounds__(kBlock) void k_diag32(int64_t n, const double *d,
@@ -336,7 +342,8 @@ void prepare_mixed(Problem &P) {
   P.z32 = P.arena.alloc<float>(P.L[0].n);
   int bad = 0;
   HPCGX_HIP(hipMemcpy(&bad, overflow.p, sizeof bad, hipMemcpyDeviceToHost));
-  if (bad) throw Error(HPCGX_EUNSUP, "an off-diagonal value overflows FP16");
+  if (bad & 1) throw Error(HPCGX_EUNSUP, "an off-diagonal value overflows FP16");
+  P.half_exact = (bad & 2) == 0;
   P.mixed_ready = true;
 }
 
