@@ -198,6 +198,49 @@ def test_cg_mixed_reaches_tolerance(hp):
     assert np.max(np.abs(P.to_numpy(x) - P.to_numpy(x64))) < 1e-7
 
 
+def test_inexact_fp16_off_diagonals(hp):
+    """Off-diagonal values FP16 cannot hold (-1 - k/7000, symmetric): the
+    V-cycle uses their FP16 roundings as designed, but the outer A p must use
+    the FP64 values, as the oracle's mixed CG does; so must the block path's
+    FP64 SpMV."""
+    import torch
+    dims = (16, 16, 24)
+    say("inexact fp16 off-diagonals", dims)
+    h = O.Hierarchy(*dims)
+    levels = []
+    for l in range(4):
+        m = h.level(l)  # live views: the oracle sees the new values too
+        n = h.nrow(l)
+        cols = m["cols"].reshape(n, 27)
+        vals = m["vals"].reshape(n, 27)
+        rows = np.repeat(np.arange(n), 27).reshape(n, 27)
+        off = (cols != rows) & (np.arange(27)[None, :] < m["nnz_row"][:, None])
+        vals[off] = -1.0 - ((rows[off] + cols[off]) % 7 + 1) / 7000.0
+        assert not np.all(vals[off].astype(np.float16).astype(np.float64) == vals[off])
+        d = dict(nnz_row=m["nnz_row"], cols=m["cols"], vals=m["vals"],
+                 diag_slot=m["diag_slot"])
+        if l < 3:
+            d["f2c"] = m["f2c"]
+        levels.append(d)
+    h.use_color_order()
+    P = hp.Problem.from_reference_levels(levels)
+    P.prepare_mixed()
+    want = h.cg_mixed(max_iter=10, tol=0.0, recompute=10)["traj"]
+    b, x = P.to_device_order(h.b), P.zeros()
+    for k in (1, 4, 10):
+        x.zero_()
+        r = P.cg_mixed(b, x, max_iter=k, tolerance=0.0, recompute=10)
+        assert r["normr"] / r["normr0"] == pytest.approx(want[k], rel=1e-9), k
+    n, s = P.nrow(0), 8
+    Xn = np.random.default_rng(5).uniform(-1, 1, (n, s))
+    Y = P.to_device_order_multi(Xn) * 0
+    P.spmv_block(P.to_device_order_multi(Xn), Y)
+    got = P.to_numpy_multi(Y)
+    for k in range(s):
+        assert np.array_equal(got[:, k], O.spmv(h.level(0), Xn[:, k])), k
+    del P
+
+
 def test_replace_diagonal_reaches_fp32_copy(hp):
     say("diag32")
     P = hp.Problem.generate(8, 8, 8, nlevels=1)
