@@ -169,11 +169,11 @@ __device__ inline void slots32(const Col16 &C, const uint32_t *__restrict__ vc,
                                const half_t *__restrict__ val16, int64_t n, int64_t i, F &&f) {
   const int t = C16 ? tile_of(i) : 0;
   if (C16 && C.info[t] != kTile32) {
-    const int32_t *__restrict__ base = C.base + t;
+    const int32_t *__restrict__ base = C.tbase + (int64_t)t * kBaseStride;
 #pragma unroll
     for (int j = 0; j < kSlots; ++j) {
       const uint32_t w = mload<STREAM>(vc + (int64_t)j * n + i);
-      f((float)vc_val(w), base[(int64_t)j * C.ntiles] + (int32_t)(w >> 16));
+      f((float)vc_val(w), base[j] + (int32_t)(w >> 16));
     }
   } else {
 #pragma unroll
@@ -287,13 +287,13 @@ __global__ __launch_bounds__(kBlock) void k_bspmv_rows(
     for (int k = 0; k < V; ++k) y[k] = 0.0;
     const int t = H ? tile_of(i) : 0;
     if (H && C.info[t] != kTile32) {
-      const int32_t *__restrict__ base = C.base + t;
+      const int32_t *__restrict__ base = C.tbase + (int64_t)t * kBaseStride;
       const int ds = dslot[i];
       const double di = diag[i];
 #pragma unroll
       for (int j = 0; j < kSlots; ++j) {
         const uint32_t w = mload<STREAM>(vc + (int64_t)j * n + i);
-        const int32_t c = base[(int64_t)j * C.ntiles] + (int32_t)(w >> 16);
+        const int32_t c = base[j] + (int32_t)(w >> 16);
         const double a = j == ds ? di : (double)vc_val(w);
         ldv<V>(X + (int64_t)c * S + o, xv);
 #pragma unroll

@@ -403,6 +403,10 @@ int hpcgx_tune(hpcgx_problem *p, const char *key, double value) {
       P.use_c8 = value != 0.0;
       return;
     }
+    if (k == "fp16_pairs") {
+      P.use_pairs = value != 0.0;
+      return;
+    }
     if (k == "fleet_blocks") {
       if (value < 1 || value > 65536) throw Error(HPCGX_EINVAL, "fleet_blocks must be in [1, 65536]");
       P.fleet.grid = (int)value;

@@ -28,6 +28,8 @@ constexpr int kWaveSlabFactor = 2;         // slab width / matrix bandwidth
 constexpr int kWaveRows = 2;               // rows per thread of a wavefront task
 constexpr int kTile = 64;  // rows sharing narrow column bases (one wavefront)
 constexpr int kSlotWords = (kSlots + 3) / 4;  // 8-bit offsets of 4 slots per word
+constexpr int kPairWords = (kSlots + 1) / 2;  // FP16 values of 2 slots per word
+constexpr int kBaseStride = 32;  // ints per tile in Level::tbase (128 bytes)
 enum TileClass { kTile8 = 0, kTile32 = 1, kTile16 = 2 };
 constexpr int kBandLines = 1;  // grid lines per band of the in-colour order (1 = natural)
 
@@ -129,6 +131,7 @@ struct Level {
   // zeroed, the FP32 diagonal, FP32 coarse-level vectors
   _Float16 *val16 = nullptr;
   uint32_t *vc = nullptr;  // c16 levels: col16 << 16 | FP16 bits, one word per slot
+  uint32_t *v16p = nullptr;  // c16 levels: FP16 bits of slots 2w, 2w+1 in word w ([kPairWords][n])
   float *diag32 = nullptr;
   float *rc32 = nullptr, *xc32 = nullptr;
   // Narrow column ids.  Tile t = device rows [kTile t, kTile t + kTile).  The
@@ -140,6 +143,10 @@ struct Level {
   //   tinfo[t] == kTile8:  cbase[j * ntiles + t] + byte (j % 4) of
   //                        off8[(j / 4) * n + i]   (every slot spans <= 255)
   //   tinfo[t] == kTile16: cbase[j * ntiles + t] + col16[j * n + i]
+  // tbase[t * kBaseStride + j] holds the same bases tile by tile.  The FP64
+  // kernels read cbase, whose lines serve 16 neighbouring tiles; the FP32
+  // kernels read tbase, where a tile's bases come in a few wide scalar loads
+  // (measured: each is the faster layout for its own kernels).
   //   tinfo[t] == kTile32: col[j * n + i] (more than 27 distinct offsets, or
   //                        a slot spanning more than 65535 columns)
   // `col` holds the 32-bit ids of every tile, col16 those of every 8- and
@@ -151,6 +158,7 @@ struct Level {
   uint16_t *col16 = nullptr;
   uint32_t *off8 = nullptr;  // [kSlotWords * n]
   int32_t *cbase = nullptr;
+  int32_t *tbase = nullptr;
   int32_t *tinfo = nullptr;
   // transfer to the next level: f2c[coarse device row] = fine device row
   int64_t nc = 0;
@@ -182,6 +190,7 @@ struct Problem {
   bool use_wave = false;      // multicolour sweeps as wavefronts (measured slower)
   bool use_c16 = true;        // read narrow column ids where a level has them
   bool use_c8 = true;         // ... 8-bit ones where a tile has them
+  bool use_pairs = true;      // FP32 colour steps of 8-bit tiles read FP16 value pairs
   int spmv_dot_blocks = 4096; // persistent grid of the fused SpMV + dot
   bool mixed_ready = false;   // FP16/FP32 copies built
   bool half_exact = false;    // every off-diagonal value is exact in FP16
@@ -259,13 +268,14 @@ struct Problem {
 struct Col16 {
   const uint16_t *off;
   const uint32_t *off8;
-  const int32_t *base;
+  const int32_t *base;   // slot-major bases (Level::cbase)
+  const int32_t *tbase;  // tile-major bases (Level::tbase)
   const int32_t *info;
   int64_t ntiles;
   int use8;  // read 8-bit offsets in kTile8 tiles (else their 16-bit ids)
 };
 inline Col16 col16_of(const Problem &P, const Level &L) {
-  return Col16{L.col16, L.off8, L.cbase, L.tinfo, L.ntiles, P.use_c8 ? 1 : 0};
+  return Col16{L.col16, L.off8, L.cbase, L.tbase, L.tinfo, L.ntiles, P.use_c8 ? 1 : 0};
 }
 // 16-bit kernels apply to rows [begin, ...) of L when this holds.
 inline bool use_col16(const Problem &P, const Level &L, int64_t begin) {

@@ -145,12 +145,12 @@ __device__ inline void gs32_row16(const Col16 &C, const uint32_t *__restrict__ v
     gs32_row<STREAM>(i, n, col, val16, diag32, r, x);
     return;
   }
-  const int32_t *__restrict__ base = C.base + t;
+  const int32_t *__restrict__ base = C.tbase + (int64_t)t * kBaseStride;
   float sum = r[i];
 #pragma unroll
   for (int j = 0; j < kSlots; ++j) {
     const uint32_t w = mload<STREAM>(vc + (int64_t)j * n + i);
-    const int32_t c = base[(int64_t)j * C.ntiles] + (int32_t)(w >> 16);
+    const int32_t c = base[j] + (int32_t)(w >> 16);
     sum -= (float)vc_val(w) * x[c];
   }
   x[i] = sum / diag32[i];
@@ -165,16 +165,73 @@ __device__ inline float row_dot32_16(const Col16 &C, const uint32_t *__restrict_
   const int t = tile_of(i);
   const int info = C.info[t];
   if (info == kTile32) return row_dot32<STREAM>(i, n, col, val16, diag32, x);
-  const int32_t *__restrict__ base = C.base + t;
+  const int32_t *__restrict__ base = C.tbase + (int64_t)t * kBaseStride;
   float s = 0.0f;
 #pragma unroll
   for (int j = 0; j < kSlots; ++j) {
     const uint32_t w = mload<STREAM>(vc + (int64_t)j * n + i);
-    const int32_t c = base[(int64_t)j * C.ntiles] + (int32_t)(w >> 16);
+    const int32_t c = base[j] + (int32_t)(w >> 16);
     s += (float)vc_val(w) * x[c];
   }
   s += diag32[i] * x[i];
   return s;
+}
+
+// FP32 colour step of an 8-bit tile (Level::off8, Level::v16p): per row
+// kSlotWords offset words and kPairWords value words, 3 bytes per slot, in
+// gs32_row16's order of terms.  Other tiles take gs32_row16.
+template <bool STREAM>
+__device__ inline void gs32_row8(const Col16 &C, const uint32_t *__restrict__ vp,
+                                 const uint32_t *__restrict__ vc, int64_t i, int64_t n,
+                                 const int32_t *__restrict__ col,
+                                 const half_t *__restrict__ val16,
+                                 const float *__restrict__ diag32,
+                                 const float *__restrict__ r, float *x) {
+  const int t = tile_of(i);
+  const int info = C.info[t];
+  if (info != kTile8 || !C.use8) {
+    gs32_row16<STREAM>(C, vc, i, n, col, val16, diag32, r, x);
+    return;
+  }
+  const int32_t *__restrict__ base = C.tbase + (int64_t)t * kBaseStride;
+  uint32_t o8[kSlotWords], pv[kPairWords];
+#pragma unroll
+  for (int g = 0; g < kSlotWords; ++g) o8[g] = mload<STREAM>(C.off8 + (int64_t)g * n + i);
+#pragma unroll
+  for (int w = 0; w < kPairWords; ++w) pv[w] = mload<STREAM>(vp + (int64_t)w * n + i);
+  float sum = r[i];
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int32_t c = base[j] + (int32_t)((o8[j / 4] >> (8 * (j % 4))) & 0xffu);
+    const half_t a = __builtin_bit_cast(half_t, (uint16_t)(pv[j / 2] >> (16 * (j % 2))));
+    sum -= (float)a * x[c];
+  }
+  x[i] = sum / diag32[i];
+}
+
+// v16p[w*n + i] = bits of val16[(2w)*n + i] | bits of val16[(2w+1)*n + i] << 16
+// (0 above the last slot), one thread per row.
+__global__ __launch_bounds__(kBlock) void k_pack_v16p(int64_t n, const half_t *__restrict__ val16,
+                                                      uint32_t *__restrict__ vp) {
+  const int64_t i = gtid();
+  if (i >= n) return;
+#pragma unroll
+  for (int w = 0; w < kPairWords; ++w) {
+    const uint32_t lo = __builtin_bit_cast(uint16_t, val16[(int64_t)(2 * w) * n + i]);
+    const uint32_t hi =
+        2 * w + 1 < kSlots ? __builtin_bit_cast(uint16_t, val16[(int64_t)(2 * w + 1) * n + i]) : 0u;
+    vp[(int64_t)w * n + i] = lo | (hi << 16);
+  }
+}
+
+template <bool STREAM, bool FINEST>
+__global__ __launch_bounds__(kBlock) void k_gs32_range8(
+    int64_t begin, int64_t end, int64_t n, Col16 C, const uint32_t *__restrict__ vp,
+    const uint32_t *__restrict__ vc, const int32_t *__restrict__ col,
+    const half_t *__restrict__ val16, const float *__restrict__ diag32,
+    const float *__restrict__ r, float *x) {
+  const int64_t i = begin + gtid();
+  if (i < end) gs32_row8<STREAM>(C, vp, vc, i, n, col, val16, diag32, r, x);
 }
 
 // vc[k] = col16[k] << 16 | bits of val16[k], one thread per row (a launch
@@ -288,6 +345,20 @@ void gs32_step(const Problem &P, const Level &L, int64_t begin, int64_t end,
                const float *r, float *x, hipStream_t s) {
   if (end <= begin) return;
   const unsigned g = grid_for(end - begin);
+  if (use_col16(P, L, begin) && L.v16p && P.use_pairs && P.use_c8 && L.byte_tiles > 0) {
+    const Col16 C = col16_of(P, L);
+    if (L.finest)
+      k_gs32_range8<true, true><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.v16p, L.vc, L.col,
+                                                     L.val16, L.diag32, r, x);
+    else if (L.stream)
+      k_gs32_range8<true, false><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.v16p, L.vc, L.col,
+                                                      L.val16, L.diag32, r, x);
+    else
+      k_gs32_range8<false, false><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.v16p, L.vc, L.col,
+                                                       L.val16, L.diag32, r, x);
+    HPCGX_LAUNCHED();
+    return;
+  }
   if (use_col16(P, L, begin)) {
     const Col16 C = col16_of(P, L);
     if (L.finest)
@@ -332,6 +403,11 @@ void prepare_mixed(Problem &P) {
       L.vc = P.arena.alloc<uint32_t>((size_t)m);
       k_pack_vc<<<grid_for(L.n), kBlock>>>(L.n, L.col16, L.val16, L.vc);
       HPCGX_LAUNCHED();
+      if (L.byte_tiles > 0) {
+        L.v16p = P.arena.alloc<uint32_t>((size_t)kPairWords * L.n);
+        k_pack_v16p<<<grid_for(L.n), kBlock>>>(L.n, L.val16, L.v16p);
+        HPCGX_LAUNCHED();
+      }
     }
     if (L.nc) {
       L.rc32 = P.arena.alloc<float>(L.nc);

@@ -267,7 +267,7 @@ __device__ inline int wave_max_i32(int v) {
 __global__ __launch_bounds__(kBlock) void k_tile_layout(
     int64_t n, int64_t ntiles, const int32_t *__restrict__ iperm, int32_t *col,
     double *val, const uint8_t *__restrict__ nnz_row, uint8_t *dslot,
-    uint16_t *col16, uint32_t *off8, int32_t *cbase, int32_t *tinfo) {
+    uint16_t *col16, uint32_t *off8, int32_t *cbase, int32_t *tbase, int32_t *tinfo) {
   const int64_t t = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
   if (t >= ntiles) return;  // whole wavefronts
   const int lane = threadIdx.x & 63;
@@ -326,7 +326,10 @@ __global__ __launch_bounds__(kBlock) void k_tile_layout(
       o8[u / 4] |= ((uint32_t)(cc - base) & 0xffu) << (8 * (u % 4));
       if (has && c == i) ds = u;
     }
-    if (lane == 0) cbase[(int64_t)u * ntiles + t] = base;
+    if (lane == 0) {
+      cbase[(int64_t)u * ntiles + t] = base;
+      tbase[t * kBaseStride + u] = base;
+    }
   }
   if (row) {
     dslot[i] = (uint8_t)ds;
@@ -681,11 +684,12 @@ void finalize_level(Problem &P, int l, NaturalLevel &N, int flags,
     L.ntiles = T;
     L.col16 = P.arena.alloc<uint16_t>((size_t)kSlots * n);
     L.cbase = P.arena.alloc<int32_t>((size_t)kSlots * T);
+    L.tbase = P.arena.alloc<int32_t>((size_t)kBaseStride * T);
     L.tinfo = P.arena.alloc<int32_t>(T);
     L.off8 = P.arena.alloc<uint32_t>((size_t)kSlotWords * n);
     const int64_t wpb = kBlock / 64;
     k_tile_layout<<<(unsigned)((T + wpb - 1) / wpb), kBlock, 0, s>>>(
-        n, T, L.iperm, L.col, L.val, L.nnz_row, L.dslot, L.col16, L.off8, L.cbase, L.tinfo);
+        n, T, L.iperm, L.col, L.val, L.nnz_row, L.dslot, L.col16, L.off8, L.cbase, L.tbase, L.tinfo);
     HPCGX_LAUNCHED();
     std::vector<int32_t> info(T);
     HPCGX_HIP(hipMemcpyAsync(info.data(), L.tinfo, sizeof(int32_t) * T,

@@ -94,16 +94,18 @@ def test_symgs32_bitexact(hp, dims, ordering):
 
 
 @pytest.mark.parametrize("dims", [(32, 48, 64), (64, 64, 64), (8, 8, 24)])
-@pytest.mark.parametrize("c16,c8", [(1, 1), (1, 0), (0, 1)])
-def test_symgs32_col16(hp, dims, c16, c8):
-    """Multicolour FP32 sweeps and FP32 SpMV with 8-bit column offsets (the
-    default), the packed 16-bit column / FP16 value words, and 32-bit ids:
-    the oracle's bits on every level, including colour blocks whose length
-    is not a multiple of 128 rows."""
-    say("symgs32 col16", dims, c16, c8)
+@pytest.mark.parametrize("c16,c8,pairs", [(1, 1, 1), (1, 1, 0), (1, 0, 1), (0, 1, 1)])
+def test_symgs32_col16(hp, dims, c16, c8, pairs):
+    """Multicolour FP32 sweeps and FP32 SpMV with 8-bit column offsets and
+    FP16 value pairs (the default), 8-bit offsets with the packed words, the
+    packed 16-bit column / FP16 value words, and 32-bit ids: the oracle's
+    bits on every level, including colour blocks whose length is not a
+    multiple of 128 rows."""
+    say("symgs32 col16", dims, c16, c8, pairs)
     P = hp.Problem.generate(*dims, nlevels=4)
     P.tune("col16", c16)
     P.tune("col8", c8)
+    P.tune("fp16_pairs", pairs)
     h = hier(dims, False)
     for l in range(4):
         m = h.level(l)
@@ -111,10 +113,10 @@ def test_symgs32_col16(hp, dims, c16, c8):
         want = O.symgs32(m, r, x0, O.color_order(O.colors(m)[0]))
         x = dev32(P, x0, l)
         P.symgs32(dev32(P, r, l), x, level=l, ordering="multicolor")
-        assert np.array_equal(host32(P, x, l), want), (c16, c8, l)
+        assert np.array_equal(host32(P, x, l), want), (c16, c8, pairs, l)
         y = P.zeros32(l)
         P.spmv32(dev32(P, x0, l), y, level=l)
-        assert np.array_equal(host32(P, y, l), O.spmv32(m, x0)), (c16, c8, l)
+        assert np.array_equal(host32(P, y, l), O.spmv32(m, x0)), (c16, c8, pairs, l)
     del P
 
 
