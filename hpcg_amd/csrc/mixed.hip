@@ -209,6 +209,35 @@ __device__ inline void gs32_row8(const Col16 &C, const uint32_t *__restrict__ vp
   x[i] = sum / diag32[i];
 }
 
+// row_dot32_16 on an 8-bit tile from the FP16 value pairs (gs32_row8's
+// reads); other tiles take row_dot32_16.
+template <bool STREAM>
+__device__ inline float row_dot32_8(const Col16 &C, const uint32_t *__restrict__ vp,
+                                    const uint32_t *__restrict__ vc, int64_t i, int64_t n,
+                                    const int32_t *__restrict__ col,
+                                    const half_t *__restrict__ val16,
+                                    const float *__restrict__ diag32,
+                                    const float *__restrict__ x) {
+  const int t = tile_of(i);
+  const int info = C.info[t];
+  if (info != kTile8 || !C.use8) return row_dot32_16<STREAM>(C, vc, i, n, col, val16, diag32, x);
+  const int32_t *__restrict__ base = C.tbase + (int64_t)t * kBaseStride;
+  uint32_t o8[kSlotWords], pv[kPairWords];
+#pragma unroll
+  for (int g = 0; g < kSlotWords; ++g) o8[g] = mload<STREAM>(C.off8 + (int64_t)g * n + i);
+#pragma unroll
+  for (int w = 0; w < kPairWords; ++w) pv[w] = mload<STREAM>(vp + (int64_t)w * n + i);
+  float s = 0.0f;
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int32_t c = base[j] + (int32_t)((o8[j / 4] >> (8 * (j % 4))) & 0xffu);
+    const half_t a = __builtin_bit_cast(half_t, (uint16_t)(pv[j / 2] >> (16 * (j % 2))));
+    s += (float)a * x[c];
+  }
+  s += diag32[i] * x[i];
+  return s;
+}
+
 // v16p[w*n + i] = bits of val16[(2w)*n + i] | bits of val16[(2w+1)*n + i] << 16
 // (0 above the last slot), one thread per row.
 __global__ __launch_bounds__(kBlock) void k_pack_v16p(int64_t n, const half_t *__restrict__ val16,
@@ -319,17 +348,20 @@ __global__ __launch_bounds__(kBlock) void k_prolong32(
 
 // The transfers in fine order (Level::rperm): thread t owns fine row rf0 + t
 // (whole tiles per wavefront, packed words when C16) and coarse row rperm[t].
-template <bool STREAM, bool C16>
+// C16 = 2: FP16 value pairs on 8-bit tiles (row_dot32_8).
+template <bool STREAM, int C16>
 __global__ __launch_bounds__(kBlock) void k_resid_restrict32_f(
     int64_t nc, int64_t rf0, const int32_t *__restrict__ rperm, int64_t n, Col16 C,
-    const uint32_t *__restrict__ vc, const int32_t *__restrict__ col,
-    const half_t *__restrict__ val16, const float *__restrict__ diag32,
-    const float *__restrict__ rf, const float *__restrict__ xf, float *__restrict__ rc) {
+    const uint32_t *__restrict__ vc, const uint32_t *__restrict__ vp,
+    const int32_t *__restrict__ col, const half_t *__restrict__ val16,
+    const float *__restrict__ diag32, const float *__restrict__ rf,
+    const float *__restrict__ xf, float *__restrict__ rc) {
   const int64_t t = gtid();
   if (t >= nc) return;
   const int64_t f = rf0 + t;
-  const float ax = C16 ? row_dot32_16<STREAM>(C, vc, f, n, col, val16, diag32, xf)
-                       : row_dot32<STREAM>(f, n, col, val16, diag32, xf);
+  const float ax = C16 == 2   ? row_dot32_8<STREAM>(C, vp, vc, f, n, col, val16, diag32, xf)
+                   : C16 == 1 ? row_dot32_16<STREAM>(C, vc, f, n, col, val16, diag32, xf)
+                              : row_dot32<STREAM>(f, n, col, val16, diag32, xf);
   rc[rperm[t]] = rf[f] - ax;
 }
 
@@ -483,18 +515,17 @@ void mg32(Problem &P, int l, int ordering, const float *r, float *x,
     if (L.rperm) {
       const Col16 C = col16_of(P, L);
       const bool c16 = L.vc && use_col16(P, L, L.rf0);
-      if (L.stream && c16)
-        k_resid_restrict32_f<true, true><<<g, kBlock, 0, s>>>(
-            L.nc, L.rf0, L.rperm, L.n, C, L.vc, L.col, L.val16, L.diag32, r, x, L.rc32);
-      else if (L.stream)
-        k_resid_restrict32_f<true, false><<<g, kBlock, 0, s>>>(
-            L.nc, L.rf0, L.rperm, L.n, C, L.vc, L.col, L.val16, L.diag32, r, x, L.rc32);
-      else if (c16)
-        k_resid_restrict32_f<false, true><<<g, kBlock, 0, s>>>(
-            L.nc, L.rf0, L.rperm, L.n, C, L.vc, L.col, L.val16, L.diag32, r, x, L.rc32);
-      else
-        k_resid_restrict32_f<false, false><<<g, kBlock, 0, s>>>(
-            L.nc, L.rf0, L.rperm, L.n, C, L.vc, L.col, L.val16, L.diag32, r, x, L.rc32);
+      const bool pairs = c16 && L.v16p && P.use_pairs && P.use_c8;
+#define HPCGX_RR32(ST, K)                                                                  \
+  k_resid_restrict32_f<ST, K><<<g, kBlock, 0, s>>>(L.nc, L.rf0, L.rperm, L.n, C, L.vc, L.v16p, \
+                                                   L.col, L.val16, L.diag32, r, x, L.rc32)
+      if (L.stream && pairs) HPCGX_RR32(true, 2);
+      else if (L.stream && c16) HPCGX_RR32(true, 1);
+      else if (L.stream) HPCGX_RR32(true, 0);
+      else if (pairs) HPCGX_RR32(false, 2);
+      else if (c16) HPCGX_RR32(false, 1);
+      else HPCGX_RR32(false, 0);
+#undef HPCGX_RR32
     } else if (L.stream) {
       k_resid_restrict32<true><<<g, kBlock, 0, s>>>(L.nc, L.n, L.f2c, L.col, L.val16,
                                                     L.diag32, r, x, L.rc32);

@@ -183,6 +183,28 @@ def test_cg_mixed_trajectory(hp):
         assert r["normr"] / r["normr0"] == pytest.approx(want[k], rel=tol), k
 
 
+@pytest.mark.parametrize("dims", [(32, 48, 64), (64, 64, 128)])
+def test_cg_mixed_same_bits_every_format(hp, dims):
+    """The mixed CG reads its matrix as FP16 value pairs with 8-bit offsets,
+    as packed value + 16-bit column words, or as FP16 values with 32-bit ids:
+    the same numbers in the same order, so the same bits."""
+    import torch
+    say("cg mixed formats", dims)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    b, x, _ = P.vectors()
+    got = []
+    for c16, pairs in ((1, 1), (1, 0), (0, 1)):
+        P.tune("col16", c16)
+        P.tune("fp16_pairs", pairs)
+        x.zero_()
+        r = P.cg_mixed(b, x, max_iter=12, tolerance=0.0, recompute=5)
+        got.append((r["normr"], x.clone()))
+    for k in (1, 2):
+        assert got[k][0] == got[0][0], k
+        assert torch.equal(got[k][1], got[0][1]), k
+    del P
+
+
 def test_cg_mixed_reaches_tolerance(hp):
     dims = (32, 48, 64)
     say("cg mixed to 1e-9", dims)
