@@ -239,8 +239,8 @@ __global__ __launch_bounds__(kBlock) void k_gs_range(
 template <bool STREAM, bool FINEST>
 __global__ __launch_bounds__(kBlock) void k_gs_range16(
     int64_t begin, int64_t end, int64_t n, Col16 C, const int32_t *__restrict__ col,
-    const double *__restrict__ val, const double *__restrict__ r, double *x) {
-  const int64_t i = begin + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const double *__restrict__ val, const double *__restrict__ r, double *x, bool xcd) {
+  const int64_t i = begin + xcd_block(blockIdx.x, gridDim.x, xcd) * kBlock + threadIdx.x;
   if (i < end) gs_row16<STREAM>(C, i, n, col, val, r, x);
 }
 
@@ -783,11 +783,14 @@ void launch_symgs_rows(const Problem &P, const Level &L, int64_t begin, int64_t 
     const Col16 C = col16_of(P, L);
     const unsigned g = grid_for(end - begin);
     if (L.finest)
-      k_gs_range16<true, true><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col, L.val, r, x);
+      k_gs_range16<true, true><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col, L.val, r, x,
+                                                    L.xcd);
     else if (L.stream)
-      k_gs_range16<true, false><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col, L.val, r, x);
+      k_gs_range16<true, false><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col, L.val, r, x,
+                                                     L.xcd);
     else
-      k_gs_range16<false, false><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col, L.val, r, x);
+      k_gs_range16<false, false><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col, L.val, r, x,
+                                                      L.xcd);
     HPCGX_LAUNCHED();
     return;
   }

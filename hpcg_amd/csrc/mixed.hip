@@ -253,13 +253,22 @@ __global__ __launch_bounds__(kBlock) void k_pack_v16p(int64_t n, const half_t *_
   }
 }
 
+// The hardware hands workgroup b to XCD b % 8; with `xcd` XCD k takes the
+// k-th eighth of the rows in order (its L2 then holds its own x window).
+__device__ inline int64_t xcd_block(int64_t b, int64_t G, bool on) {
+  if (!on) return b;
+  const int64_t G8 = G - G % 8;
+  if (b >= G8) return b;
+  return (b % 8) * (G8 / 8) + b / 8;
+}
+
 template <bool STREAM, bool FINEST>
 __global__ __launch_bounds__(kBlock) void k_gs32_range8(
     int64_t begin, int64_t end, int64_t n, Col16 C, const uint32_t *__restrict__ vp,
     const uint32_t *__restrict__ vc, const int32_t *__restrict__ col,
     const half_t *__restrict__ val16, const float *__restrict__ diag32,
-    const float *__restrict__ r, float *x) {
-  const int64_t i = begin + gtid();
+    const float *__restrict__ r, float *x, bool xcd) {
+  const int64_t i = begin + xcd_block(blockIdx.x, gridDim.x, xcd) * kBlock + threadIdx.x;
   if (i < end) gs32_row8<STREAM>(C, vp, vc, i, n, col, val16, diag32, r, x);
 }
 
@@ -381,13 +390,13 @@ void gs32_step(const Problem &P, const Level &L, int64_t begin, int64_t end,
     const Col16 C = col16_of(P, L);
     if (L.finest)
       k_gs32_range8<true, true><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.v16p, L.vc, L.col,
-                                                     L.val16, L.diag32, r, x);
+                                                     L.val16, L.diag32, r, x, L.xcd);
     else if (L.stream)
       k_gs32_range8<true, false><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.v16p, L.vc, L.col,
-                                                      L.val16, L.diag32, r, x);
+                                                      L.val16, L.diag32, r, x, L.xcd);
     else
       k_gs32_range8<false, false><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.v16p, L.vc, L.col,
-                                                       L.val16, L.diag32, r, x);
+                                                       L.val16, L.diag32, r, x, L.xcd);
     HPCGX_LAUNCHED();
     return;
   }
