@@ -161,6 +161,7 @@ int hpcgx_generate_problem(int nx, int ny, int nz, int nlevels, int flags,
       const int64_t nc = (l + 1 < nlevels) ? n / 8 : 0;
       nat[l].alloc(n, nc);
       nat[l].N.nx = x; nat[l].N.ny = y; nat[l].N.nz = z;
+      nat[l].N.parts = {hpcgx::GridPart{0, x, y, z}};
       hpcgx::generate_natural(nat[l].N, s);
       // schedules and reordering of this level before generating the next
       // keep the peak footprint at about two copies of one level
@@ -214,6 +215,8 @@ int hpcgx_optimize_problem(const hpcgx_host_level *levels, int nlevels,
       }
       nat[l].alloc(n, nc);
       nat[l].N.nx = H.nx; nat[l].N.ny = H.ny; nat[l].N.nz = H.nz;
+      if (H.nx > 0 && H.ny > 0 && H.nz > 0 && (int64_t)H.nx * H.ny * H.nz == n)
+        nat[l].N.parts = {hpcgx::GridPart{0, H.nx, H.ny, H.nz}};
       HPCGX_HIP(hipMemcpy(nat[l].N.col, col.data(), sizeof(int32_t) * col.size(), hipMemcpyHostToDevice));
       HPCGX_HIP(hipMemcpy(nat[l].N.val, val.data(), sizeof(double) * val.size(), hipMemcpyHostToDevice));
       HPCGX_HIP(hipMemcpy(nat[l].N.nnz_row, nnz.data(), n, hipMemcpyHostToDevice));
@@ -664,6 +667,7 @@ int hpcgx_generate_fleet(int count, const int *nx, const int *ny, const int *nz,
       int64_t off = 0, offc = 0;
       for (int q = 0; q < count; ++q) {
         const int x = dims[3 * q] >> l, y = dims[3 * q + 1] >> l, z = dims[3 * q + 2] >> l;
+        nat[l].N.parts.push_back(hpcgx::GridPart{off, x, y, z});
         hpcgx::generate_natural_part(nat[l].N, x, y, z, off, offc, s);
         off += (int64_t)x * y * z;
         offc += (int64_t)x * y * z / 8;

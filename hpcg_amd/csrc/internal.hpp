@@ -332,8 +332,15 @@ void launch_symgs_wave(Problem &P, const Level &L, int dir, const double *r,
 int wave_grid_size();
 
 // ---- setup (setup.hip) ----
+struct GridPart {  // an nx*ny*nz grid in natural order at rows [off, off + nx*ny*nz)
+  int64_t off;
+  int nx, ny, nz;
+};
 struct NaturalLevel {  // device buffers of one level in natural row order
   int nx = 0, ny = 0, nz = 0;
+  // The grids the rows come from, when known: they propose the schedule
+  // (finalize_level verifies the proposal row by row before using it).
+  std::vector<GridPart> parts;
   int64_t n = 0, nnz = 0;
   int32_t *col = nullptr;     // [kSlots * n] column-major
   double *val = nullptr;      // [kSlots * n]

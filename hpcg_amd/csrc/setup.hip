@@ -204,6 +204,67 @@ __global__ __launch_bounds__(kBlock) void k_kahn_step(
   }
 }
 
+// Proposed schedule for rows that come from known grids: level x + 2y + 4z
+// and colour (x & 1) + 2 (y & 1) + 4 (z & 1) of the row's own grid.  A row
+// outside every part gets -1 (the verification then rejects the proposal).
+__global__ __launch_bounds__(kBlock) void k_propose_schedule(int64_t n, int nparts,
+                                                             const GridPart *__restrict__ parts,
+                                                             int32_t *level, int32_t *color) {
+  const int64_t i = gtid();
+  if (i >= n) return;
+  int lo = 0, hi = nparts - 1;  // last part with off <= i
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (parts[mid].off <= i) lo = mid; else hi = mid - 1;
+  }
+  const GridPart p = parts[lo];
+  const int64_t local = i - p.off;
+  if (local < 0 || local >= (int64_t)p.nx * p.ny * p.nz) {
+    level[i] = -1;
+    color[i] = -1;
+    return;
+  }
+  const int x = (int)(local % p.nx);
+  const int64_t t = local / p.nx;
+  const int y = (int)(t % p.ny), z = (int)(t / p.ny);
+  level[i] = x + 2 * y + 4 * z;
+  color[i] = (x & 1) | ((y & 1) << 1) | ((z & 1) << 2);
+}
+
+// Accepts a proposed (level, colour) only if every row has exactly what the
+// Kahn traversal gives it: level = 1 + the largest level of its lower
+// neighbours (0 without any), colour = the smallest colour none of its lower
+// neighbours has.  By induction over the rows in natural order the proposal
+// then equals the traversal's result.  bad: 1 = a row disagrees.
+__global__ __launch_bounds__(kBlock) void k_verify_schedule(int64_t n, const int32_t *col,
+                                                            const uint8_t *nnz_row,
+                                                            const int32_t *level,
+                                                            const int32_t *color, int *bad,
+                                                            int *max_level) {
+  const int64_t i = gtid();
+  bool ok = true;
+  int lv = 0;
+  if (i < n) {
+    const int m = nnz_row[i];
+    unsigned used = 0;
+    int below = -1;
+    for (int j = 0; j < m; ++j) {
+      const int64_t c = col[(int64_t)j * n + i];
+      if (c >= i) continue;
+      const int cc = color[c];
+      ok = ok && cc >= 0 && cc < 32;
+      if (cc >= 0 && cc < 32) used |= 1u << cc;
+      below = max(below, (int)level[c]);
+    }
+    lv = level[i];
+    ok = ok && color[i] == __ffs(~used) - 1 && lv == below + 1;
+  }
+  if (!ok) atomicOr(bad, 1);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) lv = max(lv, __shfl_xor(lv, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(max_level, lv);
+}
+
 __global__ void k_kahn_advance(int t, int64_t *lptr, const unsigned *tail) {
   lptr[t + 2] = (int64_t)*tail;
 }
@@ -571,9 +632,30 @@ void finalize_level(Problem &P, int l, NaturalLevel &N, int flags,
     if (b & 4) throw Error(HPCGX_EUNSUP, "sparsity pattern is not symmetric");
   }
 
-  // Kahn traversal: levels and first-fit colours
+  // Levels and first-fit colours: a proposal from the rows' grids when it
+  // passes the row-by-row check, else the Kahn traversal
   int nsched = 0;
-  {
+  bool proposed = false;
+  if (!N.parts.empty()) {
+    const int np = (int)N.parts.size();
+    TempBuf parts(sizeof(GridPart) * np), flags(2 * sizeof(int));
+    HPCGX_HIP(hipMemcpyAsync(parts.p, N.parts.data(), sizeof(GridPart) * np,
+                             hipMemcpyHostToDevice, s));
+    HPCGX_HIP(hipMemsetAsync(flags.p, 0, 2 * sizeof(int), s));
+    k_propose_schedule<<<grid_for(n), kBlock, 0, s>>>(n, np, parts.as<GridPart>(),
+                                                      level.as<int32_t>(), color.as<int32_t>());
+    HPCGX_LAUNCHED();
+    k_verify_schedule<<<grid_for(n), kBlock, 0, s>>>(n, N.col, N.nnz_row, level.as<int32_t>(),
+                                                     color.as<int32_t>(), flags.as<int>(),
+                                                     flags.as<int>() + 1);
+    HPCGX_LAUNCHED();
+    int f[2] = {1, 0};
+    HPCGX_HIP(hipMemcpyAsync(f, flags.p, sizeof f, hipMemcpyDeviceToHost, s));
+    HPCGX_HIP(hipStreamSynchronize(s));
+    proposed = f[0] == 0 && f[1] + 1 <= kMaxSched;
+    if (proposed) nsched = f[1] + 1;
+  }
+  if (!proposed) {
     const int64_t cap = std::min<int64_t>(n, kMaxSched) + 2;
     TempBuf lptr(sizeof(int64_t) * cap), tail(sizeof(unsigned));
     HPCGX_HIP(hipMemsetAsync(lptr.p, 0, sizeof(int64_t) * cap, s));

@@ -87,6 +87,33 @@ def test_schedules_match_oracle(hp, dims):
         assert np.array_equal(P.schedule(l, "color"), col)
 
 
+def test_schedule_proposal_checked(hp):
+    """Levels with grid dimensions propose the schedule from the grid; wrong
+    dimensions of the same row count must fail the row-by-row check, and the
+    Kahn traversal's schedule is used instead: the oracle's either way."""
+    dims = (16, 24, 32)
+    h = O.Hierarchy(*dims)
+    for nxyz, tag in ((dims, "right"), ((24, 16, 32), "wrong")):
+        say("schedule proposal", tag)
+        levels = []
+        for l in range(4):
+            m = h.level(l)
+            d = dict(nnz_row=m["nnz_row"], cols=m["cols"], vals=m["vals"],
+                     diag_slot=m["diag_slot"], nx=nxyz[0] >> l, ny=nxyz[1] >> l,
+                     nz=nxyz[2] >> l)
+            if l < 3:
+                d["f2c"] = m["f2c"]
+            levels.append(d)
+        P = hp.Problem.from_reference_levels(levels, keep_schedule=True)
+        for l in range(4):
+            lev, nl = O.levels(h.level(l))
+            col, nc = O.colors(h.level(l))
+            assert P.info(l)["nlevels_sched"] == nl, (tag, l)
+            assert np.array_equal(P.schedule(l, "level"), lev), (tag, l)
+            assert np.array_equal(P.schedule(l, "color"), col), (tag, l)
+        del P
+
+
 @pytest.mark.parametrize("dims", SMALL)
 def test_problem_vectors(hp, torch, dims):
     say("vectors", dims)
