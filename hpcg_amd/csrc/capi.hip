@@ -406,6 +406,10 @@ int hpcgx_tune(hpcgx_problem *p, const char *key, double value) {
       P.use_c8 = value != 0.0;
       return;
     }
+    if (k == "stream_p") {
+      P.stream_p = value != 0.0;
+      return;
+    }
     if (k == "fp16_pairs") {
       P.use_pairs = value != 0.0;
       return;

@@ -191,6 +191,7 @@ struct Problem {
   bool use_c16 = true;        // read narrow column ids where a level has them
   bool use_c8 = true;         // ... 8-bit ones where a tile has them
   bool use_pairs = true;      // FP32 colour steps of 8-bit tiles read FP16 value pairs
+  bool stream_p = false;      // p update with non-temporal accesses ("stream_p")
   int spmv_dot_blocks = 4096; // persistent grid of the fused SpMV + dot
   bool mixed_ready = false;   // FP16/FP32 copies built
   bool half_exact = false;    // every off-diagonal value is exact in FP16
@@ -320,10 +321,11 @@ void launch_update_xr(Problem &P, int64_t n, double *x, double *r,
 // mixed precision: dot with, and p update from, the FP32 V-cycle output
 void launch_dot32(Problem &P, int64_t n, const double *x, const float *y, double *out,
                   int finish, hipStream_t s);
+// stream: non-temporal vector accesses on the fine level (Problem::stream_p)
 void launch_update_p32(int64_t n, const float *z, double *p, const double *scalars,
-                       hipStream_t s);
+                       hipStream_t s, bool stream);
 void launch_update_p(int64_t n, const double *z, double *p,
-                     const double *scalars, hipStream_t s);
+                     const double *scalars, hipStream_t s, bool stream);
 void launch_residual0(Problem &P, int64_t n, const double *b, const double *Ap,
                       double *r, hipStream_t s);
 void launch_replace_diag(Level &L, const double *d, hipStream_t s);

@@ -78,6 +78,16 @@ __device__ inline T mload(const T *p) {
   if (STREAM) return __builtin_nontemporal_load(p);
   return *p;
 }
+// A result the kernel never reads back: streamed past L2 on large levels so
+// it does not evict the gathered vector.  The CG vector kernels stream their
+// vectors from kStreamRows rows on (16 Mi rows: 128 MiB per FP64 vector).
+template <bool STREAM, class T>
+__device__ inline void mstore(T *p, T v) {
+  if (STREAM)
+    __builtin_nontemporal_store(v, p);
+  else
+    *p = v;
+}
 
 // The hardware hands workgroup b to XCD b % 8.  Remapped, XCD x walks its own
 // contiguous eighth of the logical blocks, so each XCD's L2 works on one
@@ -519,6 +529,9 @@ __device__ inline void grid_finish(double s, double *partials, unsigned *ticket,
   }
 }
 
+constexpr int64_t kStreamRows = (int64_t)1 << 24;
+inline bool stream_vec(int64_t n) { return n >= kStreamRows; }
+
 inline unsigned reduce_grid(int64_t n) {
   const int64_t g = (n + kBlock - 1) / kBlock;
   return (unsigned)(g < kReduceBlocks ? (g < 1 ? 1 : g) : kReduceBlocks);
@@ -528,6 +541,7 @@ inline unsigned reduce_grid(int64_t n) {
 // order and add into one running sum in that order; the main loop loads
 // four of those rows before it uses any, so four rows' loads are in flight
 // per wavefront (the sums are the same as one row at a time).
+template <bool STREAM>
 __global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const double *x,
                                                  const double *y,
                                                  double *partials,
@@ -541,13 +555,13 @@ __global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const double *x,
     double a[4], b[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      a[u] = x[i + u * stride];
-      b[u] = y[i + u * stride];
+      a[u] = mload<STREAM>(x + i + u * stride);
+      b[u] = mload<STREAM>(y + i + u * stride);
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) s += a[u] * b[u];
   }
-  for (; i < n; i += stride) s += x[i] * y[i];
+  for (; i < n; i += stride) s += mload<STREAM>(x + i) * mload<STREAM>(y + i);
   s = block_sum(s);
   grid_finish(s, partials, ticket, scalars, out, finish);
 }
@@ -603,8 +617,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot_h(
         y = row_dot<STREAM>(col, val, p, n, i);
       } else {
         const int32_t *__restrict__ base = C.base + t;
-        const int ds = dslot[i];
-        const double di = diag[i];
+        const int ds = mload<STREAM>(dslot + i);
+        const double di = mload<STREAM>(diag + i);
         y = 0.0;
 #pragma unroll
         for (int j = 0; j < kSlots; ++j) {
@@ -615,7 +629,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot_h(
           y += a * p[c];
         }
       }
-      Ap[i] = y;
+      mstore<STREAM>(Ap + i, y);
       s += p[i] * y;
     }
   }
@@ -625,7 +639,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot_h(
 
 // x += alpha p; r -= alpha Ap; r.r (CG.cpp:121-124).  R32: also writes r
 // rounded to FP32, the input of the mixed-precision V-cycle.
-template <bool R32>
+template <bool R32, bool STREAM>
 __global__ __launch_bounds__(kBlock) void k_update_xr(
     int64_t n, double *__restrict__ x, double *__restrict__ r,
     const double *__restrict__ p, const double *__restrict__ Ap,
@@ -640,26 +654,26 @@ __global__ __launch_bounds__(kBlock) void k_update_xr(
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int64_t k = i + u * stride;
-      xv[u] = x[k];
-      pv[u] = p[k];
-      rv[u] = r[k];
-      av[u] = Ap[k];
+      xv[u] = mload<STREAM>(x + k);
+      pv[u] = mload<STREAM>(p + k);
+      rv[u] = mload<STREAM>(r + k);
+      av[u] = mload<STREAM>(Ap + k);
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int64_t k = i + u * stride;
-      x[k] = xv[u] + alpha * pv[u];
+      mstore<STREAM>(x + k, xv[u] + alpha * pv[u]);
       const double ri = rv[u] + malpha * av[u];
-      r[k] = ri;
-      if (R32) r32[k] = (float)ri;
+      mstore<STREAM>(r + k, ri);
+      if (R32) mstore<STREAM>(r32 + k, (float)ri);
       s += ri * ri;
     }
   }
   for (; i < n; i += stride) {
-    x[i] = x[i] + alpha * p[i];
-    const double ri = r[i] + malpha * Ap[i];
-    r[i] = ri;
-    if (R32) r32[i] = (float)ri;
+    mstore<STREAM>(x + i, mload<STREAM>(x + i) + alpha * mload<STREAM>(p + i));
+    const double ri = mload<STREAM>(r + i) + malpha * mload<STREAM>(Ap + i);
+    mstore<STREAM>(r + i, ri);
+    if (R32) mstore<STREAM>(r32 + i, (float)ri);
     s += ri * ri;
   }
   s = block_sum(s);
@@ -667,6 +681,7 @@ __global__ __launch_bounds__(kBlock) void k_update_xr(
 }
 
 // r.z with z the FP32 V-cycle output, widened exactly (CG.cpp:110, 113).
+template <bool STREAM>
 __global__ __launch_bounds__(kBlock) void k_dot32(int64_t n, const double *x, const float *y,
                                                    double *partials, unsigned *ticket,
                                                    double *scalars, double *out, int finish) {
@@ -678,26 +693,29 @@ __global__ __launch_bounds__(kBlock) void k_dot32(int64_t n, const double *x, co
     float b[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      a[u] = x[i + u * stride];
-      b[u] = y[i + u * stride];
+      a[u] = mload<STREAM>(x + i + u * stride);
+      b[u] = mload<STREAM>(y + i + u * stride);
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) s += a[u] * (double)b[u];
   }
-  for (; i < n; i += stride) s += x[i] * (double)y[i];
+  for (; i < n; i += stride) s += mload<STREAM>(x + i) * (double)mload<STREAM>(y + i);
   s = block_sum(s);
   grid_finish(s, partials, ticket, scalars, out, finish);
 }
 
 // p = z + beta p with z in FP32, widened exactly (CG.cpp:115).
+template <bool STREAM>
 __global__ __launch_bounds__(kBlock) void k_update_p32(int64_t n, const float *__restrict__ z,
                                                         double *__restrict__ p,
                                                         const double *__restrict__ scalars) {
   const int64_t i = gtid();
-  if (i < n) p[i] = (double)z[i] + scalars[kBeta] * p[i];
+  if (i < n)
+    mstore<STREAM>(p + i, (double)mload<STREAM>(z + i) + scalars[kBeta] * mload<STREAM>(p + i));
 }
 
 // r = b - Ap and r.r (CG.cpp:88-89)
+template <bool STREAM>
 __global__ __launch_bounds__(kBlock) void k_residual0(
     int64_t n, const double *__restrict__ b, const double *__restrict__ Ap,
     double *__restrict__ r, double *partials, unsigned *ticket,
@@ -709,19 +727,19 @@ __global__ __launch_bounds__(kBlock) void k_residual0(
     double bv[4], av[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      bv[u] = b[i + u * stride];
-      av[u] = Ap[i + u * stride];
+      bv[u] = mload<STREAM>(b + i + u * stride);
+      av[u] = mload<STREAM>(Ap + i + u * stride);
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const double ri = bv[u] + -1.0 * av[u];
-      r[i + u * stride] = ri;
+      mstore<STREAM>(r + i + u * stride, ri);
       s += ri * ri;
     }
   }
   for (; i < n; i += stride) {
-    const double ri = b[i] + -1.0 * Ap[i];
-    r[i] = ri;
+    const double ri = mload<STREAM>(b + i) + -1.0 * mload<STREAM>(Ap + i);
+    mstore<STREAM>(r + i, ri);
     s += ri * ri;
   }
   s = block_sum(s);
@@ -729,11 +747,12 @@ __global__ __launch_bounds__(kBlock) void k_residual0(
 }
 
 // p = z + beta p (CG.cpp:115)
+template <bool STREAM>
 __global__ __launch_bounds__(kBlock) void k_update_p(
     int64_t n, const double *__restrict__ z, double *__restrict__ p,
     const double *__restrict__ scalars) {
   const int64_t i = gtid();
-  if (i < n) p[i] = z[i] + scalars[kBeta] * p[i];
+  if (i < n) mstore<STREAM>(p + i, mload<STREAM>(z + i) + scalars[kBeta] * mload<STREAM>(p + i));
 }
 
 }  // namespace
@@ -887,8 +906,12 @@ void launch_waxpby(int64_t n, double alpha, const double *x, double beta,
 
 void launch_dot(Problem &P, int64_t n, const double *x, const double *y,
                 double *out, int finish, hipStream_t s) {
-  k_dot<<<reduce_grid(n), kBlock, 0, s>>>(n, x, y, P.partials, P.ticket,
-                                          P.scalars, out, finish);
+  if (stream_vec(n))
+    k_dot<true><<<reduce_grid(n), kBlock, 0, s>>>(n, x, y, P.partials, P.ticket, P.scalars, out,
+                                                  finish);
+  else
+    k_dot<false><<<reduce_grid(n), kBlock, 0, s>>>(n, x, y, P.partials, P.ticket, P.scalars, out,
+                                                   finish);
   HPCGX_LAUNCHED();
 }
 
@@ -936,38 +959,60 @@ void launch_spmv_dot_half(Problem &P, const Level &L, const double *p, double *A
 
 void launch_update_xr(Problem &P, int64_t n, double *x, double *r,
                       const double *p, const double *Ap, hipStream_t s, float *r32) {
-  if (r32)
-    k_update_xr<true><<<reduce_grid(n), kBlock, 0, s>>>(n, x, r, p, Ap, r32, P.partials,
-                                                        P.ticket, P.scalars);
+  const bool st = stream_vec(n);
+  const unsigned g = reduce_grid(n);
+  if (r32 && st)
+    k_update_xr<true, true><<<g, kBlock, 0, s>>>(n, x, r, p, Ap, r32, P.partials, P.ticket,
+                                                 P.scalars);
+  else if (r32)
+    k_update_xr<true, false><<<g, kBlock, 0, s>>>(n, x, r, p, Ap, r32, P.partials, P.ticket,
+                                                  P.scalars);
+  else if (st)
+    k_update_xr<false, true><<<g, kBlock, 0, s>>>(n, x, r, p, Ap, nullptr, P.partials,
+                                                  P.ticket, P.scalars);
   else
-    k_update_xr<false><<<reduce_grid(n), kBlock, 0, s>>>(n, x, r, p, Ap, nullptr, P.partials,
-                                                         P.ticket, P.scalars);
+    k_update_xr<false, false><<<g, kBlock, 0, s>>>(n, x, r, p, Ap, nullptr, P.partials,
+                                                   P.ticket, P.scalars);
   HPCGX_LAUNCHED();
 }
 
 void launch_dot32(Problem &P, int64_t n, const double *x, const float *y, double *out,
                   int finish, hipStream_t s) {
-  k_dot32<<<reduce_grid(n), kBlock, 0, s>>>(n, x, y, P.partials, P.ticket, P.scalars, out,
-                                            finish);
+  if (stream_vec(n))
+    k_dot32<true><<<reduce_grid(n), kBlock, 0, s>>>(n, x, y, P.partials, P.ticket, P.scalars,
+                                                    out, finish);
+  else
+    k_dot32<false><<<reduce_grid(n), kBlock, 0, s>>>(n, x, y, P.partials, P.ticket, P.scalars,
+                                                     out, finish);
   HPCGX_LAUNCHED();
 }
 
 void launch_update_p32(int64_t n, const float *z, double *p, const double *scalars,
-                       hipStream_t s) {
-  k_update_p32<<<grid_for(n), kBlock, 0, s>>>(n, z, p, scalars);
+                       hipStream_t s, bool stream) {
+  if (stream && stream_vec(n))
+    k_update_p32<true><<<grid_for(n), kBlock, 0, s>>>(n, z, p, scalars);
+  else
+    k_update_p32<false><<<grid_for(n), kBlock, 0, s>>>(n, z, p, scalars);
   HPCGX_LAUNCHED();
 }
 
 void launch_update_p(int64_t n, const double *z, double *p,
-                     const double *scalars, hipStream_t s) {
-  k_update_p<<<grid_for(n), kBlock, 0, s>>>(n, z, p, scalars);
+                     const double *scalars, hipStream_t s, bool stream) {
+  if (stream && stream_vec(n))
+    k_update_p<true><<<grid_for(n), kBlock, 0, s>>>(n, z, p, scalars);
+  else
+    k_update_p<false><<<grid_for(n), kBlock, 0, s>>>(n, z, p, scalars);
   HPCGX_LAUNCHED();
 }
 
 void launch_residual0(Problem &P, int64_t n, const double *b, const double *Ap,
                       double *r, hipStream_t s) {
-  k_residual0<<<reduce_grid(n), kBlock, 0, s>>>(n, b, Ap, r, P.partials,
-                                                P.ticket, P.scalars);
+  if (stream_vec(n))
+    k_residual0<true><<<reduce_grid(n), kBlock, 0, s>>>(n, b, Ap, r, P.partials, P.ticket,
+                                                        P.scalars);
+  else
+    k_residual0<false><<<reduce_grid(n), kBlock, 0, s>>>(n, b, Ap, r, P.partials, P.ticket,
+                                                         P.scalars);
   HPCGX_LAUNCHED();
 }
 

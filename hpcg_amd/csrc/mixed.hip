@@ -599,7 +599,7 @@ void cg_mixed(Problem &P, int ordering, const double *b, double *x,
     if (k == 1)
       to_f64(n, P.z32, P.p, s);
     else
-      launch_update_p32(n, P.z32, P.p, P.scalars, s);
+      launch_update_p32(n, P.z32, P.p, P.scalars, s, P.stream_p);
     launch_spmv_dot_half(P, A, P.p, P.Ap, s);
     launch_update_xr(P, n, x, P.r, P.p, P.Ap, s, P.r32);
     if (recompute > 0 && k % recompute == 0) {

@@ -150,7 +150,7 @@ void cg_iteration(Problem &P, int ordering, bool precond, double *x, hipStream_t
   else
     launch_copy(P.r, P.z, n, s);
   launch_dot(P, n, P.r, P.z, nullptr, kFinishBeta, s);
-  launch_update_p(n, P.z, P.p, P.scalars, s);
+  launch_update_p(n, P.z, P.p, P.scalars, s, P.stream_p);
   launch_spmv_dot(P, A, P.p, P.Ap, s);
   launch_update_xr(P, n, x, P.r, P.p, P.Ap, s);
 }
@@ -232,7 +232,7 @@ void cg(Problem &P, int ordering, const double *b, double *x, int max_iter,
     if (k == 1)
       launch_copy(P.z, P.p, n, s);                       // :109
     else
-      launch_update_p(n, P.z, P.p, P.scalars, s);        // :115
+      launch_update_p(n, P.z, P.p, P.scalars, s, P.stream_p);        // :115
     T.tock(t2);
     T.tick();
     launch_spmv_dot(P, A, P.p, P.Ap, s);                 // :118-120

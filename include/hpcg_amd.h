@@ -318,6 +318,8 @@ int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, i
  *                      (default 1)
  *   "col8"             1/0: use the 8-bit offsets of tiles that have them
  *                      (else their 16-bit ones; default 1)
+ *   "stream_p"         1/0: the CG p update reads and writes the fine-level
+ *                      vectors with non-temporal accesses (default 0)
  *   "fp16_pairs"       1/0: the FP32 V-cycle's colour steps on 8-bit tiles
  *                      read FP16 values two slots per word beside the 8-bit
  *                      offsets (3 bytes per slot; else the 4-byte packed
