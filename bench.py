@@ -128,7 +128,7 @@ def converge(P, b, lv, a, log):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         if mode == "fp64":
-            r = P.cg(b, x, max_iter=a.max_iters, tolerance=a.tol_compare)
+            r = P.cg(b, x, max_iter=a.max_iters, tolerance=a.tol_compare, graph=True)
         else:
             r = P.cg_mixed(b, x, max_iter=a.max_iters, tolerance=a.tol_compare, recompute=10)
         torch.cuda.synchronize()

@@ -398,12 +398,17 @@ int hpcgx_tune(hpcgx_problem *p, const char *key, double value) {
     hpcgx::Problem &P = p->P;
     P.graph.reset();  // kernel choices may change: recapture on next use
     P.fleet.graph.reset();
+    P.mgraph.reset();
     if (k == "col16") {
       P.use_c16 = value != 0.0;
       return;
     }
     if (k == "col8") {
       P.use_c8 = value != 0.0;
+      return;
+    }
+    if (k == "mixed_graph") {
+      P.use_mixed_graph = value != 0.0;
       return;
     }
     if (k == "stream_p") {

@@ -227,6 +227,10 @@ struct Problem {
       if (cap) (void)hipStreamDestroy(cap);
     }
   } graph;
+  // captured body of the mixed-precision CG's iterations 2.. (mixed.hip),
+  // keyed by x and the ordering; used while use_mixed_graph is set
+  IterGraph mgraph;
+  bool use_mixed_graph = true;
   // A fleet: the block-diagonal union of `count` independent problems
   // (fleet.hip).  Level l holds every problem's level l, natural rows
   // concatenated problem by problem (problem q: rows [off[l][q], off[l][q+1])).

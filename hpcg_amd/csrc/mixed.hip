@@ -575,6 +575,48 @@ double read_scalar(const Problem &P, int slot, hipStream_t s) {
 
 // CG.cpp:59-143 with the FP32 V-cycle as M^-1 and the FP64 residual
 // recomputed every `recompute` iterations.
+namespace {
+// One iteration of the mixed-precision CG without the residual recompute
+// (CG.cpp:102-124 with the FP32 V-cycle), enqueued on s.
+void mixed_iteration(Problem &P, int ordering, double *x, bool first, hipStream_t s) {
+  const Level &A = P.L[0];
+  const int64_t n = A.n;
+  mg32(P, 0, ordering, P.r32, P.z32, s);
+  launch_dot32(P, n, P.r, P.z32, nullptr, kFinishBeta, s);
+  if (first)
+    to_f64(n, P.z32, P.p, s);
+  else
+    launch_update_p32(n, P.z32, P.p, P.scalars, s, P.stream_p);
+  launch_spmv_dot_half(P, A, P.p, P.Ap, s);
+  launch_update_xr(P, n, x, P.r, P.p, P.Ap, s, P.r32);
+}
+
+// The captured body of iterations 2.. (rebuilt when its key changes; fills
+// are kernels, so the graph holds no memset nodes).
+hipGraphExec_t mixed_graph(Problem &P, int ordering, double *x) {
+  auto &G = P.mgraph;
+  if (G.exec && G.x == x && G.ordering == ordering) return G.exec;
+  G.reset();
+  if (!G.cap) HPCGX_HIP(hipStreamCreateWithFlags(&G.cap, hipStreamNonBlocking));
+  HPCGX_HIP(hipStreamBeginCapture(G.cap, hipStreamCaptureModeThreadLocal));
+  hipGraph_t graph = nullptr;
+  try {
+    mixed_iteration(P, ordering, x, false, G.cap);
+  } catch (...) {
+    (void)hipStreamEndCapture(G.cap, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  HPCGX_HIP(hipStreamEndCapture(G.cap, &graph));
+  const hipError_t e = hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  HPCGX_HIP(e);
+  G.x = x;
+  G.ordering = ordering;
+  return G.exec;
+}
+}  // namespace
+
 void cg_mixed(Problem &P, int ordering, const double *b, double *x,
               int max_iter, double tol, int recompute, int *niters,
               double *normr, double *normr0, double *times, hipStream_t s) {
@@ -594,14 +636,10 @@ void cg_mixed(Problem &P, int ordering, const double *b, double *x,
   // gives the bits of widening it first.
   to_f32(n, P.r, P.r32, s);
   for (int k = 1; k <= max_iter && nr / *normr0 > tol; ++k) {
-    mg32(P, 0, ordering, P.r32, P.z32, s);
-    launch_dot32(P, n, P.r, P.z32, nullptr, kFinishBeta, s);
-    if (k == 1)
-      to_f64(n, P.z32, P.p, s);
+    if (k >= 2 && P.use_mixed_graph && !P.klog.on)
+      HPCGX_HIP(hipGraphLaunch(mixed_graph(P, ordering, x), s));
     else
-      launch_update_p32(n, P.z32, P.p, P.scalars, s, P.stream_p);
-    launch_spmv_dot_half(P, A, P.p, P.Ap, s);
-    launch_update_xr(P, n, x, P.r, P.p, P.Ap, s, P.r32);
+      mixed_iteration(P, ordering, x, k == 1, s);
     if (recompute > 0 && k % recompute == 0) {
       // r = b - A x in FP64 (replaces the recursively updated r)
       launch_spmv(P, A, x, P.Ap, s);

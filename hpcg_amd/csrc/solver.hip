@@ -186,8 +186,9 @@ hipGraphExec_t iteration_graph(Problem &P, int ordering, bool precond, double *x
 //   r.z            -> dot kernel that also forms beta = rtz/oldrtz (:112-114)
 //   Ap = A p, p.Ap -> one SpMV kernel that also forms alpha (:118-120)
 //   x, r updates, r.r -> one kernel (:121-124)
-// With HPCGX_CG_GRAPH (and no tolerance test, timers or wavefront sweeps)
-// iterations 2.. replay one captured graph of cg_iteration.
+// With HPCGX_CG_GRAPH (and no timers or wavefront sweeps) iterations 2..
+// replay one captured graph of cg_iteration; with a tolerance the host reads
+// the residual after each replay.
 void cg(Problem &P, int ordering, const double *b, double *x, int max_iter,
         double tol, bool precond, int flags, int *niters, double *normr,
         double *normr0, double *times, hipStream_t s) {
@@ -210,13 +211,13 @@ void cg(Problem &P, int ordering, const double *b, double *x, int max_iter,
   const bool check_each = tol > 0.0;
   bool wave = false;
   for (int l = 0; l < P.nlevels; ++l) wave = wave || (P.L[l].wave && P.use_wave);
-  const bool graphed = (flags & HPCGX_CG_GRAPH) && !check_each && !T.on && !P.klog.on && !wave;
+  const bool graphed = (flags & HPCGX_CG_GRAPH) && !T.on && !P.klog.on && !wave;
   int k = 1;
   for (; k <= max_iter && nr / *normr0 > tol; ++k) {     // :100
     if (graphed && k >= 2) {
       HPCGX_HIP(hipGraphLaunch(iteration_graph(P, ordering, precond, x), s));
       *niters = k;
-      if (k == max_iter) nr = std::sqrt(read_scalar(P, kNormr2, s));
+      if (check_each || k == max_iter) nr = std::sqrt(read_scalar(P, kNormr2, s));
       continue;
     }
     T.tick();

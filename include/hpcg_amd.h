@@ -208,10 +208,10 @@ int hpcgx_waxpby(int64_t n, double alpha, const double *x, double beta,
 enum {
   HPCGX_CG_TIMERS = 1,
   /* run iterations 2.. as replays of one captured hipGraph of the
-   * iteration (tolerance 0 only; falls back to eager launches when a
-   * tolerance, timers, the kernel-timing log or wavefront sweeps are on).
-   * The graph is kept with the problem and rebuilt when x, the ordering,
-   * the preconditioning switch or a tuning knob changes. */
+   * iteration (eager launches when timers, the kernel-timing log or
+   * wavefront sweeps are on; with a tolerance the residual is read after
+   * each replay).  The graph is kept with the problem and rebuilt when x,
+   * the ordering, the preconditioning switch or a tuning knob changes. */
   HPCGX_CG_GRAPH = 2
 };
 int hpcgx_cg(hpcgx_problem *p, int ordering, const double *b, double *x,
@@ -244,7 +244,9 @@ int hpcgx_mg32(hpcgx_problem *p, int level, int ordering, const float *r,
  * and sums over the FP16 off-diagonal values and the FP64 diagonal (equal
  * to the FP64 SpMV while the off-diagonal values are exact in FP16); after
  * every `recompute`-th iteration r = b - A x is recomputed with the FP64
- * matrix (0 = never).  times[0] accumulates the wall time.  Synchronous. */
+ * matrix (0 = never).  Iterations 2.. replay one captured hipGraph of the
+ * iteration unless the "mixed_graph" knob is 0.  times[0] accumulates the
+ * wall time.  Synchronous. */
 int hpcgx_cg_mixed(hpcgx_problem *p, int ordering, const double *b, double *x,
                    int max_iter, double tolerance, int recompute, int *niters,
                    double *normr, double *normr0, double *times, void *stream);
@@ -318,6 +320,8 @@ int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, i
  *                      (default 1)
  *   "col8"             1/0: use the 8-bit offsets of tiles that have them
  *                      (else their 16-bit ones; default 1)
+ *   "mixed_graph"      1/0: hpcgx_cg_mixed replays a captured iteration
+ *                      (default 1)
  *   "stream_p"         1/0: the CG p update reads and writes the fine-level
  *                      vectors with non-temporal accesses (default 0)
  *   "fp16_pairs"       1/0: the FP32 V-cycle's colour steps on 8-bit tiles

@@ -205,6 +205,35 @@ def test_cg_mixed_same_bits_every_format(hp, dims):
     del P
 
 
+@pytest.mark.parametrize("tol", [0.0, 1e-6])
+def test_cg_mixed_graph_replays(hp, tol):
+    """The captured mixed iteration gives the eager launches' bits, call
+    after call, with eager work on the same problem between the calls and a
+    recapture after a tuning change (recompute every 4 iterations)."""
+    import torch
+    dims = (32, 48, 64)
+    say("cg mixed graph", tol)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    b, x, _ = P.vectors()
+    P.tune("mixed_graph", 0)
+    x.zero_()
+    want = P.cg_mixed(b, x, max_iter=19, tolerance=tol, recompute=4)
+    xw = x.clone()
+    P.tune("mixed_graph", 1)
+    for rep in range(3):
+        if rep == 2:
+            P.tune("fp16_pairs", 0)  # drops the graph; same bits by construction
+        x.zero_()
+        r = P.cg_mixed(b, x, max_iter=19, tolerance=tol, recompute=4)
+        assert r["niters"] == want["niters"] and r["normr"] == want["normr"], rep
+        assert torch.equal(x, xw), rep
+        z = P.zeros32()
+        P.mg32(P.zeros32() + 1.0, z)
+        x.zero_()
+        P.cg(b, x, max_iter=2, tolerance=0.0)
+    del P
+
+
 def test_cg_mixed_reaches_tolerance(hp):
     dims = (32, 48, 64)
     say("cg mixed to 1e-9", dims)

@@ -421,6 +421,23 @@ def test_cg_trajectory(hp, torch, dims, its, ordering):
         assert r["normr"] / r["normr0"] == pytest.approx(want[k], rel=1e-9), k
 
 
+def test_cg_graph_with_tolerance(hp, torch):
+    """With a tolerance the captured iterations stop where the eager ones do,
+    with the same bits."""
+    dims = (32, 48, 64)
+    say("cg graph tolerance")
+    P = hp.Problem.generate(*dims, nlevels=4)
+    b, x, _ = P.vectors()
+    want = P.cg(b, x, max_iter=100, tolerance=1e-7, ordering="multicolor")
+    xw = x.clone()
+    for rep in range(2):
+        x.zero_()
+        r = P.cg(b, x, max_iter=100, tolerance=1e-7, ordering="multicolor", graph=True)
+        assert r["niters"] == want["niters"] and r["normr"] == want["normr"], rep
+        assert torch.equal(x, xw), rep
+    del P
+
+
 @pytest.mark.parametrize("ordering,precond", [("multicolor", True), ("lexicographic", True),
                                               ("multicolor", False)])
 def test_cg_graph_replays(hp, torch, ordering, precond):
