@@ -664,6 +664,7 @@ void prepare_block(Problem &P, int s) {
   prepare_mixed(P);
   auto &B = P.blk;
   if (B.s >= s) return;
+  P.bgraph.reset();  // it holds the old buffers
   B.release(P);
   const int64_t n = P.L[0].n;
   B.R = P.arena.alloc<double>((size_t)n * s);
@@ -750,45 +751,94 @@ void spmv_block(Problem &P, int l, int s, const double *X, double *Y, hipStream_
   HPCGX_LAUNCHED();
 }
 
+namespace {
+// Y = A X for the block CG's fine level; with `dot`, p_k . (A p)_k and alpha.
+void block_spmv_dot(Problem &P, int s, const double *x, double *y, bool dot, hipStream_t st) {
+  auto &B = P.blk;
+  const Level &A = P.L[0];
+  const int64_t total = A.n * s;
+  if (rows_kernels(s)) {
+    // W lanes per row: a persistent grid when it reduces, one work-item per
+    // (row, lane) otherwise
+    const unsigned grow = (unsigned)std::max<int64_t>(
+        1, std::min<int64_t>((A.n * (s / 2) + kBlock - 1) / kBlock, kMaxPartials));
+    spmv_rows(P, A, s, x, y, dot, dot ? grow : grid_for(A.n * (s / 2)), st);
+    return;
+  }
+  const unsigned g = dot ? block_reduce_grid(total, s) : grid_for(total);
+  if (A.stream)
+    k_bspmv_dot<true><<<g, kBlock, 0, st>>>(A.n, s, A.col, A.val, x, y, dot, B.partials,
+                                            B.ticket, B.scalars);
+  else
+    k_bspmv_dot<false><<<g, kBlock, 0, st>>>(A.n, s, A.col, A.val, x, y, dot, B.partials,
+                                             B.ticket, B.scalars);
+  HPCGX_LAUNCHED();
+}
+
+// r = b - A x, r32, |r|^2 (CG.cpp:86-89, and the periodic recompute)
+void block_residual(Problem &P, int s, const double *Bv, const double *X, hipStream_t st) {
+  auto &B = P.blk;
+  const int64_t total = P.L[0].n * s;
+  block_spmv_dot(P, s, X, B.AP, false, st);
+  k_bresidual0<<<block_reduce_grid(total, s), kBlock, 0, st>>>(total, s, Bv, B.AP, B.R, B.R32,
+                                                               B.partials, B.ticket, B.scalars);
+  HPCGX_LAUNCHED();
+}
+
+// One block iteration without the recompute (CG.cpp:102-124 per column).
+void block_iteration(Problem &P, int ordering, int s, double *X, bool first, hipStream_t st) {
+  auto &B = P.blk;
+  const int64_t total = P.L[0].n * s;
+  const unsigned gr = block_reduce_grid(total, s);
+  mg32_block(P, 0, ordering, s, B.R32, B.Z32, st);
+  k_bdot32<<<gr, kBlock, 0, st>>>(total, s, B.R, B.Z32, B.partials, B.ticket, B.scalars);
+  HPCGX_LAUNCHED();
+  k_bupdate_p32<<<grid_for(total), kBlock, 0, st>>>(total, s, B.Z32, B.Pv, B.scalars, first);
+  HPCGX_LAUNCHED();
+  block_spmv_dot(P, s, B.Pv, B.AP, true, st);
+  k_bupdate_xr<<<gr, kBlock, 0, st>>>(total, s, X, B.R, B.Pv, B.AP, B.R32, B.partials, B.ticket,
+                                      B.scalars);
+  HPCGX_LAUNCHED();
+}
+
+// The captured body of iterations 2.. (rebuilt when its key changes).
+hipGraphExec_t block_graph(Problem &P, int ordering, int s, double *X) {
+  auto &G = P.bgraph;
+  if (G.exec && G.x == X && G.ordering == ordering && P.bgraph_s == s) return G.exec;
+  G.reset();
+  if (!G.cap) HPCGX_HIP(hipStreamCreateWithFlags(&G.cap, hipStreamNonBlocking));
+  HPCGX_HIP(hipStreamBeginCapture(G.cap, hipStreamCaptureModeThreadLocal));
+  hipGraph_t graph = nullptr;
+  try {
+    block_iteration(P, ordering, s, X, false, G.cap);
+  } catch (...) {
+    (void)hipStreamEndCapture(G.cap, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  HPCGX_HIP(hipStreamEndCapture(G.cap, &graph));
+  const hipError_t e = hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  HPCGX_HIP(e);
+  G.x = X;
+  G.ordering = ordering;
+  P.bgraph_s = s;
+  return G.exec;
+}
+}  // namespace
+
 // CG.cpp:59-143 for s right-hand sides with the FP32 V-cycle (cg_mixed's
 // arithmetic for each of them).  Iterates until every system reaches the
-// tolerance or max_iter; normr/normr0 per right-hand side.
+// tolerance or max_iter; normr/normr0 per right-hand side.  Iterations 2..
+// replay one captured graph (unless the "mixed_graph" knob is off).
 void cg_block_mixed(Problem &P, int ordering, int s, const double *Bv, double *X, int max_iter,
                     double tol, int recompute, int *niters, double *normr, double *normr0,
                     double *times, hipStream_t st) {
   prepare_block(P, s);
   const auto t0 = std::chrono::steady_clock::now();
-  auto &B = P.blk;
-  const Level &A = P.L[0];
-  const int64_t total = A.n * s;
-  const unsigned gr = block_reduce_grid(total, s);
-  // W-lanes-per-row SpMV: a persistent grid when it reduces, one work-item
-  // per (row, lane) otherwise
-  const unsigned grow = (unsigned)std::max<int64_t>(
-      1, std::min<int64_t>((A.n * (s / 2) + kBlock - 1) / kBlock, kMaxPartials));
-  auto spmv_dot = [&](const double *x, double *y, bool dot) {
-    if (rows_kernels(s)) {
-      spmv_rows(P, A, s, x, y, dot, dot ? grow : grid_for(A.n * (s / 2)), st);
-      return;
-    }
-    const unsigned g = dot ? gr : grid_for(total);
-    if (A.stream)
-      k_bspmv_dot<true><<<g, kBlock, 0, st>>>(A.n, s, A.col, A.val, x, y, dot, B.partials,
-                                              B.ticket, B.scalars);
-    else
-      k_bspmv_dot<false><<<g, kBlock, 0, st>>>(A.n, s, A.col, A.val, x, y, dot, B.partials,
-                                               B.ticket, B.scalars);
-    HPCGX_LAUNCHED();
-  };
-  auto residual = [&]() {  // r = b - A x, r32, |r|^2
-    spmv_dot(X, B.AP, false);
-    k_bresidual0<<<gr, kBlock, 0, st>>>(total, s, Bv, B.AP, B.R, B.R32, B.partials, B.ticket,
-                                        B.scalars);
-    HPCGX_LAUNCHED();
-  };
   std::vector<double> n2;
   *niters = 0;
-  residual();  // :86-89 (p = x, Ap = A p, r = b - Ap)
+  block_residual(P, s, Bv, X, st);  // :86-89 (p = x, Ap = A p, r = b - Ap)
   read_block_scalars(P, s, n2, st);
   for (int k = 0; k < s; ++k) normr0[k] = normr[k] = std::sqrt(n2[k]);
   auto done = [&]() {
@@ -798,16 +848,11 @@ void cg_block_mixed(Problem &P, int ordering, int s, const double *Bv, double *X
   };
   const bool check_each = tol > 0.0;
   for (int it = 1; it <= max_iter && !done(); ++it) {
-    mg32_block(P, 0, ordering, s, B.R32, B.Z32, st);
-    k_bdot32<<<gr, kBlock, 0, st>>>(total, s, B.R, B.Z32, B.partials, B.ticket, B.scalars);
-    HPCGX_LAUNCHED();
-    k_bupdate_p32<<<grid_for(total), kBlock, 0, st>>>(total, s, B.Z32, B.Pv, B.scalars, it == 1);
-    HPCGX_LAUNCHED();
-    spmv_dot(B.Pv, B.AP, true);
-    k_bupdate_xr<<<gr, kBlock, 0, st>>>(total, s, X, B.R, B.Pv, B.AP, B.R32, B.partials,
-                                        B.ticket, B.scalars);
-    HPCGX_LAUNCHED();
-    if (recompute > 0 && it % recompute == 0) residual();
+    if (it >= 2 && P.use_mixed_graph && !P.klog.on)
+      HPCGX_HIP(hipGraphLaunch(block_graph(P, ordering, s, X), st));
+    else
+      block_iteration(P, ordering, s, X, it == 1, st);
+    if (recompute > 0 && it % recompute == 0) block_residual(P, s, Bv, X, st);
     *niters = it;
     if (check_each || it == max_iter) {
       read_block_scalars(P, s, n2, st);

@@ -399,6 +399,7 @@ int hpcgx_tune(hpcgx_problem *p, const char *key, double value) {
     P.graph.reset();  // kernel choices may change: recapture on next use
     P.fleet.graph.reset();
     P.mgraph.reset();
+    P.bgraph.reset();
     if (k == "col16") {
       P.use_c16 = value != 0.0;
       return;

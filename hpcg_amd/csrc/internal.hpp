@@ -231,6 +231,10 @@ struct Problem {
   // keyed by x and the ordering; used while use_mixed_graph is set
   IterGraph mgraph;
   bool use_mixed_graph = true;
+  // captured body of the block CG's iterations 2.. (block.hip), keyed by X,
+  // the ordering and s; dropped when the block state is reallocated
+  IterGraph bgraph;
+  int bgraph_s = 0;
   // A fleet: the block-diagonal union of `count` independent problems
   // (fleet.hip).  Level l holds every problem's level l, natural rows
   // concatenated problem by problem (problem q: rows [off[l][q], off[l][q+1])).

@@ -320,8 +320,8 @@ int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, i
  *                      (default 1)
  *   "col8"             1/0: use the 8-bit offsets of tiles that have them
  *                      (else their 16-bit ones; default 1)
- *   "mixed_graph"      1/0: hpcgx_cg_mixed replays a captured iteration
- *                      (default 1)
+ *   "mixed_graph"      1/0: hpcgx_cg_mixed and hpcgx_cg_block_mixed replay a
+ *                      captured iteration (default 1)
  *   "stream_p"         1/0: the CG p update reads and writes the fine-level
  *                      vectors with non-temporal accesses (default 0)
  *   "fp16_pairs"       1/0: the FP32 V-cycle's colour steps on 8-bit tiles

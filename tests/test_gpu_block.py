@@ -125,3 +125,30 @@ def test_cg_block_reaches_tolerance(hp):
         res = float((B[:, k] - Y[:, k]).norm()) / float(B[:, k].norm())
         assert res <= 2e-9, (k, res)
     del P
+
+
+def test_cg_block_graph_replays(hp):
+    """The captured block iteration gives the eager launches' bits, across
+    calls with different s (each recaptures) and eager work between them."""
+    import torch
+    dims = (32, 48, 64)
+    say("cg block graph", dims)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    n = P.nrow(0)
+    want = {}
+    P.tune("mixed_graph", 0)
+    for s in (4, 8):
+        B = P.to_device_order_multi(rhs(n, s))
+        X = torch.zeros_like(B)
+        r = P.cg_block_mixed(B, X, max_iter=13, tolerance=0.0, recompute=5)
+        want[s] = (B, r, X.clone())
+    P.tune("mixed_graph", 1)
+    for s in (8, 4, 8):
+        B, w, Xw = want[s]
+        X = torch.zeros_like(B)
+        r = P.cg_block_mixed(B, X, max_iter=13, tolerance=0.0, recompute=5)
+        assert r["niters"] == w["niters"] and r["normr"] == w["normr"], s
+        assert torch.equal(X, Xw), s
+        x = P.zeros()
+        P.cg_mixed(B[:, 0].contiguous(), x, max_iter=2, tolerance=0.0)
+    del P
