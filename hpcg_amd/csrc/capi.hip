@@ -198,28 +198,12 @@ int hpcgx_optimize_problem(const hpcgx_host_level *levels, int nlevels,
       const bool has_next = l + 1 < nlevels;
       if (has_next && !H.f2cOperator) throw Error(HPCGX_EINVAL, "f2cOperator missing");
       const int64_t nc = has_next ? levels[l + 1].nrow : 0;
-      // column-major ELL built on the host, slots in the reference's row order
-      std::vector<int32_t> col((size_t)kSlots * n);
-      std::vector<double> val((size_t)kSlots * n);
-      std::vector<uint8_t> nnz(n);
-      for (int64_t i = 0; i < n; ++i) {
-        const int m = H.nonzerosInRow[i];
-        if (m < 1 || m > kSlots) throw Error(HPCGX_EUNSUP, "row with 0 or more than 27 entries");
-        nnz[i] = (uint8_t)m;
-        const int *ci = H.mtxIndL[i];
-        const double *vi = H.matrixValues[i];
-        for (int j = 0; j < kSlots; ++j) {
-          col[(size_t)j * n + i] = j < m ? ci[j] : (int32_t)i;
-          val[(size_t)j * n + i] = j < m ? vi[j] : 0.0;
-        }
-      }
       nat[l].alloc(n, nc);
       nat[l].N.nx = H.nx; nat[l].N.ny = H.ny; nat[l].N.nz = H.nz;
       if (H.nx > 0 && H.ny > 0 && H.nz > 0 && (int64_t)H.nx * H.ny * H.nz == n)
         nat[l].N.parts = {hpcgx::GridPart{0, H.nx, H.ny, H.nz}};
-      HPCGX_HIP(hipMemcpy(nat[l].N.col, col.data(), sizeof(int32_t) * col.size(), hipMemcpyHostToDevice));
-      HPCGX_HIP(hipMemcpy(nat[l].N.val, val.data(), sizeof(double) * val.size(), hipMemcpyHostToDevice));
-      HPCGX_HIP(hipMemcpy(nat[l].N.nnz_row, nnz.data(), n, hipMemcpyHostToDevice));
+      // column-major ELL on the device, slots in the reference's row order
+      hpcgx::upload_host_level(H, nat[l].N, s);
       if (has_next) {
         for (int64_t c = 0; c < nc; ++c)
           if (H.f2cOperator[c] < 0 || H.f2cOperator[c] >= n)

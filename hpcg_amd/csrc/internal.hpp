@@ -370,6 +370,10 @@ void build_chunks(Problem &P, Level &L, bool interleave);
 void drop_wave(Problem &P, Level &L);
 void problem_vectors(const Problem &P, double *b, double *x, double *xexact,
                      hipStream_t s);
+// ---- host levels (hostload.hip) ----
+// N's col / val / nnz_row (allocated for H.nrow rows) from the reference's
+// row-pointer storage; throws on a row with 0 or more than 27 entries
+void upload_host_level(const hpcgx_host_level &H, NaturalLevel &N, hipStream_t s);
 
 // ---- mixed precision (mixed.hip) ----
 struct TempBufD {  // a temporary device buffer freed on scope exit

@@ -81,14 +81,17 @@ class Problem:
         return cls(h)
 
     @classmethod
-    def from_reference_levels(cls, levels, keep_schedule=False, band_lines=0):
+    def from_reference_levels(cls, levels, keep_schedule=False, band_lines=0,
+                              reversed_storage=False):
         """OptimizeProblem from host matrices in the reference's layout.
 
         ``levels`` is a list of dicts with numpy arrays ``nnz_row`` (n),
         ``cols`` (n*27, row i in [27i, 27i+27)), ``vals`` (n*27),
         ``diag_slot`` (n), optional ``f2c`` and dims ``nx, ny, nz``.  The
         arrays-of-row-pointers the reference's SparseMatrix holds are built
-        over them.
+        over them.  reversed_storage: keep row i's entries in block n-1-i, so
+        the row pointers are not one block in row order (as with rows
+        allocated one by one).
         """
         keep = []
         arr = (HostLevel * len(levels))()
@@ -98,6 +101,10 @@ class Problem:
             cols = np.ascontiguousarray(m["cols"], dtype=np.int32)
             vals = np.ascontiguousarray(m["vals"], dtype=np.float64)
             rows = np.arange(n, dtype=np.int64)
+            if reversed_storage:
+                cols = np.ascontiguousarray(cols.reshape(n, 27)[::-1]).reshape(-1)
+                vals = np.ascontiguousarray(vals.reshape(n, 27)[::-1]).reshape(-1)
+                rows = rows[::-1].copy()
             ind_ptr = (cols.ctypes.data + 4 * 27 * rows).astype(np.uint64)
             val_ptr = (vals.ctypes.data + 8 * 27 * rows).astype(np.uint64)
             dia_ptr = (vals.ctypes.data + 8 * (27 * rows + m["diag_slot"].astype(np.int64))

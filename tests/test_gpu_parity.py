@@ -87,6 +87,30 @@ def test_schedules_match_oracle(hp, dims):
         assert np.array_equal(P.schedule(l, "color"), col)
 
 
+def test_host_path_rows_out_of_order(hp, torch, g16):
+    """Rows whose storage is not one block in row order (row i's entries in
+    block n-1-i) are gathered row by row: the same problem as the generator's."""
+    say("host path, rows out of order")
+    levels = []
+    for l in range(4):
+        d = dict(nnz_row=g16["L%d_nnz_row" % l], cols=g16["L%d_cols" % l],
+                 vals=g16["L%d_vals" % l], diag_slot=g16["L%d_diag_slot" % l])
+        if l < 3:
+            d["f2c"] = g16["L%d_f2c" % l]
+        levels.append(d)
+    Ph = hp.Problem.from_reference_levels(levels, keep_schedule=True, reversed_storage=True)
+    Pg = problem(hp, (16, 16, 16))
+    for l in range(4):
+        assert np.array_equal(Ph.schedule(l, "color"), Pg.schedule(l, "color"))
+    x = Pg.to_device_order(g16["u1"])
+    y = Pg.zeros()
+    Ph.spmv(x, y)
+    assert np.array_equal(Ph.to_numpy(y), g16["spmv_u1"])
+    z = Ph.zeros()
+    Ph.mg(Ph.to_device_order(g16["u1"]), z, ordering="lexicographic")
+    assert np.array_equal(Ph.to_numpy(z), g16["mg_u1"])
+
+
 def test_schedule_proposal_checked(hp):
     """Levels with grid dimensions propose the schedule from the grid; wrong
     dimensions of the same row count must fail the row-by-row check, and the
