@@ -2,6 +2,8 @@
 // the C++ surface.
 #include "DeviceProblem.hpp"
 
+#include <algorithm>
+
 #include <hip/hip_runtime_api.h>
 
 #include <chrono>
@@ -26,6 +28,10 @@ void free_context(DeviceContext *c) {
   if (c->stage) (void)hipFree(c->stage);
   for (double *v : c->vec)
     if (v) (void)hipFree(v);
+  for (int k = 0; k < 2; ++k) {
+    if (c->pinned[k]) (void)hipHostFree(c->pinned[k]);
+    if (c->copied[k]) (void)hipEventDestroy(c->copied[k]);
+  }
   if (c->problem) (void)hpcgx_problem_destroy(c->problem);
   if (g_current == c) g_current = nullptr;
   delete c;
@@ -77,6 +83,11 @@ DeviceContext *EnsureDeviceProblem(SparseMatrix &A) {
   const size_t bytes = sizeof(double) * (size_t)c->nrow[0];
   hipError_t e = hipMalloc(&c->stage, bytes);
   for (int k = 0; k < 4 && e == hipSuccess; ++k) e = hipMalloc(&c->vec[k], bytes);
+  for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+    e = hipHostMalloc(reinterpret_cast<void **>(&c->pinned[k]), sizeof(double) * kCopyChunk,
+                      hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->copied[k], hipEventDisableTiming);
+  }
   if (e != hipSuccess) {
     hip_report("OptimizeProblem: scratch vectors", e);
     free_context(c);
@@ -113,9 +124,23 @@ int Upload(const DeviceLevel &L, const Vector &v, double *dev) {
                  v.localLength, (long long)n);
     return HPCGX_EINVAL;
   }
-  hipError_t e = hipMemcpy(L.ctx->stage, v.values, sizeof(double) * n, hipMemcpyHostToDevice);
+  DeviceContext &c = *L.ctx;
+  hipError_t e = hipSuccess;
+  for (int64_t off = 0, k = 0; off < n && e == hipSuccess; off += kCopyChunk, ++k) {
+    const int q = (int)(k & 1);
+    const int64_t m = std::min<int64_t>(kCopyChunk, n - off);
+    // buffer q free again: its last copy (in this call or an earlier one) is done
+    e = hipEventSynchronize(c.copied[q]);
+    if (e != hipSuccess) break;
+    double *dst = c.pinned[q];
+    const double *src = v.values + off;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < m; ++i) dst[i] = src[i];
+    e = hipMemcpyAsync(c.stage + off, dst, sizeof(double) * m, hipMemcpyHostToDevice, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(c.copied[q], nullptr);
+  }
   if (e != hipSuccess) return hip_report("host to device copy", e);
-  int rc = hpcgx_to_device_order(L.ctx->problem, L.level, L.ctx->stage, dev, nullptr);
+  int rc = hpcgx_to_device_order(c.problem, L.level, c.stage, dev, nullptr);
   return rc ? Report("to_device_order", rc) : 0;
 }
 
@@ -126,9 +151,30 @@ int Download(const DeviceLevel &L, const double *dev, Vector &v) {
                  v.localLength, (long long)n);
     return HPCGX_EINVAL;
   }
-  int rc = hpcgx_to_natural_order(L.ctx->problem, L.level, dev, L.ctx->stage, nullptr);
+  DeviceContext &c = *L.ctx;
+  int rc = hpcgx_to_natural_order(c.problem, L.level, dev, c.stage, nullptr);
   if (rc) return Report("to_natural_order", rc);
-  hipError_t e = hipMemcpy(v.values, L.ctx->stage, sizeof(double) * n, hipMemcpyDeviceToHost);
+  // chunk k+1 crosses PCIe while the host unpacks chunk k
+  const int64_t chunks = (n + kCopyChunk - 1) / kCopyChunk;
+  auto issue = [&](int64_t k) {
+    const int q = (int)(k & 1);
+    const int64_t off = k * kCopyChunk, m = std::min<int64_t>(kCopyChunk, n - off);
+    hipError_t r = hipMemcpyAsync(c.pinned[q], c.stage + off, sizeof(double) * m,
+                                  hipMemcpyDeviceToHost, nullptr);
+    return r == hipSuccess ? hipEventRecord(c.copied[q], nullptr) : r;
+  };
+  hipError_t e = chunks > 0 ? issue(0) : hipSuccess;
+  for (int64_t k = 0; k < chunks && e == hipSuccess; ++k) {
+    const int q = (int)(k & 1);
+    e = hipEventSynchronize(c.copied[q]);
+    if (e == hipSuccess && k + 1 < chunks) e = issue(k + 1);
+    if (e != hipSuccess) break;
+    const int64_t off = k * kCopyChunk, m = std::min<int64_t>(kCopyChunk, n - off);
+    const double *src = c.pinned[q];
+    double *dst = v.values + off;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < m; ++i) dst[i] = src[i];
+  }
   return e == hipSuccess ? 0 : hip_report("device to host copy", e);
 }
 

@@ -11,6 +11,8 @@
 #ifndef HPCG_AMD_DEVICEPROBLEM_HPP
 #define HPCG_AMD_DEVICEPROBLEM_HPP
 
+#include <hip/hip_runtime_api.h>
+
 #include <cstdint>
 
 #include "../../include/hpcg_amd.h"
@@ -35,7 +37,13 @@ struct DeviceContext {
   double *stage;         // natural-order staging, nrow[0] doubles
   double *vec[4];        // device-order scratch vectors, nrow[0] doubles each
   double build_seconds;  // wall time of hpcgx_optimize_problem + scratch
+  double *pinned[2];     // pinned host staging of the vector copies (kCopyChunk doubles each)
+  hipEvent_t copied[2];  // a staging buffer's last copy has finished
 };
+
+// Vector copies move in chunks of this many doubles through two pinned
+// buffers: host threads fill one while the other crosses PCIe.
+constexpr int64_t kCopyChunk = (int64_t)1 << 23;
 
 // Row orderings of the Gauss-Seidel sweeps: the optimized functions use
 // the multicolour schedule; the reference-ordering functions of the driver
