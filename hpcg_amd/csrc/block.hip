@@ -184,20 +184,15 @@ __device__ inline void slots32(const Col16 &C, const uint32_t *__restrict__ vc,
   }
 }
 
-// One colour step, rows [begin, end) (begin a multiple of kTile when C16):
-// mixed.hip's gs32_row for each column.
+// Row i, columns [o, o + 4): mixed.hip's gs32_row for each column.
 template <bool STREAM, int S, bool C16>
-__global__ __launch_bounds__(kBlock) void k_bgs32_rows(int64_t begin, int64_t end, int64_t n,
-                                                       Col16 C, const uint32_t *__restrict__ vc,
-                                                       const int32_t *__restrict__ col,
-                                                       const half_t *__restrict__ val16,
-                                                       const float *__restrict__ diag32,
-                                                       const float *__restrict__ R, float *X) {
-  constexpr int V = 4, W = S / V;
-  const int64_t g = gtid();
-  const int64_t i = begin + g / W;
-  const int o = (int)(g % W) * V;
-  if (i >= end) return;
+__device__ inline void bgs32_row(int64_t i, int o, int64_t n, const Col16 &C,
+                                 const uint32_t *__restrict__ vc,
+                                 const int32_t *__restrict__ col,
+                                 const half_t *__restrict__ val16,
+                                 const float *__restrict__ diag32,
+                                 const float *__restrict__ R, float *X) {
+  constexpr int V = 4;
   float sum[V], xv[V];
   ldv<V>(R + i * S + o, sum);
   slots32<STREAM, C16>(C, vc, col, val16, n, i, [&](float a, int32_t c) {
@@ -211,12 +206,50 @@ __global__ __launch_bounds__(kBlock) void k_bgs32_rows(int64_t begin, int64_t en
   stv<V>(X + i * S + o, sum);
 }
 
+// One colour step, rows [begin, end) (begin a multiple of kTile when C16).
+template <bool STREAM, int S, bool C16>
+__global__ __launch_bounds__(kBlock) void k_bgs32_rows(int64_t begin, int64_t end, int64_t n,
+                                                       Col16 C, const uint32_t *__restrict__ vc,
+                                                       const int32_t *__restrict__ col,
+                                                       const half_t *__restrict__ val16,
+                                                       const float *__restrict__ diag32,
+                                                       const float *__restrict__ R, float *X) {
+  constexpr int W = S / 4;
+  const int64_t g = gtid();
+  const int64_t i = begin + g / W;
+  if (i >= end) return;
+  bgs32_row<STREAM, S, C16>(i, (int)(g % W) * 4, n, C, vc, col, val16, diag32, R, X);
+}
+
+// One colour of the block sweeps (Level::gs_rows): the W lanes of block t
+// update rows begin + q nb + t for q = 0 .. kGsBlockRows-1 (reversed when
+// BWD).  With C16, begin and nb are multiples of kTile.
+template <bool STREAM, int S, bool C16, bool BWD>
+__global__ __launch_bounds__(kBlock) void k_bgs32_blocks(int64_t begin, int64_t nb, int64_t n,
+                                                         Col16 C,
+                                                         const uint32_t *__restrict__ vc,
+                                                         const int32_t *__restrict__ col,
+                                                         const half_t *__restrict__ val16,
+                                                         const float *__restrict__ diag32,
+                                                         const float *__restrict__ R, float *X) {
+  constexpr int W = S / 4;
+  const int64_t g = gtid();
+  const int64_t t = g / W;
+  if (t >= nb) return;
+  const int o = (int)(g % W) * 4;
+#pragma unroll 1
+  for (int q = 0; q < kGsBlockRows; ++q) {
+    const int64_t i = begin + (int64_t)(BWD ? kGsBlockRows - 1 - q : q) * nb + t;
+    bgs32_row<STREAM, S, C16>(i, o, n, C, vc, col, val16, diag32, R, X);
+  }
+}
+
 // rc = rf[f2c] - (A xf)[f2c] in fine order (Level::rperm): mixed.hip's
 // row_dot32 for each column.
 template <bool STREAM, int S, bool C16>
 __global__ __launch_bounds__(kBlock) void k_bresid_restrict32_rows(
-    int64_t nc, int64_t rf0, const int32_t *__restrict__ rperm, int64_t n, Col16 C,
-    const uint32_t *__restrict__ vc, const int32_t *__restrict__ col,
+    int64_t nc, const int32_t *__restrict__ ftile, const int32_t *__restrict__ rperm, int64_t n,
+    Col16 C, const uint32_t *__restrict__ vc, const int32_t *__restrict__ col,
     const half_t *__restrict__ val16, const float *__restrict__ diag32,
     const float *__restrict__ RF, const float *__restrict__ XF, float *__restrict__ RC) {
   constexpr int V = 4, W = S / V;
@@ -224,7 +257,7 @@ __global__ __launch_bounds__(kBlock) void k_bresid_restrict32_rows(
   const int64_t t = g / W;
   const int o = (int)(g % W) * V;
   if (t >= nc) return;
-  const int64_t f = rf0 + t;
+  const int64_t f = fine_row(ftile, t);
   float a[V], xv[V];
 #pragma unroll
   for (int k = 0; k < V; ++k) a[k] = 0.0f;
@@ -246,7 +279,8 @@ __global__ __launch_bounds__(kBlock) void k_bresid_restrict32_rows(
 
 // xf[f2c] += xc in fine order
 template <int S>
-__global__ __launch_bounds__(kBlock) void k_bprolong32_rows(int64_t nc, int64_t rf0,
+__global__ __launch_bounds__(kBlock) void k_bprolong32_rows(int64_t nc,
+                                                            const int32_t *__restrict__ ftile,
                                                             const int32_t *__restrict__ rperm,
                                                             const float *__restrict__ XC,
                                                             float *__restrict__ XF) {
@@ -255,12 +289,13 @@ __global__ __launch_bounds__(kBlock) void k_bprolong32_rows(int64_t nc, int64_t 
   const int64_t t = g / W;
   const int o = (int)(g % W) * V;
   if (t >= nc) return;
+  const int64_t f = fine_row(ftile, t);
   float a[V], b[V];
-  ldv<V>(XF + (rf0 + t) * S + o, a);
+  ldv<V>(XF + f * S + o, a);
   ldv<V>(XC + (int64_t)rperm[t] * S + o, b);
 #pragma unroll
   for (int k = 0; k < V; ++k) a[k] += b[k];
-  stv<V>(XF + (rf0 + t) * S + o, a);
+  stv<V>(XF + f * S + o, a);
 }
 
 // Y = A X in FP64 (ComputeSPMV_ref.cpp:62-69's order for each column) and,
@@ -539,24 +574,59 @@ void gs_rows(const Problem &P, const Level &L, int s, int64_t b, int64_t e, cons
   else gs_rows_s<16>(P, L, b, e, R, X, st);
 }
 
+template <int S, bool BWD>
+void gs_blocks_s(const Problem &P, const Level &L, int64_t b, int64_t nb, const float *R,
+                 float *X, hipStream_t st) {
+  const Col16 C = col16_of(P, L);
+  const bool c16 = L.vc && use_col16(P, L, b) && nb % kTile == 0;
+  const unsigned g = grid_for(nb * (S / 4));
+  if (L.stream && c16)
+    k_bgs32_blocks<true, S, true, BWD><<<g, kBlock, 0, st>>>(b, nb, L.n, C, L.vc, L.col,
+                                                             L.val16, L.diag32, R, X);
+  else if (L.stream)
+    k_bgs32_blocks<true, S, false, BWD><<<g, kBlock, 0, st>>>(b, nb, L.n, C, L.vc, L.col,
+                                                              L.val16, L.diag32, R, X);
+  else if (c16)
+    k_bgs32_blocks<false, S, true, BWD><<<g, kBlock, 0, st>>>(b, nb, L.n, C, L.vc, L.col,
+                                                              L.val16, L.diag32, R, X);
+  else
+    k_bgs32_blocks<false, S, false, BWD><<<g, kBlock, 0, st>>>(b, nb, L.n, C, L.vc, L.col,
+                                                               L.val16, L.diag32, R, X);
+  HPCGX_LAUNCHED();
+}
+
+// Colour c of a block sweep (Level::gs_rows > 1) with the W-lanes kernels.
+void gs_blocks(const Problem &P, const Level &L, int s, int c, bool bwd, const float *R,
+               float *X, hipStream_t st) {
+  const int64_t b = L.color_ptr[c], e = L.color_ptr[c + 1];
+  if (e <= b) return;
+  const int64_t nb = (e - b) / L.gs_rows;
+  if (s == 4) bwd ? gs_blocks_s<4, true>(P, L, b, nb, R, X, st)
+                  : gs_blocks_s<4, false>(P, L, b, nb, R, X, st);
+  else if (s == 8) bwd ? gs_blocks_s<8, true>(P, L, b, nb, R, X, st)
+                       : gs_blocks_s<8, false>(P, L, b, nb, R, X, st);
+  else bwd ? gs_blocks_s<16, true>(P, L, b, nb, R, X, st)
+           : gs_blocks_s<16, false>(P, L, b, nb, R, X, st);
+}
+
 template <int S>
 void restrict_rows_s(const Problem &P, const Level &L, const float *RF, const float *XF,
                      float *RC, hipStream_t st) {
   const Col16 C = col16_of(P, L);
-  const bool c16 = L.vc && use_col16(P, L, L.rf0);
+  const bool c16 = L.vc && use_col16(P, L, 0);
   const unsigned g = grid_for(L.nc * (S / 4));
   if (L.stream && c16)
     k_bresid_restrict32_rows<true, S, true><<<g, kBlock, 0, st>>>(
-        L.nc, L.rf0, L.rperm, L.n, C, L.vc, L.col, L.val16, L.diag32, RF, XF, RC);
+        L.nc, L.ftile, L.rperm, L.n, C, L.vc, L.col, L.val16, L.diag32, RF, XF, RC);
   else if (L.stream)
     k_bresid_restrict32_rows<true, S, false><<<g, kBlock, 0, st>>>(
-        L.nc, L.rf0, L.rperm, L.n, C, L.vc, L.col, L.val16, L.diag32, RF, XF, RC);
+        L.nc, L.ftile, L.rperm, L.n, C, L.vc, L.col, L.val16, L.diag32, RF, XF, RC);
   else if (c16)
     k_bresid_restrict32_rows<false, S, true><<<g, kBlock, 0, st>>>(
-        L.nc, L.rf0, L.rperm, L.n, C, L.vc, L.col, L.val16, L.diag32, RF, XF, RC);
+        L.nc, L.ftile, L.rperm, L.n, C, L.vc, L.col, L.val16, L.diag32, RF, XF, RC);
   else
     k_bresid_restrict32_rows<false, S, false><<<g, kBlock, 0, st>>>(
-        L.nc, L.rf0, L.rperm, L.n, C, L.vc, L.col, L.val16, L.diag32, RF, XF, RC);
+        L.nc, L.ftile, L.rperm, L.n, C, L.vc, L.col, L.val16, L.diag32, RF, XF, RC);
   HPCGX_LAUNCHED();
 }
 
@@ -570,11 +640,11 @@ void restrict_rows(const Problem &P, const Level &L, int s, const float *RF, con
 void prolong_rows(const Level &L, int s, const float *XC, float *XF, hipStream_t st) {
   const unsigned g = grid_for(L.nc * (s / 4));
   if (s == 4)
-    k_bprolong32_rows<4><<<g, kBlock, 0, st>>>(L.nc, L.rf0, L.rperm, XC, XF);
+    k_bprolong32_rows<4><<<g, kBlock, 0, st>>>(L.nc, L.ftile, L.rperm, XC, XF);
   else if (s == 8)
-    k_bprolong32_rows<8><<<g, kBlock, 0, st>>>(L.nc, L.rf0, L.rperm, XC, XF);
+    k_bprolong32_rows<8><<<g, kBlock, 0, st>>>(L.nc, L.ftile, L.rperm, XC, XF);
   else
-    k_bprolong32_rows<16><<<g, kBlock, 0, st>>>(L.nc, L.rf0, L.rperm, XC, XF);
+    k_bprolong32_rows<16><<<g, kBlock, 0, st>>>(L.nc, L.ftile, L.rperm, XC, XF);
   HPCGX_LAUNCHED();
 }
 
@@ -613,7 +683,11 @@ void spmv_rows(const Problem &P, const Level &L, int s, const double *X, double 
 void bsymgs32(Problem &P, int l, int ordering, int s, const float *R, float *X,
               hipStream_t st) {
   const Level &L = P.L[l];
-  if (ordering == HPCGX_ORDER_MULTICOLOR && rows_kernels(s)) {
+  if (ordering == HPCGX_ORDER_MULTICOLOR && rows_kernels(s) && L.gs_rows > 1) {
+    for (int pass = 0; pass < 2; ++pass)
+      for (int q = 0; q < L.ncolors; ++q)
+        gs_blocks(P, L, s, pass == 0 ? q : L.ncolors - 1 - q, pass == 1, R, X, st);
+  } else if (ordering == HPCGX_ORDER_MULTICOLOR && rows_kernels(s)) {
     for (int pass = 0; pass < 2; ++pass)
       for (int q = 0; q < L.ncolors; ++q) {
         const int c = pass == 0 ? q : L.ncolors - 1 - q;
@@ -621,17 +695,24 @@ void bsymgs32(Problem &P, int l, int ordering, int s, const float *R, float *X,
         if (e > b) gs_rows(P, L, s, b, e, R, X, st);
       }
   } else if (ordering == HPCGX_ORDER_MULTICOLOR) {
+    // block sweeps: a block colour's runs one after another (each run's rows
+    // lie in different blocks, so they are independent)
+    const int runs = L.gs_rows;
     for (int pass = 0; pass < 2; ++pass)
       for (int q = 0; q < L.ncolors; ++q) {
         const int c = pass == 0 ? q : L.ncolors - 1 - q;
-        const int64_t b = L.color_ptr[c], e = L.color_ptr[c + 1];
-        if (e <= b) continue;
-        const unsigned g = grid_for((e - b) * s);
-        if (L.stream)
-          k_bgs32<true><<<g, kBlock, 0, st>>>(b, e, L.n, s, L.col, L.val16, L.diag32, R, X);
-        else
-          k_bgs32<false><<<g, kBlock, 0, st>>>(b, e, L.n, s, L.col, L.val16, L.diag32, R, X);
-        HPCGX_LAUNCHED();
+        const int64_t b0 = L.color_ptr[c], e0 = L.color_ptr[c + 1];
+        if (e0 <= b0) continue;
+        const int64_t nr = (e0 - b0) / runs;
+        for (int k = 0; k < runs; ++k) {
+          const int64_t b = b0 + (pass == 0 ? k : runs - 1 - k) * nr, e = b + nr;
+          const unsigned g = grid_for((e - b) * s);
+          if (L.stream)
+            k_bgs32<true><<<g, kBlock, 0, st>>>(b, e, L.n, s, L.col, L.val16, L.diag32, R, X);
+          else
+            k_bgs32<false><<<g, kBlock, 0, st>>>(b, e, L.n, s, L.col, L.val16, L.diag32, R, X);
+          HPCGX_LAUNCHED();
+        }
       }
   } else {
     for (int pass = 0; pass < 2; ++pass)

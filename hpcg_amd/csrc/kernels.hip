@@ -254,6 +254,29 @@ __global__ __launch_bounds__(kBlock) void k_gs_range16(
   if (i < end) gs_row16<STREAM>(C, i, n, col, val, r, x);
 }
 
+// One colour of the block sweeps (Level::gs_rows = kGsBlockRows): work-item
+// t updates rows begin + q nb + t for q = 0 .. kGsBlockRows-1 in order
+// (reverse order when BWD), the rows of one block.  Rows of different blocks
+// of one colour share no matrix entry, so the blocks are independent and
+// each block's rows see one another's updates in sweep order.  With C16,
+// begin and nb are multiples of kTile, so a wavefront's rows of one q lie
+// in one tile.
+template <bool STREAM, bool FINEST, bool BWD, bool C16>
+__global__ __launch_bounds__(kBlock) void k_gs_blocks(
+    int64_t begin, int64_t nb, int64_t n, Col16 C, const int32_t *__restrict__ col,
+    const double *__restrict__ val, const double *__restrict__ r, double *x, bool xcd) {
+  const int64_t t = xcd_block(blockIdx.x, gridDim.x, xcd) * kBlock + threadIdx.x;
+  if (t >= nb) return;
+#pragma unroll 1
+  for (int q = 0; q < kGsBlockRows; ++q) {
+    const int64_t i = begin + (int64_t)(BWD ? kGsBlockRows - 1 - q : q) * nb + t;
+    if (C16)
+      gs_row16<STREAM>(C, i, n, col, val, r, x);
+    else
+      gs_row<STREAM>(i, n, col, val, r, x);
+  }
+}
+
 // Two adjacent rows per thread: the matrix is read with 16-byte (values) and
 // 8-byte (column ids) loads per lane.  Needs an even level size and an even
 // range start, so that slot j of the pair is 16-byte aligned.
@@ -429,29 +452,30 @@ __global__ __launch_bounds__(kBlock) void k_resid_restrict(
   }
 }
 
-// The same in fine order (Level::rperm): thread t computes fine row
-// rf0 + t, so a wavefront reads whole tiles of the matrix (narrow column ids
-// when C16), and stores the coarse row rperm[t].
+// The same in fine order (Level::ftile, rperm): thread t computes fine row
+// fine_row(ftile, t), so a wavefront reads whole tiles of the matrix (narrow
+// column ids when C16), and stores the coarse row rperm[t].
 template <bool STREAM, bool C16>
 __global__ __launch_bounds__(kBlock) void k_resid_restrict_f(
-    int64_t nc, int64_t rf0, const int32_t *__restrict__ rperm, int64_t n, Col16 C,
-    const int32_t *__restrict__ col, const double *__restrict__ val,
+    int64_t nc, const int32_t *__restrict__ ftile, const int32_t *__restrict__ rperm, int64_t n,
+    Col16 C, const int32_t *__restrict__ col, const double *__restrict__ val,
     const double *__restrict__ rf, const double *__restrict__ xf, double *__restrict__ rc) {
   const int64_t t = gtid();
   if (t >= nc) return;
-  const int64_t f = rf0 + t;
+  const int64_t f = fine_row(ftile, t);
   const double ax = C16 ? row_dot16<STREAM>(C, col, val, xf, n, f)
                         : row_dot<STREAM>(col, val, xf, n, f);
   rc[rperm[t]] = rf[f] - ax;
 }
 
 // ComputeProlongation_ref.cpp:49 (f2c is injective: no write conflicts)
-__global__ __launch_bounds__(kBlock) void k_prolong_f(int64_t nc, int64_t rf0,
+__global__ __launch_bounds__(kBlock) void k_prolong_f(int64_t nc,
+                                                      const int32_t *__restrict__ ftile,
                                                       const int32_t *__restrict__ rperm,
                                                       const double *__restrict__ xc,
                                                       double *__restrict__ xf) {
   const int64_t t = gtid();
-  if (t < nc) xf[rf0 + t] += xc[rperm[t]];
+  if (t < nc) xf[fine_row(ftile, t)] += xc[rperm[t]];
 }
 
 __global__ __launch_bounds__(kBlock) void k_prolong(
@@ -837,6 +861,41 @@ void launch_symgs_rows(const Problem &P, const Level &L, int64_t begin, int64_t 
   HPCGX_LAUNCHED();
 }
 
+namespace {
+template <bool BWD, bool C16>
+void gs_blocks(const Problem &P, const Level &L, int64_t b, int64_t nb, const double *r,
+               double *x, hipStream_t s) {
+  const Col16 C = col16_of(P, L);
+  const unsigned g = grid_for(nb);
+  if (L.finest)
+    k_gs_blocks<true, true, BWD, C16><<<g, kBlock, 0, s>>>(b, nb, L.n, C, L.col, L.val, r, x,
+                                                           L.xcd);
+  else if (L.stream)
+    k_gs_blocks<true, false, BWD, C16><<<g, kBlock, 0, s>>>(b, nb, L.n, C, L.col, L.val, r, x,
+                                                            L.xcd);
+  else
+    k_gs_blocks<false, false, BWD, C16><<<g, kBlock, 0, s>>>(b, nb, L.n, C, L.col, L.val, r,
+                                                             x, L.xcd);
+  HPCGX_LAUNCHED();
+}
+}  // namespace
+
+void launch_symgs_colour(const Problem &P, const Level &L, int c, bool bwd, const double *r,
+                         double *x, hipStream_t s) {
+  const int64_t b = L.color_ptr[c], e = L.color_ptr[c + 1];
+  if (e <= b) return;
+  if (L.gs_rows == 1) {
+    launch_symgs_rows(P, L, b, e, r, x, s);
+    return;
+  }
+  const int64_t nb = (e - b) / L.gs_rows;
+  const bool c16 = use_col16(P, L, b) && nb % kTile == 0;
+  if (bwd && c16) gs_blocks<true, true>(P, L, b, nb, r, x, s);
+  else if (bwd) gs_blocks<true, false>(P, L, b, nb, r, x, s);
+  else if (c16) gs_blocks<false, true>(P, L, b, nb, r, x, s);
+  else gs_blocks<false, false>(P, L, b, nb, r, x, s);
+}
+
 void launch_symgs_list(const Level &L, const int32_t *rows, int64_t count,
                        const double *r, double *x, hipStream_t s) {
   if (count <= 0) return;
@@ -855,19 +914,19 @@ void launch_residual_restriction(const Problem &P, const Level &L, const double 
                                  const double *xf, double *rc, hipStream_t s) {
   if (L.rperm) {
     const Col16 C = col16_of(P, L);
-    const bool c16 = use_col16(P, L, L.rf0);
+    const bool c16 = use_col16(P, L, 0);
     const unsigned g = grid_for(L.nc);
     if (L.stream && c16)
-      k_resid_restrict_f<true, true><<<g, kBlock, 0, s>>>(L.nc, L.rf0, L.rperm, L.n, C, L.col,
+      k_resid_restrict_f<true, true><<<g, kBlock, 0, s>>>(L.nc, L.ftile, L.rperm, L.n, C, L.col,
                                                           L.val, rf, xf, rc);
     else if (L.stream)
-      k_resid_restrict_f<true, false><<<g, kBlock, 0, s>>>(L.nc, L.rf0, L.rperm, L.n, C, L.col,
+      k_resid_restrict_f<true, false><<<g, kBlock, 0, s>>>(L.nc, L.ftile, L.rperm, L.n, C, L.col,
                                                            L.val, rf, xf, rc);
     else if (c16)
-      k_resid_restrict_f<false, true><<<g, kBlock, 0, s>>>(L.nc, L.rf0, L.rperm, L.n, C, L.col,
+      k_resid_restrict_f<false, true><<<g, kBlock, 0, s>>>(L.nc, L.ftile, L.rperm, L.n, C, L.col,
                                                            L.val, rf, xf, rc);
     else
-      k_resid_restrict_f<false, false><<<g, kBlock, 0, s>>>(L.nc, L.rf0, L.rperm, L.n, C,
+      k_resid_restrict_f<false, false><<<g, kBlock, 0, s>>>(L.nc, L.ftile, L.rperm, L.n, C,
                                                             L.col, L.val, rf, xf, rc);
     HPCGX_LAUNCHED();
     return;
@@ -884,7 +943,7 @@ void launch_residual_restriction(const Problem &P, const Level &L, const double 
 void launch_prolongation(const Level &L, const double *xc, double *xf,
                          hipStream_t s) {
   if (L.rperm) {
-    k_prolong_f<<<grid_for(L.nc), kBlock, 0, s>>>(L.nc, L.rf0, L.rperm, xc, xf);
+    k_prolong_f<<<grid_for(L.nc), kBlock, 0, s>>>(L.nc, L.ftile, L.rperm, xc, xf);
     HPCGX_LAUNCHED();
     return;
   }

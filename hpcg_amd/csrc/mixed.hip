@@ -272,6 +272,31 @@ __global__ __launch_bounds__(kBlock) void k_gs32_range8(
   if (i < end) gs32_row8<STREAM>(C, vp, vc, i, n, col, val16, diag32, r, x);
 }
 
+// One colour of the block sweeps (Level::gs_rows): work-item t updates rows
+// begin + q nb + t for q = 0 .. kGsBlockRows-1 (reversed when BWD), the
+// rows of one block, as kernels.hip's k_gs_blocks does in FP64.  K: 2 = FP16
+// value pairs (gs32_row8), 1 = packed words (gs32_row16), 0 = the FP16 array
+// and 32-bit ids; K > 0 needs begin and nb multiples of kTile.
+template <bool STREAM, bool FINEST, bool BWD, int K>
+__global__ __launch_bounds__(kBlock) void k_gs32_blocks(
+    int64_t begin, int64_t nb, int64_t n, Col16 C, const uint32_t *__restrict__ vp,
+    const uint32_t *__restrict__ vc, const int32_t *__restrict__ col,
+    const half_t *__restrict__ val16, const float *__restrict__ diag32,
+    const float *__restrict__ r, float *x, bool xcd) {
+  const int64_t t = xcd_block(blockIdx.x, gridDim.x, xcd) * kBlock + threadIdx.x;
+  if (t >= nb) return;
+#pragma unroll 1
+  for (int q = 0; q < kGsBlockRows; ++q) {
+    const int64_t i = begin + (int64_t)(BWD ? kGsBlockRows - 1 - q : q) * nb + t;
+    if (K == 2)
+      gs32_row8<STREAM>(C, vp, vc, i, n, col, val16, diag32, r, x);
+    else if (K == 1)
+      gs32_row16<STREAM>(C, vc, i, n, col, val16, diag32, r, x);
+    else
+      gs32_row<STREAM>(i, n, col, val16, diag32, r, x);
+  }
+}
+
 // vc[k] = col16[k] << 16 | bits of val16[k], one thread per row (a launch
 // holds fewer than 2^32 work-items; 27 n can exceed that).
 __global__ __launch_bounds__(kBlock) void k_pack_vc(int64_t n, const uint16_t *__restrict__ off,
@@ -355,31 +380,33 @@ __global__ __launch_bounds__(kBlock) void k_prolong32(
   if (c < nc) xf[f2c[c]] += xc[c];
 }
 
-// The transfers in fine order (Level::rperm): thread t owns fine row rf0 + t
-// (whole tiles per wavefront, packed words when C16) and coarse row rperm[t].
+// The transfers in fine order (Level::ftile, rperm): thread t owns fine row
+// fine_row(ftile, t) (whole tiles per wavefront, packed words when C16) and
+// coarse row rperm[t].
 // C16 = 2: FP16 value pairs on 8-bit tiles (row_dot32_8).
 template <bool STREAM, int C16>
 __global__ __launch_bounds__(kBlock) void k_resid_restrict32_f(
-    int64_t nc, int64_t rf0, const int32_t *__restrict__ rperm, int64_t n, Col16 C,
-    const uint32_t *__restrict__ vc, const uint32_t *__restrict__ vp,
+    int64_t nc, const int32_t *__restrict__ ftile, const int32_t *__restrict__ rperm, int64_t n,
+    Col16 C, const uint32_t *__restrict__ vc, const uint32_t *__restrict__ vp,
     const int32_t *__restrict__ col, const half_t *__restrict__ val16,
     const float *__restrict__ diag32, const float *__restrict__ rf,
     const float *__restrict__ xf, float *__restrict__ rc) {
   const int64_t t = gtid();
   if (t >= nc) return;
-  const int64_t f = rf0 + t;
+  const int64_t f = fine_row(ftile, t);
   const float ax = C16 == 2   ? row_dot32_8<STREAM>(C, vp, vc, f, n, col, val16, diag32, xf)
                    : C16 == 1 ? row_dot32_16<STREAM>(C, vc, f, n, col, val16, diag32, xf)
                               : row_dot32<STREAM>(f, n, col, val16, diag32, xf);
   rc[rperm[t]] = rf[f] - ax;
 }
 
-__global__ __launch_bounds__(kBlock) void k_prolong32_f(int64_t nc, int64_t rf0,
+__global__ __launch_bounds__(kBlock) void k_prolong32_f(int64_t nc,
+                                                        const int32_t *__restrict__ ftile,
                                                         const int32_t *__restrict__ rperm,
                                                         const float *__restrict__ xc,
                                                         float *__restrict__ xf) {
   const int64_t t = gtid();
-  if (t < nc) xf[rf0 + t] += xc[rperm[t]];
+  if (t < nc) xf[fine_row(ftile, t)] += xc[rperm[t]];
 }
 
 void gs32_step(const Problem &P, const Level &L, int64_t begin, int64_t end,
@@ -424,6 +451,47 @@ void gs32_step(const Problem &P, const Level &L, int64_t begin, int64_t end,
     k_gs32_range<false, false><<<g, kBlock, 0, s>>>(begin, end, L.n, L.col,
                                                     L.val16, L.diag32, r, x);
   HPCGX_LAUNCHED();
+}
+
+template <bool BWD, int K>
+void gs32_blocks(const Problem &P, const Level &L, int64_t b, int64_t nb, const float *r,
+                 float *x, hipStream_t s) {
+  const Col16 C = col16_of(P, L);
+  const unsigned g = grid_for(nb);
+  if (L.finest)
+    k_gs32_blocks<true, true, BWD, K><<<g, kBlock, 0, s>>>(b, nb, L.n, C, L.v16p, L.vc, L.col,
+                                                           L.val16, L.diag32, r, x, L.xcd);
+  else if (L.stream)
+    k_gs32_blocks<true, false, BWD, K><<<g, kBlock, 0, s>>>(b, nb, L.n, C, L.v16p, L.vc, L.col,
+                                                            L.val16, L.diag32, r, x, L.xcd);
+  else
+    k_gs32_blocks<false, false, BWD, K><<<g, kBlock, 0, s>>>(b, nb, L.n, C, L.v16p, L.vc,
+                                                             L.col, L.val16, L.diag32, r, x,
+                                                             L.xcd);
+  HPCGX_LAUNCHED();
+}
+
+// Colour c of the FP32 multicolour sweep in one direction.
+void gs32_colour(const Problem &P, const Level &L, int c, bool bwd, const float *r, float *x,
+                 hipStream_t s) {
+  const int64_t b = L.color_ptr[c], e = L.color_ptr[c + 1];
+  if (e <= b) return;
+  if (L.gs_rows == 1) {
+    gs32_step(P, L, b, e, r, x, s);
+    return;
+  }
+  const int64_t nb = (e - b) / L.gs_rows;
+  const bool c16 = use_col16(P, L, b) && nb % kTile == 0 && L.vc;
+  const int k = !c16 ? 0 : (L.v16p && P.use_pairs && P.use_c8 && L.byte_tiles > 0) ? 2 : 1;
+  if (bwd) {
+    if (k == 2) gs32_blocks<true, 2>(P, L, b, nb, r, x, s);
+    else if (k == 1) gs32_blocks<true, 1>(P, L, b, nb, r, x, s);
+    else gs32_blocks<true, 0>(P, L, b, nb, r, x, s);
+  } else {
+    if (k == 2) gs32_blocks<false, 2>(P, L, b, nb, r, x, s);
+    else if (k == 1) gs32_blocks<false, 1>(P, L, b, nb, r, x, s);
+    else gs32_blocks<false, 0>(P, L, b, nb, r, x, s);
+  }
 }
 
 }  // namespace
@@ -474,10 +542,8 @@ void symgs32(Problem &P, int l, int ordering, const float *r, float *x,
              hipStream_t s) {
   const Level &L = P.L[l];
   if (ordering == HPCGX_ORDER_MULTICOLOR) {
-    for (int c = 0; c < L.ncolors; ++c)
-      gs32_step(P, L, L.color_ptr[c], L.color_ptr[c + 1], r, x, s);
-    for (int c = L.ncolors - 1; c >= 0; --c)
-      gs32_step(P, L, L.color_ptr[c], L.color_ptr[c + 1], r, x, s);
+    for (int c = 0; c < L.ncolors; ++c) gs32_colour(P, L, c, false, r, x, s);
+    for (int c = L.ncolors - 1; c >= 0; --c) gs32_colour(P, L, c, true, r, x, s);
   } else {
     for (int pass = 0; pass < 2; ++pass)
       for (int k = 0; k < L.nsched; ++k) {
@@ -523,10 +589,10 @@ void mg32(Problem &P, int l, int ordering, const float *r, float *x,
     const unsigned g = grid_for(L.nc);
     if (L.rperm) {
       const Col16 C = col16_of(P, L);
-      const bool c16 = L.vc && use_col16(P, L, L.rf0);
+      const bool c16 = L.vc && use_col16(P, L, 0);
       const bool pairs = c16 && L.v16p && P.use_pairs && P.use_c8;
 #define HPCGX_RR32(ST, K)                                                                  \
-  k_resid_restrict32_f<ST, K><<<g, kBlock, 0, s>>>(L.nc, L.rf0, L.rperm, L.n, C, L.vc, L.v16p, \
+  k_resid_restrict32_f<ST, K><<<g, kBlock, 0, s>>>(L.nc, L.ftile, L.rperm, L.n, C, L.vc, L.v16p, \
                                                    L.col, L.val16, L.diag32, r, x, L.rc32)
       if (L.stream && pairs) HPCGX_RR32(true, 2);
       else if (L.stream && c16) HPCGX_RR32(true, 1);
@@ -545,7 +611,7 @@ void mg32(Problem &P, int l, int ordering, const float *r, float *x,
     HPCGX_LAUNCHED();
     mg32(P, l + 1, ordering, L.rc32, L.xc32, s);
     if (L.rperm)
-      k_prolong32_f<<<g, kBlock, 0, s>>>(L.nc, L.rf0, L.rperm, L.xc32, x);
+      k_prolong32_f<<<g, kBlock, 0, s>>>(L.nc, L.ftile, L.rperm, L.xc32, x);
     else
       k_prolong32<<<g, kBlock, 0, s>>>(L.nc, L.f2c, L.xc32, x);
     HPCGX_LAUNCHED();

@@ -265,6 +265,49 @@ __global__ __launch_bounds__(kBlock) void k_verify_schedule(int64_t n, const int
   if ((threadIdx.x & 63) == 0) atomicMax(max_level, lv);
 }
 
+// Block sweeps on an nx*ny*nz grid with even dimensions, rows in natural
+// order: the 2x2x2 block of grid point (x, y, z) is (x/2, y/2, z/2), its
+// colour the parity of those three, and the point's position in it
+// (x & 1) + 2 (y & 1) + 4 (z & 1).  key = colour * kGsBlockRows + position.
+__device__ inline void grid_block(int64_t i, int nx, int ny, int64_t &block, int &colour,
+                                  int &pos) {
+  const int x = (int)(i % nx);
+  const int64_t t = i / nx;
+  const int y = (int)(t % ny), z = (int)(t / ny);
+  block = ((int64_t)(z >> 1) * (ny >> 1) + (y >> 1)) * (nx >> 1) + (x >> 1);
+  colour = ((x >> 1) & 1) | (((y >> 1) & 1) << 1) | (((z >> 1) & 1) << 2);
+  pos = (x & 1) | ((y & 1) << 1) | ((z & 1) << 2);
+}
+
+__global__ __launch_bounds__(kBlock) void k_block_key(int64_t n, int nx, int ny,
+                                                      int32_t *key) {
+  const int64_t i = gtid();
+  if (i >= n) return;
+  int64_t b;
+  int c, p;
+  grid_block(i, nx, ny, b, c, p);
+  key[i] = c * kGsBlockRows + p;
+}
+
+// bad = 1 when an entry couples two different blocks of one colour (the
+// block sweep of that colour would then not be Gauss-Seidel in its order).
+__global__ __launch_bounds__(kBlock) void k_verify_blocks(int64_t n, int nx, int ny,
+                                                          const int32_t *col,
+                                                          const uint8_t *nnz_row, int *bad) {
+  const int64_t i = gtid();
+  if (i >= n) return;
+  int64_t bi, bc;
+  int ci, cc, p;
+  grid_block(i, nx, ny, bi, ci, p);
+  const int m = nnz_row[i];
+  bool ok = true;
+  for (int j = 0; j < m; ++j) {
+    grid_block(col[(int64_t)j * n + i], nx, ny, bc, cc, p);
+    ok = ok && (cc != ci || bc == bi);
+  }
+  if (!ok) atomicOr(bad, 1);
+}
+
 __global__ void k_kahn_advance(int t, int64_t *lptr, const unsigned *tail) {
   lptr[t + 2] = (int64_t)*tail;
 }
@@ -698,7 +741,9 @@ void finalize_level(Problem &P, int l, NaturalLevel &N, int flags,
   L.nsched = nsched;
 
   // device order = stable sort of rows by colour; inside a colour, by band
-  // of grid lines when the level is an nx*ny*nz grid in natural order
+  // of grid lines when the level is an nx*ny*nz grid in natural order.
+  // Block sweeps: by (block colour, position in block), blocks in natural
+  // order inside each run.
   L.iperm = P.arena.alloc<int32_t>(n);
   L.perm = P.arena.alloc<int32_t>(n);
   {
@@ -706,8 +751,35 @@ void finalize_level(Problem &P, int l, NaturalLevel &N, int flags,
     if (band == 0) band = kBandLines;
     const bool grid = (int64_t)N.nx * N.ny * N.nz == n && N.nx > 0;
     const int nbands = (grid && band > 1 && N.ny >= 2 * band) ? (N.ny + band - 1) / band : 1;
+    bool blocks = (flags & HPCGX_GS_BLOCKS) && grid && N.nx % 2 == 0 && N.ny % 2 == 0 &&
+                  N.nz % 2 == 0;
+    if (blocks) {
+      TempBuf bad(sizeof(int));
+      HPCGX_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+      k_verify_blocks<<<grid_for(n), kBlock, 0, s>>>(n, N.nx, N.ny, N.col, N.nnz_row,
+                                                     bad.as<int>());
+      HPCGX_LAUNCHED();
+      int b = 1;
+      HPCGX_HIP(hipMemcpyAsync(&b, bad.p, sizeof b, hipMemcpyDeviceToHost, s));
+      HPCGX_HIP(hipStreamSynchronize(s));
+      blocks = b == 0;
+    }
     std::vector<int64_t> cp;
-    if (nbands > 1) {
+    L.gs_rows = 1;
+    if (blocks) {
+      TempBuf key(sizeof(int32_t) * n);
+      k_block_key<<<grid_for(n), kBlock, 0, s>>>(n, N.nx, N.ny, key.as<int32_t>());
+      HPCGX_LAUNCHED();
+      constexpr int R = kGsBlockRows;
+      const std::vector<int64_t> bp =
+          sort_rows_by_key(n, key.as<int32_t>(), kGsBlockColors * R, L.iperm, s);
+      for (int c = 0; c <= kMaxColors; ++c) cp.push_back(bp[std::min(c, kGsBlockColors) * R]);
+      for (int c = 0; c < kGsBlockColors; ++c)
+        for (int q = 0; q < R; ++q)  // every block holds one row of each position
+          HPCGX_REQUIRE(bp[c * R + q + 1] - bp[c * R + q] == (bp[c * R + R] - bp[c * R]) / R,
+                        "unequal runs of a block colour");
+      L.gs_rows = R;
+    } else if (nbands > 1) {
       TempBuf key(sizeof(int32_t) * n);
       k_band_key<<<grid_for(n), kBlock, 0, s>>>(n, N.nx, N.ny, band, nbands,
                                                 color.as<int32_t>(), key.as<int32_t>());
@@ -718,7 +790,7 @@ void finalize_level(Problem &P, int l, NaturalLevel &N, int flags,
     } else {
       cp = sort_rows_by_key(n, color.as<int32_t>(), kMaxColors, L.iperm, s);
     }
-    L.natural_in_colour = nbands == 1;
+    L.natural_in_colour = nbands == 1 && L.gs_rows == 1;
     int ncolors = 0;
     for (int c = 0; c < kMaxColors; ++c)
       if (cp[c + 1] > cp[c]) ncolors = c + 1;
@@ -809,6 +881,8 @@ void drop_wave(Problem &P, Level &L) {
 }
 
 void build_wave(Problem &P, Level &L, int slab_factor, int rows, hipStream_t s) {
+  if (L.gs_rows > 1)
+    throw Error(HPCGX_EUNSUP, "wavefront sweeps take point colours (no HPCGX_GS_BLOCKS)");
   if (!L.natural_in_colour)
     throw Error(HPCGX_EUNSUP,
                 "wavefront sweeps need natural row order inside each colour "
@@ -867,25 +941,43 @@ void link_levels(Problem &P, int l, const NaturalLevel &Nf, hipStream_t s) {
   F.xc = P.arena.alloc<double>(C.n);
   k_link<<<grid_for(C.n), kBlock, 0, s>>>(C.n, C.iperm, Nf.f2c, F.perm, F.f2c);
   HPCGX_LAUNCHED();
-  // fine-order transfers (Level::rperm) when f2c is onto a tile-aligned range
-  std::vector<int32_t> h(C.n);
-  HPCGX_HIP(hipMemcpyAsync(h.data(), F.f2c, sizeof(int32_t) * C.n, hipMemcpyDeviceToHost, s));
-  HPCGX_HIP(hipStreamSynchronize(s));
-  const auto mm = std::minmax_element(h.begin(), h.end());
-  const int64_t lo = *mm.first, hi = *mm.second;
-  if (hi - lo + 1 == C.n && lo % kTile == 0) {
-    std::vector<int32_t> rp(C.n, -1);
-    bool onto = true;
-    for (int64_t kc = 0; kc < C.n && onto; ++kc) {
-      int32_t &slot = rp[h[kc] - lo];
-      onto = slot < 0;
-      slot = (int32_t)kc;
-    }
-    if (onto) {
-      F.rf0 = lo;
-      F.rperm = P.arena.alloc<int32_t>(C.n);
-      HPCGX_HIP(hipMemcpy(F.rperm, rp.data(), sizeof(int32_t) * C.n, hipMemcpyHostToDevice));
-    }
+  // Fine-order transfers (Level::ftile, rperm): the fine rows f2c reaches,
+  // sorted, with their coarse rows; kept when every kTile of them (the last
+  // group possibly fewer) are consecutive rows from a multiple of kTile.
+  const int64_t nc = C.n;
+  std::vector<int32_t> fr(nc), cr(nc);
+  {
+    TempBuf ids(sizeof(int32_t) * nc), fs(sizeof(int32_t) * nc), cs(sizeof(int32_t) * nc);
+    k_iota<<<grid_for(nc), kBlock, 0, s>>>(nc, ids.as<int32_t>());
+    HPCGX_LAUNCHED();
+    const uint32_t *key = reinterpret_cast<const uint32_t *>(F.f2c);
+    size_t bytes = 0;
+    HPCGX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, key, fs.as<uint32_t>(),
+                                                 ids.as<int32_t>(), cs.as<int32_t>(), (int)nc, 0,
+                                                 bits_for(F.n), s));
+    TempBuf temp(bytes);
+    HPCGX_HIP(hipcub::DeviceRadixSort::SortPairs(temp.p, bytes, key, fs.as<uint32_t>(),
+                                                 ids.as<int32_t>(), cs.as<int32_t>(), (int)nc, 0,
+                                                 bits_for(F.n), s));
+    HPCGX_HIP(hipMemcpyAsync(fr.data(), fs.p, sizeof(int32_t) * nc, hipMemcpyDeviceToHost, s));
+    HPCGX_HIP(hipMemcpyAsync(cr.data(), cs.p, sizeof(int32_t) * nc, hipMemcpyDeviceToHost, s));
+    HPCGX_HIP(hipStreamSynchronize(s));
+  }
+  const int64_t ng = (nc + kTile - 1) / kTile;
+  std::vector<int32_t> ft(ng);
+  bool tiled = true;
+  for (int64_t g = 0; g < ng && tiled; ++g) {
+    const int64_t f0 = fr[g * kTile];
+    tiled = f0 % kTile == 0 && (g == 0 || f0 > fr[g * kTile - 1]);
+    for (int64_t t = g * kTile + 1; t < std::min(nc, (g + 1) * kTile) && tiled; ++t)
+      tiled = fr[t] == f0 + (t - g * kTile);
+    ft[g] = (int32_t)(f0 / kTile);
+  }
+  if (tiled) {
+    F.ftile = P.arena.alloc<int32_t>(ng);
+    F.rperm = P.arena.alloc<int32_t>(nc);
+    HPCGX_HIP(hipMemcpy(F.ftile, ft.data(), sizeof(int32_t) * ng, hipMemcpyHostToDevice));
+    HPCGX_HIP(hipMemcpy(F.rperm, cr.data(), sizeof(int32_t) * nc, hipMemcpyHostToDevice));
   }
 }
 
