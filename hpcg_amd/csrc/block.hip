@@ -233,14 +233,17 @@ __global__ __launch_bounds__(kBlock) void k_bgs32_blocks(int64_t begin, int64_t 
                                                          const float *__restrict__ diag32,
                                                          const float *__restrict__ R, float *X) {
   constexpr int W = S / 4;
-  const int64_t g = gtid();
-  const int64_t t = g / W;
-  if (t >= nb) return;
-  const int o = (int)(g % W) * 4;
+  const int64_t t0 = (int64_t)blockIdx.x * (kBlock / W);
+  if (t0 + threadIdx.x / W >= nb) return;
+  const int o = (int)(threadIdx.x % W) * 4;
 #pragma unroll 1
   for (int q = 0; q < kGsBlockRows; ++q) {
-    const int64_t i = begin + (int64_t)(BWD ? kGsBlockRows - 1 - q : q) * nb + t;
-    bgs32_row<STREAM, S, C16>(i, o, n, C, vc, col, val16, diag32, R, X);
+    // the workgroup's first block row of this q, opaque to the loop
+    // optimizer (see kernels.hip's k_gs_blocks)
+    int64_t w0 = begin + (int64_t)(BWD ? kGsBlockRows - 1 - q : q) * nb + t0;
+    int64_t nq = n;
+    __asm__ volatile("" : "+s"(w0), "+s"(nq));
+    bgs32_row<STREAM, S, C16>(w0 + threadIdx.x / W, o, nq, C, vc, col, val16, diag32, R, X);
   }
 }
 

@@ -261,19 +261,71 @@ __global__ __launch_bounds__(kBlock) void k_gs_range16(
 // each block's rows see one another's updates in sweep order.  With C16,
 // begin and nb are multiples of kTile, so a wavefront's rows of one q lie
 // in one tile.
+// gs_row16 on a kTile8 tile (8-bit offsets in use) with the row's offset
+// words already loaded.
+template <bool STREAM>
+__device__ inline void gs_row8w(const Col16 &C, int t, const uint32_t (&o8)[kSlotWords],
+                                int64_t i, int64_t n, const double *__restrict__ val,
+                                const double *__restrict__ r, double *x) {
+  const int32_t *__restrict__ base = C.base + t;
+  double sum = r[i];
+  double d = 0.0;
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int32_t c = base[(int64_t)j * C.ntiles] + (int32_t)((o8[j / 4] >> (8 * (j % 4))) & 0xffu);
+    const double v = mload<STREAM>(val + (int64_t)j * n + i);
+    sum -= v * x[c];
+    if (c == i && v != 0.0) d = v;
+  }
+  sum += x[i] * d;
+  x[i] = sum / d;
+}
+
 template <bool STREAM, bool FINEST, bool BWD, bool C16>
-__global__ __launch_bounds__(kBlock) void k_gs_blocks(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_gs_blocks(
     int64_t begin, int64_t nb, int64_t n, Col16 C, const int32_t *__restrict__ col,
     const double *__restrict__ val, const double *__restrict__ r, double *x, bool xcd) {
-  const int64_t t = xcd_block(blockIdx.x, gridDim.x, xcd) * kBlock + threadIdx.x;
-  if (t >= nb) return;
+  const int64_t t0 = xcd_block(blockIdx.x, gridDim.x, xcd) * kBlock;
+  if (t0 + threadIdx.x >= nb) return;
+  // The workgroup's first row of each q is kept opaque to the loop optimizer
+  // (else it carries one 64-bit address per slot in vector registers from
+  // row to row).  With C16 the 8-bit offset words of row q+1 are loaded
+  // before row q is computed, so that a row's gathers leave in one round
+  // trip.
+  int64_t w0 = begin + (int64_t)(BWD ? kGsBlockRows - 1 : 0) * nb + t0;
+  uint32_t o8[kSlotWords];
+  if (C16) {
+    int64_t nq = n;
+    __asm__ volatile("" : "+s"(w0), "+s"(nq));
+#pragma unroll
+    for (int g = 0; g < kSlotWords; ++g)
+      o8[g] = mload<STREAM>(C.off8 + (int64_t)g * nq + w0 + threadIdx.x);
+  }
 #pragma unroll 1
   for (int q = 0; q < kGsBlockRows; ++q) {
-    const int64_t i = begin + (int64_t)(BWD ? kGsBlockRows - 1 - q : q) * nb + t;
-    if (C16)
-      gs_row16<STREAM>(C, i, n, col, val, r, x);
+    int64_t nq = n;
+    __asm__ volatile("" : "+s"(w0), "+s"(nq));
+    const int64_t i = w0 + threadIdx.x;
+    if (!C16) {
+      gs_row<STREAM>(i, nq, col, val, r, x);
+      w0 += BWD ? -nb : nb;
+      continue;
+    }
+    int64_t w1 = w0 + (BWD ? -nb : nb);
+    uint32_t on[kSlotWords];
+    if (q + 1 < kGsBlockRows) {
+#pragma unroll
+      for (int g = 0; g < kSlotWords; ++g)
+        on[g] = mload<STREAM>(C.off8 + (int64_t)g * nq + w1 + threadIdx.x);
+    }
+    const int t = tile_of(i);
+    if (C.info[t] == kTile8 && C.use8)
+      gs_row8w<STREAM>(C, t, o8, i, nq, val, r, x);
     else
-      gs_row<STREAM>(i, n, col, val, r, x);
+      gs_row16<STREAM>(C, i, nq, col, val, r, x);
+#pragma unroll
+    for (int g = 0; g < kSlotWords; ++g) o8[g] = on[g];
+    w0 = w1;
   }
 }
 

@@ -283,17 +283,23 @@ __global__ __launch_bounds__(kBlock) void k_gs32_blocks(
     const uint32_t *__restrict__ vc, const int32_t *__restrict__ col,
     const half_t *__restrict__ val16, const float *__restrict__ diag32,
     const float *__restrict__ r, float *x, bool xcd) {
-  const int64_t t = xcd_block(blockIdx.x, gridDim.x, xcd) * kBlock + threadIdx.x;
-  if (t >= nb) return;
+  const int64_t t0 = xcd_block(blockIdx.x, gridDim.x, xcd) * kBlock;
+  if (t0 + threadIdx.x >= nb) return;
 #pragma unroll 1
   for (int q = 0; q < kGsBlockRows; ++q) {
-    const int64_t i = begin + (int64_t)(BWD ? kGsBlockRows - 1 - q : q) * nb + t;
+    // the workgroup's first row of this q and the level size, kept opaque
+    // to the loop optimizer: otherwise it keeps one 64-bit address per slot
+    // in vector registers across the rows (3.5x the registers of one row)
+    int64_t w0 = begin + (int64_t)(BWD ? kGsBlockRows - 1 - q : q) * nb + t0;
+    int64_t nq = n;
+    __asm__ volatile("" : "+s"(w0), "+s"(nq));
+    const int64_t i = w0 + threadIdx.x;
     if (K == 2)
-      gs32_row8<STREAM>(C, vp, vc, i, n, col, val16, diag32, r, x);
+      gs32_row8<STREAM>(C, vp, vc, i, nq, col, val16, diag32, r, x);
     else if (K == 1)
-      gs32_row16<STREAM>(C, vc, i, n, col, val16, diag32, r, x);
+      gs32_row16<STREAM>(C, vc, i, nq, col, val16, diag32, r, x);
     else
-      gs32_row<STREAM>(i, n, col, val16, diag32, r, x);
+      gs32_row<STREAM>(i, nq, col, val16, diag32, r, x);
   }
 }
 

@@ -86,7 +86,6 @@ def test_device_order_is_block_order(hp, dims):
         info = P.info(l)
         even = all(d % 2 == 0 for d in h.dims(l))
         assert info["gs_rows"] == (8 if even else 1), l
-        assert info["col16"], l
         got = device_order(P, l)
         if even:
             assert np.array_equal(got, block_order(*h.dims(l))), l
@@ -171,10 +170,10 @@ def test_fp32_blocks_bitexact(hp, dims, c16, c8, pairs):
     del P
 
 
-@pytest.mark.parametrize("s", [8, 16, 3])
+@pytest.mark.parametrize("s", [8, 16, 2])
 def test_block_pcg_blocks_bitexact(hp, s):
     """The block V-cycle (W lanes per row for s = 8, 16; one lane per column
-    and run-by-run launches for s = 3) equals the oracle's V-cycle on every
+    and run-by-run launches for s = 2) equals the oracle's V-cycle on every
     column."""
     import torch
     dims = (32, 48, 64)
