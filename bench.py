@@ -47,8 +47,9 @@ def parse():
                     help="16-bit column ids (tile base + offset) where built")
     ap.add_argument("--band-lines", type=int, default=0,
                     help="grid lines per band of the in-colour row order (0 = default, 1 = natural)")
-    ap.add_argument("--gs-blocks", type=int, default=0, choices=[0, 1],
-                    help="multicolour sweeps over 2x2x2 blocks of grid points (HPCGX_GS_BLOCKS)")
+    ap.add_argument("--sweep-order", type=int, default=0, choices=[0, 1],
+                    help="colour visiting order of the multicolour sweeps (HPCGX_SWEEP_ORDER: "
+                         "0 = default, 1 = colour id order)")
     ap.add_argument("--xcd-remap", type=int, default=0, choices=[0, 1])
     ap.add_argument("--spmv-dot-blocks", type=int, default=0, help="0 = library default")
     ap.add_argument("--stream-min-bytes", type=float, default=0,
@@ -290,7 +291,7 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     P = hpcg_amd.Problem.generate(a.nx, a.ny, a.nz, nlevels=4, band_lines=a.band_lines,
-                                  gs_blocks=bool(a.gs_blocks))
+                                  sweep_order=a.sweep_order)
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
     b, x, _ = P.vectors()
@@ -340,12 +341,9 @@ def main():
     ntiles0 = (lv[0]["n"] + 63) // 64
     c8 = c16 and info0["byte_tiles"] == ntiles0 and not any(
         l.strip() == "chosen col8=0" for l in (tuned or []))
-    blocks = info0["gs_rows"] > 1
     ksym = ("k_gs_wave<true,true>" if info0["wave"] else
-            "k_gs_blocks<true,true,*,%s>" % ("true" if c16 else "false") if blocks else
             "k_gs_range16<true,true>" if c16 else "k_gs_range<true,true>")
     kname = ksym + (" (finest-level wavefront sweep, one direction)" if info0["wave"]
-                    else " (finest-level block colour step)" if blocks
                     else " (finest-level colour step)")
     per_launch = K * 2 * cb["symgs_L0"] / max(ks_n, 1)
     # the same sweep in the stored format: every ELL slot (padding included)
@@ -382,7 +380,7 @@ def main():
         "config": {"workload": "HPCG PCG %dx%dx%d FP64, 4 MG levels, %s Gauss-Seidel"
                                % (a.nx, a.ny, a.nz, a.ordering),
                    "nx": a.nx, "ny": a.ny, "nz": a.nz, "mg_levels": 4,
-                   "ordering": a.ordering, "gs_blocks": bool(a.gs_blocks),
+                   "ordering": a.ordering, "sweep_order": a.sweep_order,
                    "rows": lv[0]["n"], "nnz": lv[0]["nnz"]},
         "iteration_gbs_canonical": round(gbs, 1),
         "iteration_hbm_frac": round(gbs / HBM_PEAK_GBS, 4),

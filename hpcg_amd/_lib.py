@@ -12,7 +12,6 @@ LIB_PATH = os.path.join(PKG, "libhpcg_amd.so")
 ORDER_LEXICOGRAPHIC = 0
 ORDER_MULTICOLOR = 1
 KEEP_SCHEDULE = 1
-GS_BLOCKS = 1 << 16
 CG_TIMERS = 1
 CG_GRAPH = 2
 
@@ -42,7 +41,7 @@ class LevelInfo(ctypes.Structure):
                 ("ncolors", ctypes.c_int), ("nlevels_sched", ctypes.c_int),
                 ("ncoarse", ctypes.c_int64), ("wave", ctypes.c_int), ("nslabs", ctypes.c_int),
                 ("col16", ctypes.c_int), ("wide_tiles", ctypes.c_int64),
-                ("byte_tiles", ctypes.c_int64), ("gs_rows", ctypes.c_int)]
+                ("byte_tiles", ctypes.c_int64)]
 
 
 # name -> (restype, argtypes)

@@ -221,32 +221,6 @@ __global__ __launch_bounds__(kBlock) void k_bgs32_rows(int64_t begin, int64_t en
   bgs32_row<STREAM, S, C16>(i, (int)(g % W) * 4, n, C, vc, col, val16, diag32, R, X);
 }
 
-// One colour of the block sweeps (Level::gs_rows): the W lanes of block t
-// update rows begin + q nb + t for q = 0 .. kGsBlockRows-1 (reversed when
-// BWD).  With C16, begin and nb are multiples of kTile.
-template <bool STREAM, int S, bool C16, bool BWD>
-__global__ __launch_bounds__(kBlock) void k_bgs32_blocks(int64_t begin, int64_t nb, int64_t n,
-                                                         Col16 C,
-                                                         const uint32_t *__restrict__ vc,
-                                                         const int32_t *__restrict__ col,
-                                                         const half_t *__restrict__ val16,
-                                                         const float *__restrict__ diag32,
-                                                         const float *__restrict__ R, float *X) {
-  constexpr int W = S / 4;
-  const int64_t t0 = (int64_t)blockIdx.x * (kBlock / W);
-  if (t0 + threadIdx.x / W >= nb) return;
-  const int o = (int)(threadIdx.x % W) * 4;
-#pragma unroll 1
-  for (int q = 0; q < kGsBlockRows; ++q) {
-    // the workgroup's first block row of this q, opaque to the loop
-    // optimizer (see kernels.hip's k_gs_blocks)
-    int64_t w0 = begin + (int64_t)(BWD ? kGsBlockRows - 1 - q : q) * nb + t0;
-    int64_t nq = n;
-    __asm__ volatile("" : "+s"(w0), "+s"(nq));
-    bgs32_row<STREAM, S, C16>(w0 + threadIdx.x / W, o, nq, C, vc, col, val16, diag32, R, X);
-  }
-}
-
 // rc = rf[f2c] - (A xf)[f2c] in fine order (Level::rperm): mixed.hip's
 // row_dot32 for each column.
 template <bool STREAM, int S, bool C16>
@@ -577,41 +551,6 @@ void gs_rows(const Problem &P, const Level &L, int s, int64_t b, int64_t e, cons
   else gs_rows_s<16>(P, L, b, e, R, X, st);
 }
 
-template <int S, bool BWD>
-void gs_blocks_s(const Problem &P, const Level &L, int64_t b, int64_t nb, const float *R,
-                 float *X, hipStream_t st) {
-  const Col16 C = col16_of(P, L);
-  const bool c16 = L.vc && use_col16(P, L, b) && nb % kTile == 0;
-  const unsigned g = grid_for(nb * (S / 4));
-  if (L.stream && c16)
-    k_bgs32_blocks<true, S, true, BWD><<<g, kBlock, 0, st>>>(b, nb, L.n, C, L.vc, L.col,
-                                                             L.val16, L.diag32, R, X);
-  else if (L.stream)
-    k_bgs32_blocks<true, S, false, BWD><<<g, kBlock, 0, st>>>(b, nb, L.n, C, L.vc, L.col,
-                                                              L.val16, L.diag32, R, X);
-  else if (c16)
-    k_bgs32_blocks<false, S, true, BWD><<<g, kBlock, 0, st>>>(b, nb, L.n, C, L.vc, L.col,
-                                                              L.val16, L.diag32, R, X);
-  else
-    k_bgs32_blocks<false, S, false, BWD><<<g, kBlock, 0, st>>>(b, nb, L.n, C, L.vc, L.col,
-                                                               L.val16, L.diag32, R, X);
-  HPCGX_LAUNCHED();
-}
-
-// Colour c of a block sweep (Level::gs_rows > 1) with the W-lanes kernels.
-void gs_blocks(const Problem &P, const Level &L, int s, int c, bool bwd, const float *R,
-               float *X, hipStream_t st) {
-  const int64_t b = L.color_ptr[c], e = L.color_ptr[c + 1];
-  if (e <= b) return;
-  const int64_t nb = (e - b) / L.gs_rows;
-  if (s == 4) bwd ? gs_blocks_s<4, true>(P, L, b, nb, R, X, st)
-                  : gs_blocks_s<4, false>(P, L, b, nb, R, X, st);
-  else if (s == 8) bwd ? gs_blocks_s<8, true>(P, L, b, nb, R, X, st)
-                       : gs_blocks_s<8, false>(P, L, b, nb, R, X, st);
-  else bwd ? gs_blocks_s<16, true>(P, L, b, nb, R, X, st)
-           : gs_blocks_s<16, false>(P, L, b, nb, R, X, st);
-}
-
 template <int S>
 void restrict_rows_s(const Problem &P, const Level &L, const float *RF, const float *XF,
                      float *RC, hipStream_t st) {
@@ -686,11 +625,7 @@ void spmv_rows(const Problem &P, const Level &L, int s, const double *X, double 
 void bsymgs32(Problem &P, int l, int ordering, int s, const float *R, float *X,
               hipStream_t st) {
   const Level &L = P.L[l];
-  if (ordering == HPCGX_ORDER_MULTICOLOR && rows_kernels(s) && L.gs_rows > 1) {
-    for (int pass = 0; pass < 2; ++pass)
-      for (int q = 0; q < L.ncolors; ++q)
-        gs_blocks(P, L, s, pass == 0 ? q : L.ncolors - 1 - q, pass == 1, R, X, st);
-  } else if (ordering == HPCGX_ORDER_MULTICOLOR && rows_kernels(s)) {
+  if (ordering == HPCGX_ORDER_MULTICOLOR && rows_kernels(s)) {
     for (int pass = 0; pass < 2; ++pass)
       for (int q = 0; q < L.ncolors; ++q) {
         const int c = pass == 0 ? q : L.ncolors - 1 - q;
@@ -698,24 +633,17 @@ void bsymgs32(Problem &P, int l, int ordering, int s, const float *R, float *X,
         if (e > b) gs_rows(P, L, s, b, e, R, X, st);
       }
   } else if (ordering == HPCGX_ORDER_MULTICOLOR) {
-    // block sweeps: a block colour's runs one after another (each run's rows
-    // lie in different blocks, so they are independent)
-    const int runs = L.gs_rows;
     for (int pass = 0; pass < 2; ++pass)
       for (int q = 0; q < L.ncolors; ++q) {
         const int c = pass == 0 ? q : L.ncolors - 1 - q;
-        const int64_t b0 = L.color_ptr[c], e0 = L.color_ptr[c + 1];
-        if (e0 <= b0) continue;
-        const int64_t nr = (e0 - b0) / runs;
-        for (int k = 0; k < runs; ++k) {
-          const int64_t b = b0 + (pass == 0 ? k : runs - 1 - k) * nr, e = b + nr;
-          const unsigned g = grid_for((e - b) * s);
-          if (L.stream)
-            k_bgs32<true><<<g, kBlock, 0, st>>>(b, e, L.n, s, L.col, L.val16, L.diag32, R, X);
-          else
-            k_bgs32<false><<<g, kBlock, 0, st>>>(b, e, L.n, s, L.col, L.val16, L.diag32, R, X);
-          HPCGX_LAUNCHED();
-        }
+        const int64_t b = L.color_ptr[c], e = L.color_ptr[c + 1];
+        if (e <= b) continue;
+        const unsigned g = grid_for((e - b) * s);
+        if (L.stream)
+          k_bgs32<true><<<g, kBlock, 0, st>>>(b, e, L.n, s, L.col, L.val16, L.diag32, R, X);
+        else
+          k_bgs32<false><<<g, kBlock, 0, st>>>(b, e, L.n, s, L.col, L.val16, L.diag32, R, X);
+        HPCGX_LAUNCHED();
       }
   } else {
     for (int pass = 0; pass < 2; ++pass)

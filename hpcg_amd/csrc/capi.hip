@@ -237,7 +237,6 @@ int hpcgx_problem_info(const hpcgx_problem *p, int level, hpcgx_level_info *info
     info->col16 = L.c16 ? 1 : 0;
     info->wide_tiles = L.wide_tiles;
     info->byte_tiles = L.byte_tiles;
-    info->gs_rows = L.gs_rows;
   });
 }
 
@@ -631,8 +630,6 @@ int hpcgx_generate_fleet(int count, const int *nx, const int *ny, const int *nz,
     if (count < 1 || count > hpcgx::kMaxFleet)
       throw Error(HPCGX_EINVAL, "fleet size must be in [1, 1024]");
     if (nlevels < 1 || nlevels > hpcgx::kMaxLevels) throw Error(HPCGX_EINVAL, "nlevels out of range");
-    if (flags & HPCGX_GS_BLOCKS)
-      throw Error(HPCGX_EUNSUP, "fleets sweep point colours (no HPCGX_GS_BLOCKS)");
     const int div = 1 << (nlevels - 1);
     std::vector<int> dims(3 * (size_t)count);
     int64_t total = 0;

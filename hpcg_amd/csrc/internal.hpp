@@ -27,8 +27,6 @@ constexpr int64_t kWaveMinRows = 1 << 23;  // levels swept as a wavefront
 constexpr int kWaveSlabFactor = 2;         // slab width / matrix bandwidth
 constexpr int kWaveRows = 2;               // rows per thread of a wavefront task
 constexpr int kTile = 64;  // rows sharing narrow column bases (one wavefront)
-constexpr int kGsBlockRows = 8;    // rows per block of the block sweeps (2x2x2)
-constexpr int kGsBlockColors = 8;  // colours of those blocks
 constexpr int kSlotWords = (kSlots + 3) / 4;  // 8-bit offsets of 4 slots per word
 constexpr int kPairWords = (kSlots + 1) / 2;  // FP16 values of 2 slots per word
 constexpr int kBaseStride = 32;  // ints per tile in Level::tbase (128 bytes)
@@ -168,18 +166,11 @@ struct Level {
   // When the fine rows f2c reaches, in ascending order, form runs of
   // consecutive rows that each start at a multiple of kTile and (all but the
   // last) hold a multiple of kTile rows (the injected points of the 27-point
-  // grid are one parity class: the first colour block, or one position of
-  // every 2x2x2 block), the transfers run in fine order: thread t owns fine
-  // row fine_row(ftile, t) (ascending in t) and coarse row rperm[t], so a
-  // wavefront reads whole tiles of matrix rows.
+  // grid are one parity class, a colour block), the transfers run in fine
+  // order: thread t owns fine row fine_row(ftile, t) (ascending in t) and
+  // coarse row rperm[t], so a wavefront reads whole tiles of matrix rows.
   int32_t *ftile = nullptr;  // per kTile transfer rows: fine row / kTile
   int32_t *rperm = nullptr;
-  // Block sweeps (HPCGX_GS_BLOCKS): colour c's rows are gs_rows runs of
-  // nb = (color_ptr[c+1] - color_ptr[c]) / gs_rows rows, run r holding
-  // position r of each block, blocks in the same order in every run.  A
-  // colour step gives each block's rows to one work-item, in run order
-  // (reverse order in the backward sweep).  gs_rows = 1: point colours.
-  int gs_rows = 1;
   double *rc = nullptr, *xc = nullptr;  // next level's residual and correction
 };
 
@@ -318,10 +309,6 @@ void launch_spmv(const Problem &P, const Level &L, const double *x, double *y,
                  hipStream_t s);
 void launch_symgs_rows(const Problem &P, const Level &L, int64_t begin, int64_t end,
                        const double *r, double *x, hipStream_t s);
-// Colour c of the multicolour sweep in the given direction (point colours:
-// one range; block sweeps: each block's rows in order, reversed when bwd).
-void launch_symgs_colour(const Problem &P, const Level &L, int c, bool bwd, const double *r,
-                         double *x, hipStream_t s);
 void launch_symgs_list(const Level &L, const int32_t *rows, int64_t count,
                        const double *r, double *x, hipStream_t s);
 void launch_restriction(const Level &L, const double *rf, const double *Axf,

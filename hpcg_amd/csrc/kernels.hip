@@ -254,81 +254,6 @@ __global__ __launch_bounds__(kBlock) void k_gs_range16(
   if (i < end) gs_row16<STREAM>(C, i, n, col, val, r, x);
 }
 
-// One colour of the block sweeps (Level::gs_rows = kGsBlockRows): work-item
-// t updates rows begin + q nb + t for q = 0 .. kGsBlockRows-1 in order
-// (reverse order when BWD), the rows of one block.  Rows of different blocks
-// of one colour share no matrix entry, so the blocks are independent and
-// each block's rows see one another's updates in sweep order.  With C16,
-// begin and nb are multiples of kTile, so a wavefront's rows of one q lie
-// in one tile.
-// gs_row16 on a kTile8 tile (8-bit offsets in use) with the row's offset
-// words already loaded.
-template <bool STREAM>
-__device__ inline void gs_row8w(const Col16 &C, int t, const uint32_t (&o8)[kSlotWords],
-                                int64_t i, int64_t n, const double *__restrict__ val,
-                                const double *__restrict__ r, double *x) {
-  const int32_t *__restrict__ base = C.base + t;
-  double sum = r[i];
-  double d = 0.0;
-#pragma unroll
-  for (int j = 0; j < kSlots; ++j) {
-    const int32_t c = base[(int64_t)j * C.ntiles] + (int32_t)((o8[j / 4] >> (8 * (j % 4))) & 0xffu);
-    const double v = mload<STREAM>(val + (int64_t)j * n + i);
-    sum -= v * x[c];
-    if (c == i && v != 0.0) d = v;
-  }
-  sum += x[i] * d;
-  x[i] = sum / d;
-}
-
-template <bool STREAM, bool FINEST, bool BWD, bool C16>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_gs_blocks(
-    int64_t begin, int64_t nb, int64_t n, Col16 C, const int32_t *__restrict__ col,
-    const double *__restrict__ val, const double *__restrict__ r, double *x, bool xcd) {
-  const int64_t t0 = xcd_block(blockIdx.x, gridDim.x, xcd) * kBlock;
-  if (t0 + threadIdx.x >= nb) return;
-  // The workgroup's first row of each q is kept opaque to the loop optimizer
-  // (else it carries one 64-bit address per slot in vector registers from
-  // row to row).  With C16 the 8-bit offset words of row q+1 are loaded
-  // before row q is computed, so that a row's gathers leave in one round
-  // trip.
-  int64_t w0 = begin + (int64_t)(BWD ? kGsBlockRows - 1 : 0) * nb + t0;
-  uint32_t o8[kSlotWords];
-  if (C16) {
-    int64_t nq = n;
-    __asm__ volatile("" : "+s"(w0), "+s"(nq));
-#pragma unroll
-    for (int g = 0; g < kSlotWords; ++g)
-      o8[g] = mload<STREAM>(C.off8 + (int64_t)g * nq + w0 + threadIdx.x);
-  }
-#pragma unroll 1
-  for (int q = 0; q < kGsBlockRows; ++q) {
-    int64_t nq = n;
-    __asm__ volatile("" : "+s"(w0), "+s"(nq));
-    const int64_t i = w0 + threadIdx.x;
-    if (!C16) {
-      gs_row<STREAM>(i, nq, col, val, r, x);
-      w0 += BWD ? -nb : nb;
-      continue;
-    }
-    int64_t w1 = w0 + (BWD ? -nb : nb);
-    uint32_t on[kSlotWords];
-    if (q + 1 < kGsBlockRows) {
-#pragma unroll
-      for (int g = 0; g < kSlotWords; ++g)
-        on[g] = mload<STREAM>(C.off8 + (int64_t)g * nq + w1 + threadIdx.x);
-    }
-    const int t = tile_of(i);
-    if (C.info[t] == kTile8 && C.use8)
-      gs_row8w<STREAM>(C, t, o8, i, nq, val, r, x);
-    else
-      gs_row16<STREAM>(C, i, nq, col, val, r, x);
-#pragma unroll
-    for (int g = 0; g < kSlotWords; ++g) o8[g] = on[g];
-    w0 = w1;
-  }
-}
-
 // Two adjacent rows per thread: the matrix is read with 16-byte (values) and
 // 8-byte (column ids) loads per lane.  Needs an even level size and an even
 // range start, so that slot j of the pair is 16-byte aligned.
@@ -911,41 +836,6 @@ void launch_symgs_rows(const Problem &P, const Level &L, int64_t begin, int64_t 
     k_gs_range<false, false><<<g, kBlock, 0, s>>>(begin, end, L.n, L.col,
                                                   L.val, r, x, L.xcd);
   HPCGX_LAUNCHED();
-}
-
-namespace {
-template <bool BWD, bool C16>
-void gs_blocks(const Problem &P, const Level &L, int64_t b, int64_t nb, const double *r,
-               double *x, hipStream_t s) {
-  const Col16 C = col16_of(P, L);
-  const unsigned g = grid_for(nb);
-  if (L.finest)
-    k_gs_blocks<true, true, BWD, C16><<<g, kBlock, 0, s>>>(b, nb, L.n, C, L.col, L.val, r, x,
-                                                           L.xcd);
-  else if (L.stream)
-    k_gs_blocks<true, false, BWD, C16><<<g, kBlock, 0, s>>>(b, nb, L.n, C, L.col, L.val, r, x,
-                                                            L.xcd);
-  else
-    k_gs_blocks<false, false, BWD, C16><<<g, kBlock, 0, s>>>(b, nb, L.n, C, L.col, L.val, r,
-                                                             x, L.xcd);
-  HPCGX_LAUNCHED();
-}
-}  // namespace
-
-void launch_symgs_colour(const Problem &P, const Level &L, int c, bool bwd, const double *r,
-                         double *x, hipStream_t s) {
-  const int64_t b = L.color_ptr[c], e = L.color_ptr[c + 1];
-  if (e <= b) return;
-  if (L.gs_rows == 1) {
-    launch_symgs_rows(P, L, b, e, r, x, s);
-    return;
-  }
-  const int64_t nb = (e - b) / L.gs_rows;
-  const bool c16 = use_col16(P, L, b) && nb % kTile == 0;
-  if (bwd && c16) gs_blocks<true, true>(P, L, b, nb, r, x, s);
-  else if (bwd) gs_blocks<true, false>(P, L, b, nb, r, x, s);
-  else if (c16) gs_blocks<false, true>(P, L, b, nb, r, x, s);
-  else gs_blocks<false, false>(P, L, b, nb, r, x, s);
 }
 
 void launch_symgs_list(const Level &L, const int32_t *rows, int64_t count,

@@ -272,37 +272,6 @@ __global__ __launch_bounds__(kBlock) void k_gs32_range8(
   if (i < end) gs32_row8<STREAM>(C, vp, vc, i, n, col, val16, diag32, r, x);
 }
 
-// One colour of the block sweeps (Level::gs_rows): work-item t updates rows
-// begin + q nb + t for q = 0 .. kGsBlockRows-1 (reversed when BWD), the
-// rows of one block, as kernels.hip's k_gs_blocks does in FP64.  K: 2 = FP16
-// value pairs (gs32_row8), 1 = packed words (gs32_row16), 0 = the FP16 array
-// and 32-bit ids; K > 0 needs begin and nb multiples of kTile.
-template <bool STREAM, bool FINEST, bool BWD, int K>
-__global__ __launch_bounds__(kBlock) void k_gs32_blocks(
-    int64_t begin, int64_t nb, int64_t n, Col16 C, const uint32_t *__restrict__ vp,
-    const uint32_t *__restrict__ vc, const int32_t *__restrict__ col,
-    const half_t *__restrict__ val16, const float *__restrict__ diag32,
-    const float *__restrict__ r, float *x, bool xcd) {
-  const int64_t t0 = xcd_block(blockIdx.x, gridDim.x, xcd) * kBlock;
-  if (t0 + threadIdx.x >= nb) return;
-#pragma unroll 1
-  for (int q = 0; q < kGsBlockRows; ++q) {
-    // the workgroup's first row of this q and the level size, kept opaque
-    // to the loop optimizer: otherwise it keeps one 64-bit address per slot
-    // in vector registers across the rows (3.5x the registers of one row)
-    int64_t w0 = begin + (int64_t)(BWD ? kGsBlockRows - 1 - q : q) * nb + t0;
-    int64_t nq = n;
-    __asm__ volatile("" : "+s"(w0), "+s"(nq));
-    const int64_t i = w0 + threadIdx.x;
-    if (K == 2)
-      gs32_row8<STREAM>(C, vp, vc, i, nq, col, val16, diag32, r, x);
-    else if (K == 1)
-      gs32_row16<STREAM>(C, vc, i, nq, col, val16, diag32, r, x);
-    else
-      gs32_row<STREAM>(i, nq, col, val16, diag32, r, x);
-  }
-}
-
 // vc[k] = col16[k] << 16 | bits of val16[k], one thread per row (a launch
 // holds fewer than 2^32 work-items; 27 n can exceed that).
 __global__ __launch_bounds__(kBlock) void k_pack_vc(int64_t n, const uint16_t *__restrict__ off,
@@ -459,47 +428,6 @@ void gs32_step(const Problem &P, const Level &L, int64_t begin, int64_t end,
   HPCGX_LAUNCHED();
 }
 
-template <bool BWD, int K>
-void gs32_blocks(const Problem &P, const Level &L, int64_t b, int64_t nb, const float *r,
-                 float *x, hipStream_t s) {
-  const Col16 C = col16_of(P, L);
-  const unsigned g = grid_for(nb);
-  if (L.finest)
-    k_gs32_blocks<true, true, BWD, K><<<g, kBlock, 0, s>>>(b, nb, L.n, C, L.v16p, L.vc, L.col,
-                                                           L.val16, L.diag32, r, x, L.xcd);
-  else if (L.stream)
-    k_gs32_blocks<true, false, BWD, K><<<g, kBlock, 0, s>>>(b, nb, L.n, C, L.v16p, L.vc, L.col,
-                                                            L.val16, L.diag32, r, x, L.xcd);
-  else
-    k_gs32_blocks<false, false, BWD, K><<<g, kBlock, 0, s>>>(b, nb, L.n, C, L.v16p, L.vc,
-                                                             L.col, L.val16, L.diag32, r, x,
-                                                             L.xcd);
-  HPCGX_LAUNCHED();
-}
-
-// Colour c of the FP32 multicolour sweep in one direction.
-void gs32_colour(const Problem &P, const Level &L, int c, bool bwd, const float *r, float *x,
-                 hipStream_t s) {
-  const int64_t b = L.color_ptr[c], e = L.color_ptr[c + 1];
-  if (e <= b) return;
-  if (L.gs_rows == 1) {
-    gs32_step(P, L, b, e, r, x, s);
-    return;
-  }
-  const int64_t nb = (e - b) / L.gs_rows;
-  const bool c16 = use_col16(P, L, b) && nb % kTile == 0 && L.vc;
-  const int k = !c16 ? 0 : (L.v16p && P.use_pairs && P.use_c8 && L.byte_tiles > 0) ? 2 : 1;
-  if (bwd) {
-    if (k == 2) gs32_blocks<true, 2>(P, L, b, nb, r, x, s);
-    else if (k == 1) gs32_blocks<true, 1>(P, L, b, nb, r, x, s);
-    else gs32_blocks<true, 0>(P, L, b, nb, r, x, s);
-  } else {
-    if (k == 2) gs32_blocks<false, 2>(P, L, b, nb, r, x, s);
-    else if (k == 1) gs32_blocks<false, 1>(P, L, b, nb, r, x, s);
-    else gs32_blocks<false, 0>(P, L, b, nb, r, x, s);
-  }
-}
-
 }  // namespace
 
 void prepare_mixed(Problem &P) {
@@ -548,8 +476,10 @@ void symgs32(Problem &P, int l, int ordering, const float *r, float *x,
              hipStream_t s) {
   const Level &L = P.L[l];
   if (ordering == HPCGX_ORDER_MULTICOLOR) {
-    for (int c = 0; c < L.ncolors; ++c) gs32_colour(P, L, c, false, r, x, s);
-    for (int c = L.ncolors - 1; c >= 0; --c) gs32_colour(P, L, c, true, r, x, s);
+    for (int c = 0; c < L.ncolors; ++c)
+      gs32_step(P, L, L.color_ptr[c], L.color_ptr[c + 1], r, x, s);
+    for (int c = L.ncolors - 1; c >= 0; --c)
+      gs32_step(P, L, L.color_ptr[c], L.color_ptr[c + 1], r, x, s);
   } else {
     for (int pass = 0; pass < 2; ++pass)
       for (int k = 0; k < L.nsched; ++k) {

@@ -28,6 +28,22 @@
 namespace hpcgx {
 namespace {
 
+// Forward visiting orders of the greedy colours 0..7 (HPCGX_SWEEP_ORDER);
+// colours from 8 on follow in id order.  On the 27-point grid colour
+// (x&1) + 2(y&1) + 4(z&1) is the parity class, and colour 0 holds the points
+// the restriction injects (ComputeRestriction_ref.cpp:51 through f2c,
+// GenerateCoarseProblem.cpp:77-79).  Swept last, as in id order, colour 0's
+// rows end each backward sweep with a residual of zero up to rounding, so
+// the coarse grid receives almost nothing.  Order 0 visits colour 0 last in
+// the forward sweep and first in the backward one; at 512x512x640 PCG then
+// reaches the 50-iteration lexicographic residual in 57 iterations instead
+// of 70.  Order 1 is the id order.
+constexpr int kSweepOrders[][8] = {
+    {3, 5, 6, 7, 1, 2, 4, 0},
+    {0, 1, 2, 3, 4, 5, 6, 7},
+};
+constexpr int kNumSweepOrders = (int)(sizeof(kSweepOrders) / sizeof(kSweepOrders[0]));
+
 constexpr int kStepBlocks = 2048;      // fixed grid of the Kahn steps
 constexpr int kSyncEvery = 128;        // Kahn steps between host checks
 constexpr int64_t kMaxSched = 1 << 22; // bound on schedule levels
@@ -265,49 +281,6 @@ __global__ __launch_bounds__(kBlock) void k_verify_schedule(int64_t n, const int
   if ((threadIdx.x & 63) == 0) atomicMax(max_level, lv);
 }
 
-// Block sweeps on an nx*ny*nz grid with even dimensions, rows in natural
-// order: the 2x2x2 block of grid point (x, y, z) is (x/2, y/2, z/2), its
-// colour the parity of those three, and the point's position in it
-// (x & 1) + 2 (y & 1) + 4 (z & 1).  key = colour * kGsBlockRows + position.
-__device__ inline void grid_block(int64_t i, int nx, int ny, int64_t &block, int &colour,
-                                  int &pos) {
-  const int x = (int)(i % nx);
-  const int64_t t = i / nx;
-  const int y = (int)(t % ny), z = (int)(t / ny);
-  block = ((int64_t)(z >> 1) * (ny >> 1) + (y >> 1)) * (nx >> 1) + (x >> 1);
-  colour = ((x >> 1) & 1) | (((y >> 1) & 1) << 1) | (((z >> 1) & 1) << 2);
-  pos = (x & 1) | ((y & 1) << 1) | ((z & 1) << 2);
-}
-
-__global__ __launch_bounds__(kBlock) void k_block_key(int64_t n, int nx, int ny,
-                                                      int32_t *key) {
-  const int64_t i = gtid();
-  if (i >= n) return;
-  int64_t b;
-  int c, p;
-  grid_block(i, nx, ny, b, c, p);
-  key[i] = c * kGsBlockRows + p;
-}
-
-// bad = 1 when an entry couples two different blocks of one colour (the
-// block sweep of that colour would then not be Gauss-Seidel in its order).
-__global__ __launch_bounds__(kBlock) void k_verify_blocks(int64_t n, int nx, int ny,
-                                                          const int32_t *col,
-                                                          const uint8_t *nnz_row, int *bad) {
-  const int64_t i = gtid();
-  if (i >= n) return;
-  int64_t bi, bc;
-  int ci, cc, p;
-  grid_block(i, nx, ny, bi, ci, p);
-  const int m = nnz_row[i];
-  bool ok = true;
-  for (int j = 0; j < m; ++j) {
-    grid_block(col[(int64_t)j * n + i], nx, ny, bc, cc, p);
-    ok = ok && (cc != ci || bc == bi);
-  }
-  if (!ok) atomicOr(bad, 1);
-}
-
 __global__ void k_kahn_advance(int t, int64_t *lptr, const unsigned *tail) {
   lptr[t + 2] = (int64_t)*tail;
 }
@@ -513,6 +486,19 @@ __global__ __launch_bounds__(kBlock) void k_band_key(int64_t n, int nx, int ny, 
   if (i >= n) return;
   const int y = (int)((i / nx) % ny);
   key[i] = color[i] * nbands + y / band;
+}
+
+// Colour visiting order of the multicolour sweeps: colour c (a greedy
+// first-fit id) is visited in position rank[c] of the forward sweep (the
+// backward sweep visits the reverse).
+struct ColourRank {
+  int r[kMaxColors];
+};
+__global__ __launch_bounds__(kBlock) void k_colour_rank(int64_t n, const int32_t *color,
+                                                        ColourRank rank, int32_t *key) {
+  const int64_t i = gtid();
+  if (i >= n) return;
+  key[i] = rank.r[color[i]];
 }
 
 // Stable sort of row ids by a small non-negative integer key (< nbins):
@@ -740,10 +726,9 @@ void finalize_level(Problem &P, int l, NaturalLevel &N, int flags,
   }
   L.nsched = nsched;
 
-  // device order = stable sort of rows by colour; inside a colour, by band
-  // of grid lines when the level is an nx*ny*nz grid in natural order.
-  // Block sweeps: by (block colour, position in block), blocks in natural
-  // order inside each run.
+  // device order = stable sort of rows by the colour's sweep position;
+  // inside a colour, by band of grid lines when the level is an nx*ny*nz
+  // grid in natural order
   L.iperm = P.arena.alloc<int32_t>(n);
   L.perm = P.arena.alloc<int32_t>(n);
   {
@@ -751,52 +736,36 @@ void finalize_level(Problem &P, int l, NaturalLevel &N, int flags,
     if (band == 0) band = kBandLines;
     const bool grid = (int64_t)N.nx * N.ny * N.nz == n && N.nx > 0;
     const int nbands = (grid && band > 1 && N.ny >= 2 * band) ? (N.ny + band - 1) / band : 1;
-    bool blocks = (flags & HPCGX_GS_BLOCKS) && grid && N.nx % 2 == 0 && N.ny % 2 == 0 &&
-                  N.nz % 2 == 0;
-    if (blocks) {
-      TempBuf bad(sizeof(int));
-      HPCGX_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), s));
-      k_verify_blocks<<<grid_for(n), kBlock, 0, s>>>(n, N.nx, N.ny, N.col, N.nnz_row,
-                                                     bad.as<int>());
+    // sweep position of each greedy colour
+    TempBuf ckey(sizeof(int32_t) * n);
+    {
+      ColourRank rank;
+      for (int c = 0; c < kMaxColors; ++c) rank.r[c] = c;
+      const int sel = (flags >> 20) & 0xf;
+      if (sel >= kNumSweepOrders) throw Error(HPCGX_EINVAL, "unknown sweep order");
+      for (int q = 0; q < 8; ++q) rank.r[kSweepOrders[sel][q]] = q;
+      k_colour_rank<<<grid_for(n), kBlock, 0, s>>>(n, color.as<int32_t>(), rank,
+                                                   ckey.as<int32_t>());
       HPCGX_LAUNCHED();
-      int b = 1;
-      HPCGX_HIP(hipMemcpyAsync(&b, bad.p, sizeof b, hipMemcpyDeviceToHost, s));
-      HPCGX_HIP(hipStreamSynchronize(s));
-      blocks = b == 0;
     }
     std::vector<int64_t> cp;
-    L.gs_rows = 1;
-    if (blocks) {
-      TempBuf key(sizeof(int32_t) * n);
-      k_block_key<<<grid_for(n), kBlock, 0, s>>>(n, N.nx, N.ny, key.as<int32_t>());
-      HPCGX_LAUNCHED();
-      constexpr int R = kGsBlockRows;
-      const std::vector<int64_t> bp =
-          sort_rows_by_key(n, key.as<int32_t>(), kGsBlockColors * R, L.iperm, s);
-      for (int c = 0; c <= kMaxColors; ++c) cp.push_back(bp[std::min(c, kGsBlockColors) * R]);
-      for (int c = 0; c < kGsBlockColors; ++c)
-        for (int q = 0; q < R; ++q)  // every block holds one row of each position
-          HPCGX_REQUIRE(bp[c * R + q + 1] - bp[c * R + q] == (bp[c * R + R] - bp[c * R]) / R,
-                        "unequal runs of a block colour");
-      L.gs_rows = R;
-    } else if (nbands > 1) {
+    if (nbands > 1) {
       TempBuf key(sizeof(int32_t) * n);
       k_band_key<<<grid_for(n), kBlock, 0, s>>>(n, N.nx, N.ny, band, nbands,
-                                                color.as<int32_t>(), key.as<int32_t>());
+                                                ckey.as<int32_t>(), key.as<int32_t>());
       HPCGX_LAUNCHED();
       const std::vector<int64_t> bp =
           sort_rows_by_key(n, key.as<int32_t>(), kMaxColors * nbands, L.iperm, s);
       for (int c = 0; c <= kMaxColors; ++c) cp.push_back(bp[(size_t)c * nbands]);
     } else {
-      cp = sort_rows_by_key(n, color.as<int32_t>(), kMaxColors, L.iperm, s);
+      cp = sort_rows_by_key(n, ckey.as<int32_t>(), kMaxColors, L.iperm, s);
     }
-    L.natural_in_colour = nbands == 1 && L.gs_rows == 1;
-    int ncolors = 0;
+    L.natural_in_colour = nbands == 1;
+    std::vector<int64_t> used{cp[0]};  // the non-empty colours, in sweep order
     for (int c = 0; c < kMaxColors; ++c)
-      if (cp[c + 1] > cp[c]) ncolors = c + 1;
-    cp.resize(ncolors + 1);
-    L.color_ptr = cp;
-    L.ncolors = ncolors;
+      if (cp[c + 1] > cp[c]) used.push_back(cp[c + 1]);
+    L.color_ptr = used;
+    L.ncolors = (int)used.size() - 1;
   }
   k_invert<<<grid_for(n), kBlock, 0, s>>>(n, L.iperm, L.perm);
   HPCGX_LAUNCHED();
@@ -881,8 +850,6 @@ void drop_wave(Problem &P, Level &L) {
 }
 
 void build_wave(Problem &P, Level &L, int slab_factor, int rows, hipStream_t s) {
-  if (L.gs_rows > 1)
-    throw Error(HPCGX_EUNSUP, "wavefront sweeps take point colours (no HPCGX_GS_BLOCKS)");
   if (!L.natural_in_colour)
     throw Error(HPCGX_EUNSUP,
                 "wavefront sweeps need natural row order inside each colour "

@@ -42,10 +42,10 @@ struct LaunchLog {
 };
 
 // One colour step (one launch per colour).
-void colour_step(Problem &P, const Level &L, int c, bool bwd, const double *r, double *x,
+void colour_step(Problem &P, const Level &L, int c, const double *r, double *x,
                  hipStream_t s) {
   LaunchLog log(P, L, s);
-  launch_symgs_colour(P, L, c, bwd, r, x, s);
+  launch_symgs_rows(P, L, L.color_ptr[c], L.color_ptr[c + 1], r, x, s);
   log.done();
 }
 
@@ -65,8 +65,8 @@ void symgs(Problem &P, int l, int ordering, const double *r, double *x,
     wave_sweep(P, L, 0, r, x, s);
     wave_sweep(P, L, 1, r, x, s);
   } else if (ordering == HPCGX_ORDER_MULTICOLOR) {
-    for (int c = 0; c < L.ncolors; ++c) colour_step(P, L, c, false, r, x, s);
-    for (int c = L.ncolors - 1; c >= 0; --c) colour_step(P, L, c, true, r, x, s);
+    for (int c = 0; c < L.ncolors; ++c) colour_step(P, L, c, r, x, s);
+    for (int c = L.ncolors - 1; c >= 0; --c) colour_step(P, L, c, r, x, s);
   } else {
     for (int t = 0; t < L.nsched; ++t)
       launch_symgs_list(L, L.level_rows + L.level_ptr[t],

@@ -14,7 +14,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import (CG_GRAPH, CG_TIMERS, GS_BLOCKS, KEEP_SCHEDULE, ORDER_LEXICOGRAPHIC, ORDER_MULTICOLOR,
+from ._lib import (CG_GRAPH, CG_TIMERS, KEEP_SCHEDULE, ORDER_LEXICOGRAPHIC, ORDER_MULTICOLOR,
                    HostLevel, LevelInfo, call)
 
 ORDERS = {"lexicographic": ORDER_LEXICOGRAPHIC, "level": ORDER_LEXICOGRAPHIC,
@@ -54,11 +54,12 @@ def _order(o):
         raise ValueError("unknown ordering %r" % (o,))
 
 
-def _flags(keep_schedule, band_lines, gs_blocks=False):
+def _flags(keep_schedule, band_lines, sweep_order=0):
     if not 0 <= band_lines <= 255:
         raise ValueError("band_lines must be in [0, 255]")
-    return ((KEEP_SCHEDULE if keep_schedule else 0) | (band_lines << 8)
-            | (GS_BLOCKS if gs_blocks else 0))
+    if not 0 <= sweep_order <= 15:
+        raise ValueError("sweep_order must be in [0, 15]")
+    return (KEEP_SCHEDULE if keep_schedule else 0) | (band_lines << 8) | (sweep_order << 20)
 
 
 class Problem:
@@ -72,20 +73,20 @@ class Problem:
     # ---- construction --------------------------------------------------
     @classmethod
     def generate(cls, nx, ny, nz, nlevels=4, keep_schedule=False, band_lines=0,
-                 gs_blocks=False):
+                 sweep_order=0):
         """GenerateProblem + GenerateCoarseProblem + OptimizeProblem on the
         device (main.cpp:133-144, OptimizeProblem.cpp:37).  band_lines: rows
         inside a colour run in bands of this many grid lines (0 = default,
-        1 = natural order; see HPCGX_BAND_LINES).  gs_blocks: multicolour
-        sweeps over 2x2x2 blocks of grid points (HPCGX_GS_BLOCKS)."""
+        1 = natural order; see HPCGX_BAND_LINES).  sweep_order: visiting
+        order of the colours (HPCGX_SWEEP_ORDER; 0 = default, 1 = id order)."""
         h = ctypes.c_void_p()
         call("hpcgx_generate_problem", nx, ny, nz, nlevels,
-             _flags(keep_schedule, band_lines, gs_blocks), ctypes.byref(h))
+             _flags(keep_schedule, band_lines, sweep_order), ctypes.byref(h))
         return cls(h)
 
     @classmethod
     def from_reference_levels(cls, levels, keep_schedule=False, band_lines=0,
-                              reversed_storage=False, gs_blocks=False):
+                              reversed_storage=False, sweep_order=0):
         """OptimizeProblem from host matrices in the reference's layout.
 
         ``levels`` is a list of dicts with numpy arrays ``nnz_row`` (n),
@@ -123,7 +124,7 @@ class Problem:
             arr[l].f2cOperator = None if f2c is None else f2c.ctypes.data
         h = ctypes.c_void_p()
         call("hpcgx_optimize_problem", arr, len(levels),
-             _flags(keep_schedule, band_lines, gs_blocks), ctypes.byref(h))
+             _flags(keep_schedule, band_lines, sweep_order), ctypes.byref(h))
         del keep
         return cls(h)
 
@@ -187,7 +188,7 @@ class Problem:
         return dict(nx=i.nx, ny=i.ny, nz=i.nz, nrow=i.nrow, nnz=i.nnz, ncolors=i.ncolors,
                     nlevels_sched=i.nlevels_sched, ncoarse=i.ncoarse, wave=bool(i.wave),
                     nslabs=i.nslabs, col16=bool(i.col16), wide_tiles=i.wide_tiles,
-                    byte_tiles=i.byte_tiles, gs_rows=i.gs_rows)
+                    byte_tiles=i.byte_tiles)
 
     def nrow(self, level=0):
         return self.levels[level]["nrow"]

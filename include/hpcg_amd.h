@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define HPCGX_ABI_VERSION 2
+#define HPCGX_ABI_VERSION 1
 
 enum {
   HPCGX_OK = 0,
@@ -42,17 +42,8 @@ enum {
 
 /* flags for problem creation */
 enum {
-  HPCGX_KEEP_SCHEDULE = 1, /* keep per-row level and colour ids so that
-                              hpcgx_get_schedule can return them */
-  /* Multicolour sweeps over 2x2x2 blocks of grid points: a level whose rows
-   * come from grids with even dimensions colours its blocks (8 colours by
-   * the parity of the block coordinates) and sweeps each block's 8 rows in
-   * natural order, one work-item per block.  This is Gauss-Seidel in the
-   * row order (block colour, block, row in block), which converges in fewer
-   * PCG iterations than the 8-colour point order.  Other levels keep the
-   * point colours.  hpcgx_get_schedule still returns the greedy point
-   * colours. */
-  HPCGX_GS_BLOCKS = 1 << 16
+  HPCGX_KEEP_SCHEDULE = 1 /* keep per-row level and colour ids so that
+                             hpcgx_get_schedule can return them */
 };
 /* Order of the rows inside a colour, for a level that is an nx*ny*nz grid in
  * natural order: by bands of `lines` grid lines (y), band by band, so that
@@ -62,6 +53,17 @@ enum {
  * natural order.  Results do not depend on it: rows of one colour are
  * independent. */
 #define HPCGX_BAND_LINES(lines) (((lines) & 0xff) << 8)
+/* Visiting order of the greedy colours in the multicolour sweeps (forward;
+ * the backward sweep visits the reverse).  0 = the default: 3, 5, 6, 7, 1,
+ * 2, 4, 0, so that colour 0, which on the 27-point grid holds the points
+ * the restriction injects, is updated first in the backward sweep and its
+ * residual still sees the other colours' updates; 1 = colour id order
+ * 0, 1, ..., 7, which leaves that residual at zero and needs more PCG
+ * iterations (70 instead of 57 to the 50-iteration lexicographic residual
+ * at 512x512x640).  Other values are rejected (HPCGX_EINVAL).  Colours from
+ * 8 on follow in id order.  The device row order follows the visiting
+ * order. */
+#define HPCGX_SWEEP_ORDER(k) (((k) & 0xf) << 20)
 
 typedef struct hpcgx_problem hpcgx_problem; /* device-resident MG hierarchy */
 
@@ -90,9 +92,6 @@ typedef struct {
   int col16;             /* 1: narrow column ids (tile base + offset) built */
   int64_t wide_tiles;    /* tiles of 64 rows that keep 32-bit column ids */
   int64_t byte_tiles;    /* tiles of 64 rows with 8-bit column offsets */
-  int gs_rows;           /* rows per block of the multicolour sweeps: 8 when
-                            the level sweeps 2x2x2 blocks (HPCGX_GS_BLOCKS),
-                            1 for point colours */
 } hpcgx_level_info;
 
 const char *hpcgx_last_error(void);

@@ -156,10 +156,21 @@ def colors(m):
     return out, nc
 
 
-def color_order(color):
-    """Rows sorted by (colour, row): the Gauss-Seidel order of a multicolour
-    sweep that processes colour 0 first."""
-    return np.lexsort((np.arange(color.size), color)).astype(np.int32)
+# Forward visiting orders of the greedy colours 0..7 (colours from 8 on
+# follow in id order).  SWEEP_ORDERS[0] visits colour 0 -- on the 27-point
+# grid the even points, which the restriction injects (ComputeRestriction_ref
+# .cpp:51, GenerateCoarseProblem.cpp:77-79) -- last in the forward sweep, so
+# the backward sweep updates it first; SWEEP_ORDERS[1] is the id order.
+SWEEP_ORDERS = ((3, 5, 6, 7, 1, 2, 4, 0), (0, 1, 2, 3, 4, 5, 6, 7))
+
+
+def color_order(color, sweep=0):
+    """Rows in the Gauss-Seidel order of the multicolour sweep: by the
+    colour's position in SWEEP_ORDERS[sweep], then by row."""
+    color = np.asarray(color)
+    rank = np.arange(max(int(color.max()) + 1, 8), dtype=np.int64)
+    rank[list(SWEEP_ORDERS[sweep])] = np.arange(8)
+    return np.lexsort((np.arange(color.size), rank[color])).astype(np.int32)
 
 
 def hpcg_model(nrow, nnz, nsets, ref_iters, opt_iters):
@@ -228,10 +239,10 @@ class Hierarchy:
         o = None if order is None else np.ascontiguousarray(order, np.int32)
         lib().or_hier_set_order(self._h, l, _p(o))
 
-    def use_color_order(self):
+    def use_color_order(self, sweep=0):
         for l in range(self.nlevels):
             c, _ = colors(self.level(l))
-            self.set_order(l, color_order(c))
+            self.set_order(l, color_order(c, sweep))
 
     def mg(self, r, level=0):
         r = np.ascontiguousarray(r, np.float64)

@@ -36,7 +36,7 @@ def test_python_binding_covers_header():
 def test_library_loads_and_reports_version():
     import hpcg_amd
     L = hpcg_amd.lib()
-    assert L.hpcgx_abi_version() == 2
+    assert L.hpcgx_abi_version() == 1
     for s in declared():
         assert hasattr(L, s)
 

@@ -94,7 +94,10 @@ def test_cg_block_trajectories(hp):
         X = torch.zeros_like(B)
         r = P.cg_block_mixed(B, X, max_iter=it, tolerance=0.0, recompute=10)
         assert r["niters"] == it
-        tol = 1e-9 if it < 10 else 1e-7  # iteration 10 ends with the first recompute
+        # the dots are reassociated (per-column partials of the block
+        # kernels): measured relative drift up to 1.2e-9 before the first
+        # FP64 residual recompute (iteration 10) and 1.1e-7 at iteration 20
+        tol = 5e-9 if it < 10 else 3e-7
         for k in range(s):
             assert r["normr"][k] / r["normr0"][k] == pytest.approx(want[k][it], rel=tol), (it, k)
     # the block solution equals s single-vector solves to rounding

@@ -221,6 +221,53 @@ def test_symgs_bitexact(hp, dims, ordering):
         assert np.array_equal(P.to_numpy(x, l), want), l
 
 
+def device_order(torch, P, l):
+    """Natural rows listed in P's device order on level l."""
+    n = P.nrow(l)
+    pos = P.to_numpy(torch.arange(n, dtype=torch.float64, device="cuda"), l)
+    return np.argsort(pos).astype(np.int32)
+
+
+@pytest.mark.parametrize("sweep", [0, 1])
+def test_sweep_order(hp, torch, sweep):
+    """Device rows follow the colours' sweep positions (HPCGX_SWEEP_ORDER;
+    natural order inside a colour), and the multicolour sweep is the
+    oracle's Gauss-Seidel in that order, bit for bit."""
+    dims = (32, 48, 64)
+    say("sweep order", sweep)
+    P = problem(hp, dims) if sweep == 0 else hp.Problem.generate(*dims, nlevels=4, sweep_order=1)
+    h = hier(dims)
+    for l in range(4):
+        m = h.level(l)
+        order = O.color_order(O.colors(m)[0], sweep)
+        assert np.array_equal(device_order(torch, P, l), order), l
+        r, x0 = rnd(70 + l, h.nrow(l)), rnd(80 + l, h.nrow(l))
+        x = P.to_device_order(x0, l)
+        P.symgs(P.to_device_order(r, l), x, level=l, ordering="multicolor")
+        assert np.array_equal(P.to_numpy(x, l), O.symgs(m, r, x0, order)), l
+    with pytest.raises(hp.HpcgxError):
+        hp.Problem.generate(16, 16, 16, nlevels=4, sweep_order=2)
+
+
+def test_sweep_order_iterations(hp, torch):
+    """To the residual 50 lexicographic iterations reach, PCG with the
+    default colour order needs fewer iterations than with the id order
+    (oracle at 32x48x64: 53 and 62)."""
+    dims = (32, 48, 64)
+    say("sweep order iterations")
+    P0 = problem(hp, dims)
+    P1 = hp.Problem.generate(*dims, nlevels=4, sweep_order=1)
+    b, x, _ = P0.vectors()
+    ref = P0.cg(b, x, max_iter=50, tolerance=0.0, ordering="lexicographic")
+    tol = ref["normr"] / ref["normr0"]
+    its = []
+    for P in (P0, P1):
+        b, x, _ = P.vectors()
+        its.append(P.cg(b, x, max_iter=300, tolerance=tol, ordering="multicolor")["niters"])
+    assert abs(its[0] - 53) <= 1 and abs(its[1] - 62) <= 1, its
+    del P1
+
+
 @pytest.mark.parametrize("dims", [(16, 16, 16), (32, 48, 64)])
 def test_pair_rows_bitexact(hp, dims):
     """Colour steps with two rows per thread (16-byte matrix loads) keep the
