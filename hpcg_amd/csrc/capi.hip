@@ -110,6 +110,8 @@ void alloc_solver_state(hpcgx::Problem &P) {
   HPCGX_HIP(hipMemset(P.scalars, 0, sizeof(double) * hpcgx::kNumScalars));
   HPCGX_HIP(hipMemset(P.wave_err, 0, sizeof(int)));
   P.wave_grid = hpcgx::wave_grid_size();
+  P.nchunk_partials = P.L[0].nchunks;
+  P.chunk_partials = P.arena.alloc<double>(P.nchunk_partials);
 }
 
 // shared tail of both setup paths: schedules, reordering, links
@@ -400,6 +402,10 @@ int hpcgx_tune(hpcgx_problem *p, const char *key, double value) {
       P.stream_p = value != 0.0;
       return;
     }
+    if (k == "spmv_dot_chunks") {
+      P.spmv_dot_chunked = value != 0.0;
+      return;
+    }
     if (k == "fp16_pairs") {
       P.use_pairs = value != 0.0;
       return;
@@ -501,10 +507,12 @@ int hpcgx_autotune(hpcgx_problem *p, int ordering, int iterations, char *report,
     for (int l = 0; l < P.nlevels; ++l) any_c16 = any_c16 || P.L[l].c16;
     std::vector<Knob> knobs;
     if (any_c16) knobs.push_back({"col8", {1, 0}});
-    if (P.fleet.count)
+    if (P.fleet.count) {
       knobs.push_back({"fleet_blocks", {512, 1024, 2048, 4096}});
-    else
+    } else {
+      knobs.push_back({"spmv_dot_chunks", {0, 1}});
       knobs.push_back({"spmv_dot_blocks", {1024, 2048, 4096, 8192}});
+    }
     knobs.push_back({"spmv_order", {0, 1}});
     knobs.push_back({"xcd_remap", {0, 1}});
     std::string log;

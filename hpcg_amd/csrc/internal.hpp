@@ -194,6 +194,12 @@ struct Problem {
   bool use_pairs = true;      // FP32 colour steps of 8-bit tiles read FP16 value pairs
   bool stream_p = false;      // p update with non-temporal accesses ("stream_p")
   int spmv_dot_blocks = 4096; // persistent grid of the fused SpMV + dot
+  // The fused SpMV + dot with one workgroup per row chunk of the fine level
+  // ("spmv_dot_chunks"): the p.Ap partial of each chunk, summed in fixed
+  // order by a second launch.
+  bool spmv_dot_chunked = true;
+  int64_t nchunk_partials = 0;
+  double *chunk_partials = nullptr;   // [nchunk_partials]
   bool mixed_ready = false;   // FP16/FP32 copies built
   bool half_exact = false;    // every off-diagonal value is exact in FP16
   float *r32 = nullptr, *z32 = nullptr;  // FP32 input / output of the V-cycle

@@ -530,6 +530,23 @@ __device__ inline void grid_finish(double s, double *partials, unsigned *ticket,
   }
 }
 
+// p.Ap from the per-chunk partials of k_spmv_dot_c / k_spmv_dot_hc, in a
+// launch of its own (the launch boundary orders the partials' stores before
+// these loads): thread t of workgroup w sums chunks w*per + t, + kBlock, ...
+// in order, then block_sum and grid_finish, so the order is fixed.
+__global__ __launch_bounds__(kBlock) void k_chunk_dot(int64_t nchunks,
+                                                      const double *__restrict__ cpart,
+                                                      double *partials, unsigned *ticket,
+                                                      double *scalars) {
+  const int64_t per = (nchunks + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = (int64_t)blockIdx.x * per;
+  const int64_t hi = lo + per < nchunks ? lo + per : nchunks;
+  double s = 0.0;
+  for (int64_t q = lo + threadIdx.x; q < hi; q += kBlock) s += cpart[q];
+  s = block_sum(s);
+  grid_finish(s, partials, ticket, scalars, nullptr, kFinishAlpha);
+}
+
 constexpr int64_t kStreamRows = (int64_t)1 << 24;
 inline bool stream_vec(int64_t n) { return n >= kStreamRows; }
 
@@ -599,6 +616,72 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(
 // in slot order with FP64 products and sums: the FP64 SpMV whenever the
 // off-diagonal values are exact in FP16 (they are -1 here), at 4 instead of
 // 9 matrix bytes per entry.  32-bit tiles read the FP64 ELL.
+// k_spmv_dot with one workgroup per row chunk (logical chunk b of the
+// launch's workgroup, XCD-remapped when xcd): chunk b's p.Ap partial goes to
+// cpart[b] for k_chunk_dot.
+template <bool STREAM, bool C16>
+__global__ __launch_bounds__(kBlock) void k_spmv_dot_c(
+    const int64_t *__restrict__ chunks, int64_t n, Col16 C, const int32_t *__restrict__ col,
+    const double *__restrict__ val, const double *__restrict__ p, double *__restrict__ Ap,
+    double *__restrict__ cpart, bool xcd) {
+  const int64_t b = xcd_block(blockIdx.x, gridDim.x, xcd);
+  const int64_t i = chunks[2 * b] + threadIdx.x;
+  double s = 0.0;
+  if (i < chunks[2 * b + 1]) {
+    const double y = C16 ? row_dot16<STREAM>(C, col, val, p, n, i)
+                         : row_dot<STREAM>(col, val, p, n, i);
+    Ap[i] = y;
+    s = p[i] * y;
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) cpart[b] = s;
+}
+
+// Row i of A p in FP64 from the FP16 off-diagonal values (packed words)
+// and the FP64 diagonal, in slot order; 32-bit tiles read the FP64 matrix.
+template <bool STREAM>
+__device__ inline double row_dot_h(const Col16 &C, const uint32_t *__restrict__ vc,
+                                   const uint8_t *__restrict__ dslot,
+                                   const double *__restrict__ diag,
+                                   const int32_t *__restrict__ col,
+                                   const double *__restrict__ val,
+                                   const double *__restrict__ p, int64_t n, int64_t i) {
+  const int t = tile_of(i);
+  if (C.info[t] == kTile32) return row_dot<STREAM>(col, val, p, n, i);
+  const int32_t *__restrict__ base = C.base + t;
+  const int ds = mload<STREAM>(dslot + i);
+  const double di = mload<STREAM>(diag + i);
+  double y = 0.0;
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const uint32_t wd = mload<STREAM>(vc + (int64_t)j * n + i);
+    const int32_t c = base[(int64_t)j * C.ntiles] + (int32_t)(wd >> 16);
+    const double a = j == ds ? di : (double)__builtin_bit_cast(_Float16, (uint16_t)(wd & 0xffffu));
+    y += a * p[c];
+  }
+  return y;
+}
+
+// k_spmv_dot_h with one workgroup per row chunk; partials as k_spmv_dot_c.
+template <bool STREAM>
+__global__ __launch_bounds__(kBlock) void k_spmv_dot_hc(
+    const int64_t *__restrict__ chunks, int64_t n, Col16 C, const uint32_t *__restrict__ vc,
+    const uint8_t *__restrict__ dslot, const double *__restrict__ diag,
+    const int32_t *__restrict__ col, const double *__restrict__ val,
+    const double *__restrict__ p, double *__restrict__ Ap, double *__restrict__ cpart,
+    bool xcd) {
+  const int64_t b = xcd_block(blockIdx.x, gridDim.x, xcd);
+  const int64_t i = chunks[2 * b] + threadIdx.x;
+  double s = 0.0;
+  if (i < chunks[2 * b + 1]) {
+    const double y = row_dot_h<STREAM>(C, vc, dslot, diag, col, val, p, n, i);
+    mstore<STREAM>(Ap + i, y);
+    s = p[i] * y;
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) cpart[b] = s;
+}
+
 template <bool STREAM>
 __global__ __launch_bounds__(kBlock) void k_spmv_dot_h(
     int64_t nchunks, const int64_t *__restrict__ chunks, int64_t n, Col16 C,
@@ -612,24 +695,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot_h(
   for (int64_t w = lo + slot; w < hi; w += per) {
     const int64_t i = chunks[2 * w] + threadIdx.x;
     if (i < chunks[2 * w + 1]) {
-      const int t = tile_of(i);
-      double y;
-      if (C.info[t] == kTile32) {
-        y = row_dot<STREAM>(col, val, p, n, i);
-      } else {
-        const int32_t *__restrict__ base = C.base + t;
-        const int ds = mload<STREAM>(dslot + i);
-        const double di = mload<STREAM>(diag + i);
-        y = 0.0;
-#pragma unroll
-        for (int j = 0; j < kSlots; ++j) {
-          const uint32_t wd = mload<STREAM>(vc + (int64_t)j * n + i);
-          const int32_t c = base[(int64_t)j * C.ntiles] + (int32_t)(wd >> 16);
-          const double a = j == ds ? di
-                                   : (double)__builtin_bit_cast(_Float16, (uint16_t)(wd & 0xffffu));
-          y += a * p[c];
-        }
-      }
+      const double y = row_dot_h<STREAM>(C, vc, dslot, diag, col, val, p, n, i);
       mstore<STREAM>(Ap + i, y);
       s += p[i] * y;
     }
@@ -916,8 +982,35 @@ void launch_dot(Problem &P, int64_t n, const double *x, const double *y,
   HPCGX_LAUNCHED();
 }
 
+namespace {
+bool chunked(const Problem &P, const Level &L) {
+  return P.spmv_dot_chunked && P.chunk_partials && L.nchunks == P.nchunk_partials;
+}
+void chunk_dot(Problem &P, const Level &L, hipStream_t s) {
+  k_chunk_dot<<<reduce_grid(L.nchunks), kBlock, 0, s>>>(L.nchunks, P.chunk_partials, P.partials,
+                                                        P.ticket, P.scalars);
+  HPCGX_LAUNCHED();
+}
+}  // namespace
+
 void launch_spmv_dot(Problem &P, const Level &L, const double *p, double *Ap,
                      hipStream_t s) {
+  if (chunked(P, L)) {
+    const Col16 C = col16_of(P, L);
+    const unsigned G = (unsigned)L.nchunks;
+#define HPCGX_SDC(ST, C16)                                                                   \
+  k_spmv_dot_c<ST, C16><<<G, kBlock, 0, s>>>(L.chunks, L.n, C, L.col, L.val, p, Ap,           \
+                                             P.chunk_partials, L.xcd)
+    if (use_col16(P, L, 0)) {
+      if (L.stream) HPCGX_SDC(true, true); else HPCGX_SDC(false, true);
+    } else {
+      if (L.stream) HPCGX_SDC(true, false); else HPCGX_SDC(false, false);
+    }
+#undef HPCGX_SDC
+    HPCGX_LAUNCHED();
+    chunk_dot(P, L, s);
+    return;
+  }
   int64_t g = std::min<int64_t>(L.nchunks, std::min(P.spmv_dot_blocks, kMaxPartials));
   g = std::max<int64_t>(8, (g + 7) / 8 * 8);
   const Col16 C = col16_of(P, L);
@@ -944,9 +1037,21 @@ void launch_spmv_dot_half(Problem &P, const Level &L, const double *p, double *A
     launch_spmv_dot(P, L, p, Ap, s);
     return;
   }
+  const Col16 C = col16_of(P, L);
+  if (chunked(P, L)) {
+    const unsigned G = (unsigned)L.nchunks;
+    if (L.stream)
+      k_spmv_dot_hc<true><<<G, kBlock, 0, s>>>(L.chunks, L.n, C, L.vc, L.dslot, L.diag, L.col,
+                                               L.val, p, Ap, P.chunk_partials, L.xcd);
+    else
+      k_spmv_dot_hc<false><<<G, kBlock, 0, s>>>(L.chunks, L.n, C, L.vc, L.dslot, L.diag, L.col,
+                                                L.val, p, Ap, P.chunk_partials, L.xcd);
+    HPCGX_LAUNCHED();
+    chunk_dot(P, L, s);
+    return;
+  }
   int64_t g = std::min<int64_t>(L.nchunks, std::min(P.spmv_dot_blocks, kMaxPartials));
   g = std::max<int64_t>(8, (g + 7) / 8 * 8);
-  const Col16 C = col16_of(P, L);
   if (L.stream)
     k_spmv_dot_h<true><<<(unsigned)g, kBlock, 0, s>>>(L.nchunks, L.chunks, L.n, C, L.vc,
                                                       L.dslot, L.diag, L.col, L.val, p, Ap,

@@ -342,6 +342,11 @@ int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, i
  *   "spmv_dot_blocks"  workgroups of the persistent SpMV + dot kernel
  *                      (8..8192, rounded down to a multiple of 8; default
  *                      4096).  Changes the dot's summation order.
+ *   "spmv_dot_chunks"  1/0: the fused SpMV + dot of CG runs one workgroup
+ *                      per row chunk and a second launch sums the chunks'
+ *                      p.Ap partials in a fixed order (default 1; 0: the
+ *                      persistent kernel, whose workgroups sweep the chunks
+ *                      in strides).  Changes the dot's summation order.
  *   "spmv_order"       0: the SpMV kernels visit row chunks interleaved over
  *                      the colour blocks (default); 1: colour block by
  *                      colour block.  Changes the dots' summation order.
@@ -352,7 +357,7 @@ int hpcgx_tune(hpcgx_problem *p, const char *key, double value);
 
 /* No counterpart in the reference: picks the tuning knobs for this problem
  * and machine by timing `iterations` PCG iterations (plus one warm-up) per
- * candidate, one knob at a time ("col8", "spmv_dot_blocks", "spmv_order", "xcd_remap";
+ * candidate, one knob at a time ("col8", "spmv_dot_chunks", "spmv_dot_blocks", "spmv_order", "xcd_remap";
  * a fleet times hpcgx_cg_fleet and tunes "fleet_blocks" in place of
  * "spmv_dot_blocks"), keeping the fastest value of each.  Writes one line per timing and per
  * choice into `report` (may be NULL).  Uses the default stream; synchronous.
