@@ -281,6 +281,63 @@ __global__ __launch_bounds__(kBlock) void k_bprolong32_rows(int64_t nc,
 // the off-diagonal values are exact in FP16), else the FP64 ELL.  Work-items
 // gtid(), gtid() + stride, ... (stride a multiple of kTile), W per row.
 template <bool STREAM, int S, bool H>
+__device__ inline void bspmv_row(int64_t i, int o, int64_t n, const Col16 &C,
+                                 const uint32_t *__restrict__ vc,
+                                 const uint8_t *__restrict__ dslot,
+                                 const double *__restrict__ diag,
+                                 const int32_t *__restrict__ col,
+                                 const double *__restrict__ val,
+                                 const double *__restrict__ X, double (&y)[2]) {
+  constexpr int V = 2;
+  double xv[V];
+#pragma unroll
+  for (int k = 0; k < V; ++k) y[k] = 0.0;
+  const int t = H ? tile_of(i) : 0;
+  if (H && C.info[t] != kTile32) {
+    const int32_t *__restrict__ base = C.tbase + (int64_t)t * kBaseStride;
+    const int ds = dslot[i];
+    const double di = diag[i];
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const uint32_t w = mload<STREAM>(vc + (int64_t)j * n + i);
+      const int32_t c = base[j] + (int32_t)(w >> 16);
+      const double a = j == ds ? di : (double)vc_val(w);
+      ldv<V>(X + (int64_t)c * S + o, xv);
+#pragma unroll
+      for (int k = 0; k < V; ++k) y[k] += a * xv[k];
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const int64_t e = (int64_t)j * n + i;
+      const double a = mload<STREAM>(val + e);
+      ldv<V>(X + (int64_t)mload<STREAM>(col + e) * S + o, xv);
+#pragma unroll
+      for (int k = 0; k < V; ++k) y[k] += a * xv[k];
+    }
+  }
+}
+
+// Column sums of the workgroup's acc values (thread q holds columns
+// (q % W) * V + v): for column k, the threads holding it in thread order.
+// Valid in threads k < S.
+template <int S>
+__device__ inline double bcolumn_sum(const double (&acc)[2]) {
+  constexpr int V = 2, W = S / V;
+  __shared__ double red[kBlock][V];
+#pragma unroll
+  for (int k = 0; k < V; ++k) red[threadIdx.x][k] = acc[k];
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x < (unsigned)S) {
+    const int k = threadIdx.x, sub = k / V, v = k % V;
+    for (int q = sub; q < kBlock; q += W) t += red[q][v];
+  }
+  __syncthreads();
+  return t;
+}
+
+template <bool STREAM, int S, bool H>
 __global__ __launch_bounds__(kBlock) void k_bspmv_rows(
     int64_t n, Col16 C, const uint32_t *__restrict__ vc, const uint8_t *__restrict__ dslot,
     const double *__restrict__ diag, const int32_t *__restrict__ col,
@@ -295,32 +352,7 @@ __global__ __launch_bounds__(kBlock) void k_bspmv_rows(
   for (int64_t g = gtid(); g < n * W; g += stride) {
     const int64_t i = g / W;
     double y[V], xv[V];
-#pragma unroll
-    for (int k = 0; k < V; ++k) y[k] = 0.0;
-    const int t = H ? tile_of(i) : 0;
-    if (H && C.info[t] != kTile32) {
-      const int32_t *__restrict__ base = C.tbase + (int64_t)t * kBaseStride;
-      const int ds = dslot[i];
-      const double di = diag[i];
-#pragma unroll
-      for (int j = 0; j < kSlots; ++j) {
-        const uint32_t w = mload<STREAM>(vc + (int64_t)j * n + i);
-        const int32_t c = base[j] + (int32_t)(w >> 16);
-        const double a = j == ds ? di : (double)vc_val(w);
-        ldv<V>(X + (int64_t)c * S + o, xv);
-#pragma unroll
-        for (int k = 0; k < V; ++k) y[k] += a * xv[k];
-      }
-    } else {
-#pragma unroll
-      for (int j = 0; j < kSlots; ++j) {
-        const int64_t e = (int64_t)j * n + i;
-        const double a = mload<STREAM>(val + e);
-        ldv<V>(X + (int64_t)mload<STREAM>(col + e) * S + o, xv);
-#pragma unroll
-        for (int k = 0; k < V; ++k) y[k] += a * xv[k];
-      }
-    }
+    bspmv_row<STREAM, S, H>(i, o, n, C, vc, dslot, diag, col, val, X, y);
     stv<V>(Y + i * S + o, y);
     if (dot) {
       ldv<V>(X + i * S + o, xv);
@@ -329,19 +361,57 @@ __global__ __launch_bounds__(kBlock) void k_bspmv_rows(
     }
   }
   if (!dot) return;
-  // column k = (lane's o) + v: the workgroup's threads holding it, in order
-  __shared__ double red[kBlock][V];
-#pragma unroll
-  for (int k = 0; k < V; ++k) red[threadIdx.x][k] = acc[k];
-  __syncthreads();
-  if (threadIdx.x < (unsigned)S) {
-    const int k = threadIdx.x, sub = k / V, v = k % V;
-    double t = 0.0;
-    for (int q = sub; q < kBlock; q += W) t += red[q][v];
-    partials[(int64_t)blockIdx.x * S + k] = t;
-  }
+  const double t = bcolumn_sum<S>(acc);
+  if (threadIdx.x < (unsigned)S) partials[(int64_t)blockIdx.x * S + threadIdx.x] = t;
   __syncthreads();
   finish_tail_s(S, partials, ticket, sc, kBAlpha);
+}
+
+// The same with the rows of one chunk (Level::chunks) per W workgroups, in
+// the chunk list's order: workgroup q takes rows [u * kBlock / W, ...) of
+// chunk q / W, u = q % W.  With dot, its column sums go to cpart[q * S + k]
+// for k_chunk_bdot.
+template <bool STREAM, int S, bool H>
+__global__ __launch_bounds__(kBlock) void k_bspmv_chunks(
+    const int64_t *__restrict__ chunks, int64_t n, Col16 C, const uint32_t *__restrict__ vc,
+    const uint8_t *__restrict__ dslot, const double *__restrict__ diag,
+    const int32_t *__restrict__ col, const double *__restrict__ val,
+    const double *__restrict__ X, double *__restrict__ Y, bool dot, double *__restrict__ cpart) {
+  constexpr int V = 2, W = S / V;
+  const int64_t q = blockIdx.x, b = q / W;
+  const int64_t i = chunks[2 * b] + (q % W) * (kBlock / W) + threadIdx.x / W;
+  const int o = (int)(threadIdx.x % W) * V;
+  double acc[V] = {0.0, 0.0};
+  if (i < chunks[2 * b + 1]) {
+    double y[V], xv[V];
+    bspmv_row<STREAM, S, H>(i, o, n, C, vc, dslot, diag, col, val, X, y);
+    stv<V>(Y + i * S + o, y);
+    if (dot) {
+      ldv<V>(X + i * S + o, xv);
+#pragma unroll
+      for (int k = 0; k < V; ++k) acc[k] = xv[k] * y[k];
+    }
+  }
+  if (!dot) return;
+  const double t = bcolumn_sum<S>(acc);
+  if (threadIdx.x < (unsigned)S) cpart[q * S + threadIdx.x] = t;
+}
+
+// p_k . (A p)_k from k_bspmv_chunks's partials (a launch of its own, so
+// their stores are ordered before these loads), in fixed order: thread t of
+// workgroup w sums column t % s of parts lo + t / s, + kBlock / s, ...;
+// block_finish_s then adds the threads of each column in order.
+__global__ __launch_bounds__(kBlock) void k_chunk_bdot(int64_t nparts, int s,
+                                                       const double *__restrict__ cpart,
+                                                       double *partials, unsigned *ticket,
+                                                       double *sc) {
+  const int64_t per = (nparts + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = (int64_t)blockIdx.x * per;
+  const int64_t hi = lo + per < nparts ? lo + per : nparts;
+  const int k = threadIdx.x % s, step = kBlock / s;
+  double t = 0.0;
+  for (int64_t p = lo + threadIdx.x / s; p < hi; p += step) t += cpart[p * s + k];
+  block_finish_s(t, s, partials, ticket, sc, kBAlpha);
 }
 
 // ---- FP64 outer kernels ----------------------------------------------------
@@ -596,6 +666,26 @@ void spmv_rows_s(const Problem &P, const Level &L, const double *X, double *Y, b
   const Col16 C = col16_of(P, L);
   const bool h = L.vc && P.half_exact && use_col16(P, L, 0);
   auto &B = P.blk;
+  constexpr int W = S / 2;
+  if (P.spmv_dot_chunked && B.cpart && B.ncpart >= L.nchunks * W * S) {
+    const unsigned G = (unsigned)(L.nchunks * W);
+#define HPCGX_BSC(ST, HH)                                                                   \
+  k_bspmv_chunks<ST, S, HH><<<G, kBlock, 0, st>>>(L.chunks, L.n, C, L.vc, L.dslot, L.diag,    \
+                                                  L.col, L.val, X, Y, dot, B.cpart)
+    if (L.stream && h) HPCGX_BSC(true, true);
+    else if (L.stream) HPCGX_BSC(true, false);
+    else if (h) HPCGX_BSC(false, true);
+    else HPCGX_BSC(false, false);
+#undef HPCGX_BSC
+    HPCGX_LAUNCHED();
+    if (dot) {
+      const int64_t nparts = L.nchunks * W;
+      const unsigned R = (unsigned)std::min<int64_t>(256, nparts);
+      k_chunk_bdot<<<R, kBlock, 0, st>>>(nparts, S, B.cpart, B.partials, B.ticket, B.scalars);
+      HPCGX_LAUNCHED();
+    }
+    return;
+  }
   if (L.stream && h)
     k_bspmv_rows<true, S, true><<<g, kBlock, 0, st>>>(L.n, C, L.vc, L.dslot, L.diag, L.col,
                                                       L.val, X, Y, dot, B.partials, B.ticket,
@@ -689,6 +779,10 @@ void prepare_block(Problem &P, int s) {
     B.xc32[l] = P.arena.alloc<float>((size_t)P.L[l].nc * s);
   }
   B.partials = P.arena.alloc<double>((size_t)kMaxPartials * s);
+  if (rows_kernels(s)) {  // k_bspmv_chunks: s/2 workgroups per chunk, s sums each
+    B.ncpart = P.L[0].nchunks * (s / 2) * s;
+    B.cpart = P.arena.alloc<double>((size_t)B.ncpart);
+  }
   B.ticket = P.arena.alloc<unsigned>(1);
   HPCGX_HIP(hipMemset(B.ticket, 0, sizeof(unsigned)));
   B.scalars = P.arena.alloc<double>((size_t)5 * s);
@@ -709,6 +803,9 @@ void Problem::BlockState::release(Problem &P) {
     P.arena.release(xc32[l], sizeof(float) * P.L[l].nc * s);
   }
   P.arena.release(partials, sizeof(double) * kMaxPartials * s);
+  if (cpart) P.arena.release(cpart, sizeof(double) * ncpart);
+  cpart = nullptr;
+  ncpart = 0;
   P.arena.release(ticket, sizeof(unsigned));
   P.arena.release(scalars, sizeof(double) * 5 * s);
   s = 0;

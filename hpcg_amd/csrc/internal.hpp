@@ -212,6 +212,8 @@ struct Problem {
     float *R32 = nullptr, *Z32 = nullptr;
     float *rc32[kMaxLevels] = {}, *xc32[kMaxLevels] = {};
     double *partials = nullptr;
+    double *cpart = nullptr;  // per-workgroup column sums of k_bspmv_chunks
+    int64_t ncpart = 0;
     unsigned *ticket = nullptr;
     double *scalars = nullptr;
     void release(Problem &P);

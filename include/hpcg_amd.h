@@ -342,11 +342,13 @@ int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, i
  *   "spmv_dot_blocks"  workgroups of the persistent SpMV + dot kernel
  *                      (8..8192, rounded down to a multiple of 8; default
  *                      4096).  Changes the dot's summation order.
- *   "spmv_dot_chunks"  1/0: the fused SpMV + dot of CG runs one workgroup
- *                      per row chunk and a second launch sums the chunks'
- *                      p.Ap partials in a fixed order (default 1; 0: the
- *                      persistent kernel, whose workgroups sweep the chunks
- *                      in strides).  Changes the dot's summation order.
+ *   "spmv_dot_chunks"  1/0: the fused SpMV + dot of CG, of the mixed CG and
+ *                      of the block PCG (s = 4, 8, 16) runs one workgroup
+ *                      per row chunk (s/2 per chunk in the block PCG), and
+ *                      a second launch sums the partials in a fixed order
+ *                      (default 1; 0: the persistent kernels, whose
+ *                      workgroups sweep the rows in strides).  Changes the
+ *                      dots' summation order.
  *   "spmv_order"       0: the SpMV kernels visit row chunks interleaved over
  *                      the colour blocks (default); 1: colour block by
  *                      colour block.  Changes the dots' summation order.
