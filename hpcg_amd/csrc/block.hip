@@ -221,7 +221,7 @@ __global__ __launch_bounds__(kBlock) void k_bgs32_rows(int64_t begin, int64_t en
   bgs32_row<STREAM, S, C16>(i, (int)(g % W) * 4, n, C, vc, col, val16, diag32, R, X);
 }
 
-// rc = rf[f2c] - (A xf)[f2c] in fine order (Level::rperm): mixed.hip's
+// rc = rf[f2c] - (A xf)[f2c] in fine order (Level::ftile, rperm): mixed.hip's
 // row_dot32 for each column.
 template <bool STREAM, int S, bool C16>
 __global__ __launch_bounds__(kBlock) void k_bresid_restrict32_rows(
