@@ -12,10 +12,11 @@
 //    50-79 (which visits rows in natural order and looks only at columns
 //    j < i), computed in parallel because a row's colour depends only on
 //    rows of lower levels.
-// Rows are then renumbered colour by colour (natural order inside a colour)
-// and the 27-slot ELL is rebuilt in that order, with the slots of every tile
-// of 64 rows aligned by natural column offset so that the column ids of a
-// slot fit 16 bits above a per-tile base (k_tile_layout).
+// Rows are then renumbered colour by colour, in the colours' sweep order
+// (kSweepOrders; natural order inside a colour), and the 27-slot ELL is
+// rebuilt in that order, with the slots of every tile of 64 rows aligned by
+// natural column offset so that the column ids of a slot fit 16 bits above a
+// per-tile base (k_tile_layout).
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>

@@ -102,7 +102,8 @@ int hpcgx_abi_version(void);
 /* replaces OptimizeProblem (OptimizeProblem.cpp:37-100 of the reference,
  * declared OptimizeProblem.hpp:22): converts every level's CSR to a
  * column-major 27-slot ELL in HBM, builds the lexicographic level schedule
- * and the greedy colour schedule on the device, and reorders rows by colour.
+ * and the greedy colour schedule on the device, and reorders rows by colour
+ * (in the colours' sweep order, HPCGX_SWEEP_ORDER).
  * `levels[0]` is the fine level; levels[l].f2cOperator maps level l+1 rows to
  * level l rows (GenerateCoarseProblem.cpp:71-82). */
 int hpcgx_optimize_problem(const hpcgx_host_level *levels, int nlevels,
