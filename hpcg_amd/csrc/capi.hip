@@ -406,6 +406,10 @@ int hpcgx_tune(hpcgx_problem *p, const char *key, double value) {
       P.spmv_dot_chunked = value != 0.0;
       return;
     }
+    if (k == "affine8") {
+      P.use_affine = value != 0.0;
+      return;
+    }
     if (k == "fp16_pairs") {
       P.use_pairs = value != 0.0;
       return;

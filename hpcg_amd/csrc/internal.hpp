@@ -157,6 +157,10 @@ struct Level {
   int64_t byte_tiles = 0;   // kTile8 tiles
   uint16_t *col16 = nullptr;
   uint32_t *off8 = nullptr;  // [kSlotWords * n]
+  // per kTile8 tile: bit g set when every row's offset word g is
+  // lane * 0x01010101 (each of its slots' columns is the slot base plus the
+  // row's position in the tile), so the kernels need not load it
+  uint8_t *amask = nullptr;
   int32_t *cbase = nullptr;
   int32_t *tbase = nullptr;
   int32_t *tinfo = nullptr;
@@ -192,6 +196,7 @@ struct Problem {
   bool use_c16 = true;        // read narrow column ids where a level has them
   bool use_c8 = true;         // ... 8-bit ones where a tile has them
   bool use_pairs = true;      // FP32 colour steps of 8-bit tiles read FP16 value pairs
+  bool use_affine = true;     // FP32 V-cycle skips the offset words Level::amask marks ("affine8")
   bool stream_p = false;      // p update with non-temporal accesses ("stream_p")
   int spmv_dot_blocks = 4096; // persistent grid of the fused SpMV + dot
   // The fused SpMV + dot with one workgroup per row chunk of the fine level
@@ -291,10 +296,22 @@ struct Col16 {
   const int32_t *info;
   int64_t ntiles;
   int use8;  // read 8-bit offsets in kTile8 tiles (else their 16-bit ids)
+  const uint8_t *amask;  // Level::amask, or null: load every offset word
 };
 inline Col16 col16_of(const Problem &P, const Level &L) {
-  return Col16{L.col16, L.off8, L.cbase, L.tbase, L.tinfo, L.ntiles, P.use_c8 ? 1 : 0};
+  return Col16{L.col16, L.off8, L.cbase, L.tbase, L.tinfo, L.ntiles, P.use_c8 ? 1 : 0,
+               P.use_affine ? L.amask : nullptr};
 }
+// Offset word g of row i on a kTile8 tile: loaded, or lane * 0x01010101
+// where the tile's Level::amask bit g says so (amask uniform per tile).
+template <bool STREAM>
+__device__ inline uint32_t off8_word(const uint32_t *__restrict__ off8, unsigned amask, int g,
+                                     int64_t n, int64_t i) {
+  if ((amask >> g) & 1u) return (uint32_t)(i % kTile) * 0x01010101u;
+  if (STREAM) return __builtin_nontemporal_load(off8 + (int64_t)g * n + i);
+  return off8[(int64_t)g * n + i];
+}
+
 // Fine row of fine-order transfer row t (Level::ftile).
 __device__ inline int64_t fine_row(const int32_t *__restrict__ ftile, int64_t t) {
   return (int64_t)ftile[t / kTile] * kTile + t % kTile;

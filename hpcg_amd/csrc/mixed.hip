@@ -194,9 +194,10 @@ __device__ inline void gs32_row8(const Col16 &C, const uint32_t *__restrict__ vp
     return;
   }
   const int32_t *__restrict__ base = C.tbase + (int64_t)t * kBaseStride;
+  const unsigned am = C.amask ? C.amask[t] : 0u;
   uint32_t o8[kSlotWords], pv[kPairWords];
 #pragma unroll
-  for (int g = 0; g < kSlotWords; ++g) o8[g] = mload<STREAM>(C.off8 + (int64_t)g * n + i);
+  for (int g = 0; g < kSlotWords; ++g) o8[g] = off8_word<STREAM>(C.off8, am, g, n, i);
 #pragma unroll
   for (int w = 0; w < kPairWords; ++w) pv[w] = mload<STREAM>(vp + (int64_t)w * n + i);
   float sum = r[i];
@@ -222,9 +223,10 @@ __device__ inline float row_dot32_8(const Col16 &C, const uint32_t *__restrict__
   const int info = C.info[t];
   if (info != kTile8 || !C.use8) return row_dot32_16<STREAM>(C, vc, i, n, col, val16, diag32, x);
   const int32_t *__restrict__ base = C.tbase + (int64_t)t * kBaseStride;
+  const unsigned am = C.amask ? C.amask[t] : 0u;
   uint32_t o8[kSlotWords], pv[kPairWords];
 #pragma unroll
-  for (int g = 0; g < kSlotWords; ++g) o8[g] = mload<STREAM>(C.off8 + (int64_t)g * n + i);
+  for (int g = 0; g < kSlotWords; ++g) o8[g] = off8_word<STREAM>(C.off8, am, g, n, i);
 #pragma unroll
   for (int w = 0; w < kPairWords; ++w) pv[w] = mload<STREAM>(vp + (int64_t)w * n + i);
   float s = 0.0f;

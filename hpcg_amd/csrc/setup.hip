@@ -345,7 +345,8 @@ __device__ inline int wave_max_i32(int v) {
 __global__ __launch_bounds__(kBlock) void k_tile_layout(
     int64_t n, int64_t ntiles, const int32_t *__restrict__ iperm, int32_t *col,
     double *val, const uint8_t *__restrict__ nnz_row, uint8_t *dslot,
-    uint16_t *col16, uint32_t *off8, int32_t *cbase, int32_t *tbase, int32_t *tinfo) {
+    uint16_t *col16, uint32_t *off8, int32_t *cbase, int32_t *tbase, int32_t *tinfo,
+    uint8_t *amask) {
   const int64_t t = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
   if (t >= ntiles) return;  // whole wavefronts
   const int lane = threadIdx.x & 63;
@@ -373,11 +374,15 @@ __global__ __launch_bounds__(kBlock) void k_tile_layout(
     ++nu;
   }
   if (!fits) {
-    if (lane == 0) tinfo[t] = kTile32;
+    if (lane == 0) {
+      tinfo[t] = kTile32;
+      amask[t] = 0;
+    }
     return;
   }
 
   bool narrow = true, narrow8 = true;
+  unsigned affine = 0;  // slots whose column is base + lane in every row
   int ds = 0;
   uint32_t o8[kSlotWords] = {};
 #pragma unroll
@@ -404,6 +409,8 @@ __global__ __launch_bounds__(kBlock) void k_tile_layout(
       o8[u / 4] |= ((uint32_t)(cc - base) & 0xffu) << (8 * (u % 4));
       if (has && c == i) ds = u;
     }
+    const int32_t cl = row ? (has ? c : (used ? lo : (int32_t)i)) : base + lane;
+    if (wave_min_i32(cl - base == lane ? 1 : 0) == 1) affine |= 1u << u;
     if (lane == 0) {
       cbase[(int64_t)u * ntiles + t] = base;
       tbase[t * kBaseStride + u] = base;
@@ -414,7 +421,15 @@ __global__ __launch_bounds__(kBlock) void k_tile_layout(
 #pragma unroll
     for (int g = 0; g < kSlotWords; ++g) off8[(int64_t)g * n + i] = o8[g];
   }
-  if (lane == 0) tinfo[t] = !narrow ? kTile32 : narrow8 ? kTile8 : kTile16;
+  if (lane == 0) {
+    tinfo[t] = !narrow ? kTile32 : narrow8 ? kTile8 : kTile16;
+    uint8_t m = 0;  // offset word g: slots 4g .. 4g+3 (below kSlots) all affine
+    for (int g = 0; g < kSlotWords; ++g) {
+      const unsigned want = ((1u << 4) - 1u) << (4 * g) & ((1u << kSlots) - 1u);
+      if ((affine & want) == want) m |= (uint8_t)(1u << g);
+    }
+    amask[t] = narrow8 ? m : 0;
+  }
 }
 
 // Rebuild the ELL in device order: device row k is natural row iperm[k];
@@ -810,10 +825,12 @@ void finalize_level(Problem &P, int l, NaturalLevel &N, int flags,
     L.cbase = P.arena.alloc<int32_t>((size_t)kSlots * T);
     L.tbase = P.arena.alloc<int32_t>((size_t)kBaseStride * T);
     L.tinfo = P.arena.alloc<int32_t>(T);
+    L.amask = P.arena.alloc<uint8_t>(T);
     L.off8 = P.arena.alloc<uint32_t>((size_t)kSlotWords * n);
     const int64_t wpb = kBlock / 64;
     k_tile_layout<<<(unsigned)((T + wpb - 1) / wpb), kBlock, 0, s>>>(
-        n, T, L.iperm, L.col, L.val, L.nnz_row, L.dslot, L.col16, L.off8, L.cbase, L.tbase, L.tinfo);
+        n, T, L.iperm, L.col, L.val, L.nnz_row, L.dslot, L.col16, L.off8, L.cbase, L.tbase, L.tinfo,
+        L.amask);
     HPCGX_LAUNCHED();
     std::vector<int32_t> info(T);
     HPCGX_HIP(hipMemcpyAsync(info.data(), L.tinfo, sizeof(int32_t) * T,
