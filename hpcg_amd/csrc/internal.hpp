@@ -196,7 +196,7 @@ struct Problem {
   bool use_c16 = true;        // read narrow column ids where a level has them
   bool use_c8 = true;         // ... 8-bit ones where a tile has them
   bool use_pairs = true;      // FP32 colour steps of 8-bit tiles read FP16 value pairs
-  bool use_affine = true;     // FP32 V-cycle skips the offset words Level::amask marks ("affine8")
+  bool use_affine = true;     // skip the offset words Level::amask marks ("affine8")
   bool stream_p = false;      // p update with non-temporal accesses ("stream_p")
   int spmv_dot_blocks = 4096; // persistent grid of the fused SpMV + dot
   // The fused SpMV + dot with one workgroup per row chunk of the fine level

@@ -173,7 +173,14 @@ template <bool STREAM, class F>
 __device__ inline void narrow_row(const Col16 &C, int t, int info, int64_t n, int64_t i,
                                   F &&f) {
   const int32_t *__restrict__ base = C.base + t;
-  if (info == kTile8 && C.use8) {
+  if (info == kTile8 && C.use8 && C.amask && C.amask[t] == (1u << kSlotWords) - 1u) {
+    // every offset word of the tile is affine: slot j's column is its base
+    // plus the row's position in the tile
+    const int32_t pos = (int32_t)(i % kTile);
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j)
+      f(j, (int64_t)j * n + i, base[(int64_t)j * C.ntiles] + pos);
+  } else if (info == kTile8 && C.use8) {
 #pragma unroll
     for (int g = 0; g < kSlotWords; ++g) {
       const uint32_t w = mload<STREAM>(C.off8 + (int64_t)g * n + i);

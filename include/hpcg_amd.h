@@ -340,11 +340,12 @@ int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, i
  *                      read FP16 values two slots per word beside the 8-bit
  *                      offsets (3 bytes per slot; else the 4-byte packed
  *                      value + 16-bit column word; default 1)
- *   "affine8"          1/0: the FP32 V-cycle skips the 8-bit offset words
- *                      that setup found to equal the row's position in its
- *                      tile for every row of the tile (each of the word's
- *                      slots then reads the slot base plus that position;
- *                      default 1)
+ *   "affine8"          1/0: skip the 8-bit offset words that setup found to
+ *                      equal the row's position in its tile for every row of
+ *                      the tile (each of the word's slots then reads the
+ *                      slot base plus that position): the FP32 V-cycle per
+ *                      word, the FP64 kernels on tiles whose words all are
+ *                      (default 1)
  *   "spmv_dot_blocks"  workgroups of the persistent SpMV + dot kernel
  *                      (8..8192, rounded down to a multiple of 8; default
  *                      4096).  Changes the dot's summation order.
