@@ -112,6 +112,10 @@ void alloc_solver_state(hpcgx::Problem &P) {
   P.wave_grid = hpcgx::wave_grid_size();
   P.nchunk_partials = P.L[0].nchunks;
   P.chunk_partials = P.arena.alloc<double>(P.nchunk_partials);
+  // one partial per wavefront of each finest colour step's workgroups: at
+  // most one workgroup per colour more than n / kBlock (up to 64 colours)
+  P.nrz_partials = ((n + hpcgx::kBlock - 1) / hpcgx::kBlock + 64) * hpcgx::kRzWaves;
+  P.rz_part = P.arena.alloc<double>(P.nrz_partials);
 }
 
 // shared tail of both setup paths: schedules, reordering, links
@@ -408,6 +412,10 @@ int hpcgx_tune(hpcgx_problem *p, const char *key, double value) {
     }
     if (k == "affine8") {
       P.use_affine = value != 0.0;
+      return;
+    }
+    if (k == "rz_fused") {
+      P.rz_fused = value != 0.0;
       return;
     }
     if (k == "fp16_pairs") {

@@ -17,6 +17,7 @@ constexpr int kSlots = 27;        // ELL width: the 27-point stencil's maximum
 constexpr int kMaxLevels = 8;
 constexpr int kMaxColors = 32;    // first-fit colours of a <=27-entry row: <=27
 constexpr int kBlock = 256;       // threads per workgroup for streaming kernels
+constexpr int kRzWaves = kBlock / 64;  // r.z partials per colour-step workgroup
 // A launch holds fewer than 2^32 work-items (the dispatch packet's grid
 // size is 32 bits): kernels over all n*27 ELL entries loop over the slots
 // inside one work-item per row.
@@ -205,6 +206,11 @@ struct Problem {
   bool spmv_dot_chunked = true;
   int64_t nchunk_partials = 0;
   double *chunk_partials = nullptr;   // [nchunk_partials]
+  // r.z from partials that the finest level's last colour sweep leaves
+  // ("rz_fused"), summed in fixed order by a second launch.
+  bool rz_fused = true;
+  int64_t nrz_partials = 0;
+  double *rz_part = nullptr;          // [nrz_partials]
   bool mixed_ready = false;   // FP16/FP32 copies built
   bool half_exact = false;    // every off-diagonal value is exact in FP16
   float *r32 = nullptr, *z32 = nullptr;  // FP32 input / output of the V-cycle
@@ -332,8 +338,11 @@ void launch_gather(const int32_t *idx, const double *src, double *dst,
                    int64_t n, hipStream_t s);
 void launch_spmv(const Problem &P, const Level &L, const double *x, double *y,
                  hipStream_t s);
+// With dpart, wavefront w of logical workgroup b of the launch also stores
+// the sum of r_i * (new x_i) over its rows to dpart[b * kRzWaves + w] (16-bit
+// column id path only).
 void launch_symgs_rows(const Problem &P, const Level &L, int64_t begin, int64_t end,
-                       const double *r, double *x, hipStream_t s);
+                       const double *r, double *x, hipStream_t s, double *dpart = nullptr);
 void launch_symgs_list(const Level &L, const int32_t *rows, int64_t count,
                        const double *r, double *x, hipStream_t s);
 void launch_restriction(const Level &L, const double *rf, const double *Axf,
@@ -348,6 +357,9 @@ void launch_waxpby(int64_t n, double alpha, const double *x, double beta,
 enum DotFinish { kFinishStore = 0, kFinishBeta = 1, kFinishAlpha = 2 };
 void launch_dot(Problem &P, int64_t n, const double *x, const double *y,
                 double *out, int finish, hipStream_t s);
+// Sum of part[0..count) in fixed order, finished like launch_dot.
+void launch_partials_dot(Problem &P, const double *part, int64_t count, int finish,
+                         hipStream_t s);
 void launch_spmv_dot(Problem &P, const Level &L, const double *p, double *Ap,
                      hipStream_t s);
 // the mixed path's outer Ap = A p, p.Ap from the FP16 off-diagonal values
@@ -441,10 +453,15 @@ void cg_mixed(Problem &P, int ordering, const double *b, double *x,
               double *normr, double *normr0, double *times, hipStream_t s);
 
 // ---- solver (solver.hip) ----
+// With rz (the buffer of rz_partials), the last sweep of level l's
+// smoother also leaves the r.z partials of its rows there.
 void symgs(Problem &P, int l, int ordering, const double *r, double *x,
-           hipStream_t s);
+           hipStream_t s, double *rz = nullptr);
 void mg(Problem &P, int l, int ordering, const double *r, double *x,
-        hipStream_t s);
+        hipStream_t s, double *rz = nullptr);
+// Number of r.z partials the finest level's colour steps leave, or 0 when
+// the multicolour V-cycle in this ordering cannot produce them.
+int64_t rz_partials(const Problem &P, int ordering);
 void check_status(Problem &P);
 void cg(Problem &P, int ordering, const double *b, double *x, int max_iter,
         double tol, bool precond, int flags, int *niters, double *normr,

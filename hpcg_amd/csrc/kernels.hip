@@ -99,6 +99,24 @@ __device__ inline int64_t xcd_block(int64_t b, int64_t G, bool on) {
   return (b % 8) * (G8 / 8) + b / 8;
 }
 
+// Sum over the workgroup in a fixed tree: 64-lane shuffles, then the four
+// wavefront sums in order.  Result valid in thread 0.
+__device__ inline double block_sum(double v) {
+  __shared__ double wsum[kBlock / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) wsum[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) t += wsum[w];
+  }
+  __syncthreads();
+  return t;
+}
+
 // y_i = sum_j a_ij x_col(i,j)  (ComputeSPMV_ref.cpp:62-69)
 template <bool STREAM>
 __device__ inline double row_dot(const int32_t *__restrict__ col,
@@ -143,10 +161,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
 // diagonal would make the reference divide by zero as well), so no separate
 // diagonal array is read.
 template <bool STREAM>
-__device__ inline void gs_row(int64_t i, int64_t n,
-                              const int32_t *__restrict__ col,
-                              const double *__restrict__ val,
-                              const double *__restrict__ r, double *x) {
+__device__ inline double gs_row(int64_t i, int64_t n,
+                                const int32_t *__restrict__ col,
+                                const double *__restrict__ val,
+                                const double *__restrict__ r, double *x) {
   double sum = r[i];
   double d = 0.0;
 #pragma unroll
@@ -159,6 +177,7 @@ __device__ inline void gs_row(int64_t i, int64_t n,
   }
   sum += x[i] * d;
   x[i] = sum / d;
+  return sum / d;
 }
 
 // Narrow column ids (Level::col16 / off8).  The caller's wavefront lies in
@@ -217,16 +236,13 @@ __device__ inline double row_dot16(const Col16 &C, const int32_t *__restrict__ c
 }
 
 template <bool STREAM>
-__device__ inline void gs_row16(const Col16 &C, int64_t i, int64_t n,
-                                const int32_t *__restrict__ col,
-                                const double *__restrict__ val,
-                                const double *__restrict__ r, double *x) {
+__device__ inline double gs_row16(const Col16 &C, int64_t i, int64_t n,
+                                  const int32_t *__restrict__ col,
+                                  const double *__restrict__ val,
+                                  const double *__restrict__ r, double *x) {
   const int t = tile_of(i);
   const int info = C.info[t];
-  if (info == kTile32) {
-    gs_row<STREAM>(i, n, col, val, r, x);
-    return;
-  }
+  if (info == kTile32) return gs_row<STREAM>(i, n, col, val, r, x);
   double sum = r[i];
   double d = 0.0;
   narrow_row<STREAM>(C, t, info, n, i, [&](int, int64_t k, int32_t c) {
@@ -236,6 +252,7 @@ __device__ inline void gs_row16(const Col16 &C, int64_t i, int64_t n,
   });
   sum += x[i] * d;
   x[i] = sum / d;
+  return sum / d;
 }
 
 // All rows of one colour (a contiguous device-order range) at once: rows of
@@ -259,6 +276,27 @@ __global__ __launch_bounds__(kBlock) void k_gs_range16(
     const double *__restrict__ val, const double *__restrict__ r, double *x, bool xcd) {
   const int64_t i = begin + xcd_block(blockIdx.x, gridDim.x, xcd) * kBlock + threadIdx.x;
   if (i < end) gs_row16<STREAM>(C, i, n, col, val, r, x);
+}
+
+// k_gs_range16 that also stores the sum of r_i * (new x_i) over the rows of
+// wavefront w of logical workgroup b to dpart[b * kRzWaves + w].  In the last
+// sweep of a V-cycle every row is updated once, so these partials hold r.z
+// (CG.cpp:110) without reading r and z again.  One partial per wavefront: a
+// 64-lane shuffle tree, with no LDS and no workgroup barrier (a per-workgroup
+// sum cost 40 us per launch at 512x512x640).  Capped at 80 VGPRs (6 waves per
+// SIMD, as k_gs_range16 runs): uncapped the compiler takes 120.
+template <bool STREAM>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_gs_rz16(
+    int64_t begin, int64_t end, int64_t n, Col16 C, const int32_t *__restrict__ col,
+    const double *__restrict__ val, const double *__restrict__ r, double *x, bool xcd,
+    double *__restrict__ dpart) {
+  const int64_t b = xcd_block(blockIdx.x, gridDim.x, xcd);
+  const int64_t i = begin + b * kBlock + threadIdx.x;
+  double s = 0.0;
+  if (i < end) s = r[i] * gs_row16<STREAM>(C, i, n, col, val, r, x);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) dpart[b * kRzWaves + (threadIdx.x >> 6)] = s;
 }
 
 // Two adjacent rows per thread: the matrix is read with 16-byte (values) and
@@ -483,24 +521,6 @@ __global__ __launch_bounds__(kBlock) void k_replace_diag(
 // ---------------------------------------------------------------------------
 // Deterministic grid reductions
 // ---------------------------------------------------------------------------
-// Sum over the workgroup in a fixed tree: 64-lane shuffles, then the four
-// wavefront sums in order.  Result valid in thread 0.
-__device__ inline double block_sum(double v) {
-  __shared__ double wsum[kBlock / 64];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) wsum[wave] = v;
-  __syncthreads();
-  double t = 0.0;
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int w = 0; w < kBlock / 64; ++w) t += wsum[w];
-  }
-  __syncthreads();
-  return t;
-}
-
 // Each workgroup stores its partial; the last one to finish (ticket) sums the
 // partials in index order and applies the finishing step.  The ticket is
 // reset by that workgroup, so the next launch starts from zero.
@@ -537,21 +557,21 @@ __device__ inline void grid_finish(double s, double *partials, unsigned *ticket,
   }
 }
 
-// p.Ap from the per-chunk partials of k_spmv_dot_c / k_spmv_dot_hc, in a
-// launch of its own (the launch boundary orders the partials' stores before
+// p.Ap from the per-chunk partials of k_spmv_dot_c / k_spmv_dot_hc (or r.z
+// from those of k_gs_rz16), in a launch of its own (the launch boundary orders the partials' stores before
 // these loads): thread t of workgroup w sums chunks w*per + t, + kBlock, ...
 // in order, then block_sum and grid_finish, so the order is fixed.
 __global__ __launch_bounds__(kBlock) void k_chunk_dot(int64_t nchunks,
                                                       const double *__restrict__ cpart,
                                                       double *partials, unsigned *ticket,
-                                                      double *scalars) {
+                                                      double *scalars, int finish) {
   const int64_t per = (nchunks + gridDim.x - 1) / gridDim.x;
   const int64_t lo = (int64_t)blockIdx.x * per;
   const int64_t hi = lo + per < nchunks ? lo + per : nchunks;
   double s = 0.0;
   for (int64_t q = lo + threadIdx.x; q < hi; q += kBlock) s += cpart[q];
   s = block_sum(s);
-  grid_finish(s, partials, ticket, scalars, nullptr, kFinishAlpha);
+  grid_finish(s, partials, ticket, scalars, nullptr, finish);
 }
 
 constexpr int64_t kStreamRows = (int64_t)1 << 24;
@@ -870,12 +890,20 @@ void launch_spmv(const Problem &P, const Level &L, const double *x, double *y,
 }
 
 void launch_symgs_rows(const Problem &P, const Level &L, int64_t begin, int64_t end,
-                       const double *r, double *x, hipStream_t s) {
+                       const double *r, double *x, hipStream_t s, double *dpart) {
   if (end <= begin) return;
+  if (dpart && !use_col16(P, L, begin))
+    throw Error(HPCGX_EINVAL, "colour step with r.z partials needs 16-bit column ids");
   if (use_col16(P, L, begin)) {
     const Col16 C = col16_of(P, L);
     const unsigned g = grid_for(end - begin);
-    if (L.finest)
+    if (dpart && (L.finest || L.stream))
+      k_gs_rz16<true><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col, L.val, r, x, L.xcd,
+                                           dpart);
+    else if (dpart)
+      k_gs_rz16<false><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col, L.val, r, x, L.xcd,
+                                            dpart);
+    else if (L.finest)
       k_gs_range16<true, true><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col, L.val, r, x,
                                                     L.xcd);
     else if (L.stream)
@@ -989,13 +1017,20 @@ void launch_dot(Problem &P, int64_t n, const double *x, const double *y,
   HPCGX_LAUNCHED();
 }
 
+void launch_partials_dot(Problem &P, const double *part, int64_t count, int finish,
+                         hipStream_t s) {
+  k_chunk_dot<<<reduce_grid(count), kBlock, 0, s>>>(count, part, P.partials, P.ticket,
+                                                    P.scalars, finish);
+  HPCGX_LAUNCHED();
+}
+
 namespace {
 bool chunked(const Problem &P, const Level &L) {
   return P.spmv_dot_chunked && P.chunk_partials && L.nchunks == P.nchunk_partials;
 }
 void chunk_dot(Problem &P, const Level &L, hipStream_t s) {
   k_chunk_dot<<<reduce_grid(L.nchunks), kBlock, 0, s>>>(L.nchunks, P.chunk_partials, P.partials,
-                                                        P.ticket, P.scalars);
+                                                        P.ticket, P.scalars, kFinishAlpha);
   HPCGX_LAUNCHED();
 }
 }  // namespace

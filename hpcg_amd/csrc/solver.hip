@@ -43,10 +43,16 @@ struct LaunchLog {
 
 // One colour step (one launch per colour).
 void colour_step(Problem &P, const Level &L, int c, const double *r, double *x,
-                 hipStream_t s) {
+                 hipStream_t s, double *dpart = nullptr) {
   LaunchLog log(P, L, s);
-  launch_symgs_rows(P, L, L.color_ptr[c], L.color_ptr[c + 1], r, x, s);
+  launch_symgs_rows(P, L, L.color_ptr[c], L.color_ptr[c + 1], r, x, s, dpart);
   log.done();
+}
+
+// r.z partials of a colour step over `rows` rows: one per wavefront of its
+// workgroups
+int64_t rz_slots(int64_t rows) {
+  return rows > 0 ? (rows + kBlock - 1) / kBlock * kRzWaves : 0;
 }
 
 // A whole sweep direction as one wavefront launch.
@@ -58,15 +64,25 @@ void wave_sweep(Problem &P, const Level &L, int dir, const double *r, double *x,
 }
 }  // namespace
 
+// The backward sweep visits every row once and leaves its final x, so with
+// rz each of its colour steps adds its r.z partials: colour c's at rz + the
+// partial counts of the colours visited before it.
 void symgs(Problem &P, int l, int ordering, const double *r, double *x,
-           hipStream_t s) {
+           hipStream_t s, double *rz) {
   const Level &L = P.L[l];
-  if (ordering == HPCGX_ORDER_MULTICOLOR && L.wave && P.use_wave) {
+  const bool wave = ordering == HPCGX_ORDER_MULTICOLOR && L.wave && P.use_wave;
+  if (rz && (ordering != HPCGX_ORDER_MULTICOLOR || wave))
+    throw Error(HPCGX_EINVAL, "r.z partials need the multicolour colour steps");
+  if (wave) {
     wave_sweep(P, L, 0, r, x, s);
     wave_sweep(P, L, 1, r, x, s);
   } else if (ordering == HPCGX_ORDER_MULTICOLOR) {
     for (int c = 0; c < L.ncolors; ++c) colour_step(P, L, c, r, x, s);
-    for (int c = L.ncolors - 1; c >= 0; --c) colour_step(P, L, c, r, x, s);
+    int64_t off = 0;
+    for (int c = L.ncolors - 1; c >= 0; --c) {
+      colour_step(P, L, c, r, x, s, rz ? rz + off : nullptr);
+      off += rz_slots(L.color_ptr[c + 1] - L.color_ptr[c]);
+    }
   } else {
     for (int t = 0; t < L.nsched; ++t)
       launch_symgs_list(L, L.level_rows + L.level_ptr[t],
@@ -80,7 +96,7 @@ void symgs(Problem &P, int l, int ordering, const double *r, double *x,
 // ComputeMG_ref.cpp:39-63 with the fine-level SpMV and the restriction fused
 // (only the injected rows of A*x are computed).
 void mg(Problem &P, int l, int ordering, const double *r, double *x,
-        hipStream_t s) {
+        hipStream_t s, double *rz) {
   Level &L = P.L[l];
   launch_fill(x, L.n, 0.0, s);                          // :42
   if (l + 1 < P.nlevels) {
@@ -88,10 +104,25 @@ void mg(Problem &P, int l, int ordering, const double *r, double *x,
     launch_residual_restriction(P, L, r, x, L.rc, s);   // :49-51
     mg(P, l + 1, ordering, L.rc, L.xc, s);              // :52
     launch_prolongation(L, L.xc, x, s);                 // :53
-    symgs(P, l, ordering, r, x, s);                     // :55
+    symgs(P, l, ordering, r, x, s, rz);                 // :55
   } else {
-    symgs(P, l, ordering, r, x, s);                     // :59
+    symgs(P, l, ordering, r, x, s, rz);                 // :59
   }
+}
+
+int64_t rz_partials(const Problem &P, int ordering) {
+  const Level &L = P.L[0];
+  if (!P.rz_fused || !P.rz_part || ordering != HPCGX_ORDER_MULTICOLOR ||
+      (L.wave && P.use_wave))
+    return 0;
+  int64_t count = 0;
+  for (int c = 0; c < L.ncolors; ++c) {
+    const int64_t b = L.color_ptr[c], e = L.color_ptr[c + 1];
+    if (e <= b) continue;
+    if (!use_col16(P, L, b)) return 0;
+    count += rz_slots(e - b);
+  }
+  return count <= P.nrz_partials ? count : 0;
 }
 
 namespace {
@@ -145,11 +176,15 @@ void check_status(Problem &P) {
 void cg_iteration(Problem &P, int ordering, bool precond, double *x, hipStream_t s) {
   const Level &A = P.L[0];
   const int64_t n = A.n;
+  const int64_t nrz = precond ? rz_partials(P, ordering) : 0;
   if (precond)
-    mg(P, 0, ordering, P.r, P.z, s);
+    mg(P, 0, ordering, P.r, P.z, s, nrz ? P.rz_part : nullptr);
   else
     launch_copy(P.r, P.z, n, s);
-  launch_dot(P, n, P.r, P.z, nullptr, kFinishBeta, s);
+  if (nrz)
+    launch_partials_dot(P, P.rz_part, nrz, kFinishBeta, s);
+  else
+    launch_dot(P, n, P.r, P.z, nullptr, kFinishBeta, s);
   launch_update_p(n, P.z, P.p, P.scalars, s, P.stream_p);
   launch_spmv_dot(P, A, P.p, P.Ap, s);
   launch_update_xr(P, n, x, P.r, P.p, P.Ap, s);
@@ -183,7 +218,9 @@ hipGraphExec_t iteration_graph(Problem &P, int ordering, bool precond, double *x
 }  // namespace
 
 // CG.cpp:59-143.  With the reductions fused into their producers:
-//   r.z            -> dot kernel that also forms beta = rtz/oldrtz (:112-114)
+//   r.z            -> partials from the V-cycle's last colour sweep, summed by
+//                     a launch that also forms beta = rtz/oldrtz (:112-114);
+//                     a dot kernel where no such partials are made
 //   Ap = A p, p.Ap -> one SpMV kernel that also forms alpha (:118-120)
 //   x, r updates, r.r -> one kernel (:121-124)
 // With HPCGX_CG_GRAPH (and no timers or wavefront sweeps) iterations 2..
@@ -212,6 +249,7 @@ void cg(Problem &P, int ordering, const double *b, double *x, int max_iter,
   bool wave = false;
   for (int l = 0; l < P.nlevels; ++l) wave = wave || (P.L[l].wave && P.use_wave);
   const bool graphed = (flags & HPCGX_CG_GRAPH) && !T.on && !P.klog.on && !wave;
+  const int64_t nrz = precond ? rz_partials(P, ordering) : 0;
   int k = 1;
   for (; k <= max_iter && nr / *normr0 > tol; ++k) {     // :100
     if (graphed && k >= 2) {
@@ -222,12 +260,15 @@ void cg(Problem &P, int ordering, const double *b, double *x, int max_iter,
     }
     T.tick();
     if (precond)
-      mg(P, 0, ordering, P.r, P.z, s);                   // :103
+      mg(P, 0, ordering, P.r, P.z, s, nrz ? P.rz_part : nullptr);  // :103
     else
       launch_copy(P.r, P.z, n, s);                       // :105
     T.tock(t5);
     T.tick();
-    launch_dot(P, n, P.r, P.z, nullptr, kFinishBeta, s); // :110 / :113-114
+    if (nrz)                                             // :110 / :113-114
+      launch_partials_dot(P, P.rz_part, nrz, kFinishBeta, s);
+    else
+      launch_dot(P, n, P.r, P.z, nullptr, kFinishBeta, s);
     T.tock(t1);
     T.tick();
     if (k == 1)

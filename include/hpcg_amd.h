@@ -346,6 +346,13 @@ int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, i
  *                      slot base plus that position): the FP32 V-cycle per
  *                      word, the FP64 kernels on tiles whose words all are
  *                      (default 1)
+ *   "rz_fused"         1/0: the FP64 CG takes r.z from partials that the
+ *                      V-cycle's last finest-level colour sweep leaves
+ *                      (each row's r and final z are at hand there), summed
+ *                      by a second launch, instead of a dot kernel that
+ *                      reads r and z again (default 1; applies to the
+ *                      multicolour colour steps with narrow column ids).
+ *                      Changes the dot's summation order.
  *   "spmv_dot_blocks"  workgroups of the persistent SpMV + dot kernel
  *                      (8..8192, rounded down to a multiple of 8; default
  *                      4096).  Changes the dot's summation order.

@@ -526,6 +526,39 @@ def test_spmv_dot_chunks(hp, torch):
     del P
 
 
+@pytest.mark.parametrize("dims", [(32, 48, 64), (24, 40, 56)])
+def test_rz_fused(hp, torch, dims):
+    """r.z from the partials of the V-cycle's last colour sweep (24x40x56:
+    colour blocks of 6720 rows, so each colour's last workgroup is partial):
+    the CG trajectory follows the oracle's, the same bits on every call and
+    from the captured graph, and agrees with the separate dot kernel to the
+    dot's reassociation.  The lexicographic ordering keeps the dot kernel."""
+    say("rz fused", dims)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    h = hier(dims, color=True)
+    want = h.cg(max_iter=20, tol=0.0)["traj"]
+    b, x, _ = P.vectors()
+    P.tune("rz_fused", 0)
+    ref = P.cg(b, x, max_iter=20, tolerance=0.0, ordering="multicolor")
+    P.tune("rz_fused", 1)
+    runs = []
+    for graph in (False, False, True, True):
+        x.zero_()
+        r = P.cg(b, x, max_iter=20, tolerance=0.0, ordering="multicolor", graph=graph)
+        assert r["normr"] / r["normr0"] == pytest.approx(want[20], rel=1e-9), graph
+        runs.append((r["normr"], x.clone()))
+    for nr, xv in runs[1:]:
+        assert nr == runs[0][0] and torch.equal(xv, runs[0][1])
+    assert runs[0][0] == pytest.approx(ref["normr"], rel=1e-10)
+    x.zero_()
+    lex1 = P.cg(b, x, max_iter=5, tolerance=0.0, ordering="lexicographic")
+    P.tune("rz_fused", 0)
+    x.zero_()
+    lex0 = P.cg(b, x, max_iter=5, tolerance=0.0, ordering="lexicographic")
+    assert lex1["normr"] == lex0["normr"]
+    del P
+
+
 def test_cg_graph_with_tolerance(hp, torch):
     """With a tolerance the captured iterations stop where the eager ones do,
     with the same bits."""
