@@ -269,9 +269,10 @@ __global__ __launch_bounds__(kBlock) void k_gs_range(
   if (i < end) gs_row<STREAM>(i, n, col, val, r, x);
 }
 
-// k_gs_range with 16-bit column ids (begin is a multiple of kTile).
+// k_gs_range with 16-bit column ids (begin is a multiple of kTile).  Capped
+// at 72 VGPRs, 7 waves per SIMD (the compiler's own choice, 77, allows 6).
 template <bool STREAM, bool FINEST>
-__global__ __launch_bounds__(kBlock) void k_gs_range16(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_gs_range16(
     int64_t begin, int64_t end, int64_t n, Col16 C, const int32_t *__restrict__ col,
     const double *__restrict__ val, const double *__restrict__ r, double *x, bool xcd) {
   const int64_t i = begin + xcd_block(blockIdx.x, gridDim.x, xcd) * kBlock + threadIdx.x;
