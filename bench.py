@@ -342,8 +342,10 @@ def main():
     c8 = c16 and info0["byte_tiles"] == ntiles0 and not any(
         l.strip() == "chosen col8=0" for l in (tuned or []))
     ksym = ("k_gs_wave<true,true>" if info0["wave"] else
-            "k_gs_range16<true,true>" if c16 else "k_gs_range<true,true>")
+            "k_gs_range16<true,true> + k_gs_rz16<true>" if c16 else "k_gs_range<true,true>")
     kname = ksym + (" (finest-level wavefront sweep, one direction)" if info0["wave"]
+                    else " (finest-level colour steps; k_gs_rz16 runs the 8 of the last sweep "
+                    "of each V-cycle and also leaves the r.z partials)" if c16
                     else " (finest-level colour step)")
     per_launch = K * 2 * cb["symgs_L0"] / max(ks_n, 1)
     # the same sweep in the stored format: every ELL slot (padding included)
@@ -355,12 +357,14 @@ def main():
     rating = flops["total"] / (dt + setup_s / 10.0) * 1e-9  # ReportResults.cpp:353, one set
     # HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE
     # (profiles/), when this run is the profiled workload and kernel
-    traffic = None
+    traffic = traffic_note = None
     try:
         pm = json.load(open(os.path.join(ROOT, "profiles", "pmc_traffic.json")))
+        k0 = ksym.split(" + ")[0]
         if (pm["workload"] == "%dx%dx%d fp64 %s" % (a.nx, a.ny, a.nz, a.ordering)
-                and ksym in pm["kernels"]):
-            traffic = pm["kernels"][ksym]["traffic_bytes"]
+                and k0 in pm["kernels"]):
+            traffic = pm["kernels"][k0]["traffic_bytes"]
+            traffic_note = pm["kernels"][k0].get("note")
     except (OSError, ValueError, KeyError):
         traffic = None
 
@@ -395,6 +399,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
             "traffic": round(traffic) if traffic else None,
+            **({"traffic_note": traffic_note} if traffic and traffic_note else {}),
             "bytes_per_launch": per_launch,
             "stored_format_bytes_per_launch": stored,
             "column_ids": "8-bit" if c8 else "16-bit" if c16 else "32-bit",
