@@ -284,8 +284,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) voi
 // sweep of a V-cycle every row is updated once, so these partials hold r.z
 // (CG.cpp:110) without reading r and z again.  One partial per wavefront: a
 // 64-lane shuffle tree, with no LDS and no workgroup barrier (a per-workgroup
-// sum cost 40 us per launch at 512x512x640).  Capped at 80 VGPRs (6 waves per
-// SIMD, as k_gs_range16 runs): uncapped the compiler takes 120.
+// sum cost 40 us per launch at 512x512x640).  Capped at 80 VGPRs: uncapped the
+// compiler takes 120 (4 waves per SIMD); under the cap it fits 69, 7 waves, as
+// k_gs_range16 runs.
 template <bool STREAM>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_gs_rz16(
     int64_t begin, int64_t end, int64_t n, Col16 C, const int32_t *__restrict__ col,
