@@ -78,3 +78,12 @@ def test_canonical_bytes_closed_forms():
     assert b["iteration"] == pytest.approx(3.500e10, rel=1e-3)
     b = model.canonical_bytes(model.grid_levels(512, 512, 640))
     assert b["iteration"] == pytest.approx(3.514e11, rel=1e-3)
+
+
+def test_tune_keys_match_header():
+    """Every key hpcgx_tune accepts is documented in the header's key list,
+    and every documented key is accepted (the key table is the source)."""
+    doc = set(re.findall(r'^\s*\*\s+"([a-z0-9_]+)"', open(HEADER).read(), re.M))
+    src = open(os.path.join(ROOT, "hpcg_amd", "csrc", "capi.hip")).read()
+    accepted = set(re.findall(r'k == "([a-z0-9_]+)"', src))
+    assert accepted and doc == accepted, sorted(doc ^ accepted)
