@@ -42,10 +42,12 @@ FLAGS = [
 ]
 
 
-# host C++: plain g++ against the HIP runtime headers
+# host C++: plain g++ against the HIP runtime headers.  HPCG_CONTIGUOUS_ARRAYS:
+# the driver's generator keeps each matrix array in one block, and
+# DeleteMatrix frees it that way.
 HOST_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-Wall",
-    "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include"), "-I" + os.path.join(ROOT, "include"),
+    "-D__HIP_PLATFORM_AMD__", "-DHPCG_CONTIGUOUS_ARRAYS", "-I" + os.path.join(ROCM, "include"), "-I" + os.path.join(ROOT, "include"),
 ]
 
 

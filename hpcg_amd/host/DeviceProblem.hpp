@@ -15,7 +15,7 @@
 
 #include <cstdint>
 
-#include "../../include/hpcg_amd.h"
+#include "hpcg_amd.h"  // <repo>/include, on the compiler's -I path
 #include "SparseMatrix.hpp"
 #include "Vector.hpp"
 

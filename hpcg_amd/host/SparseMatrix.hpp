@@ -84,14 +84,25 @@ inline void ReplaceMatrixDiagonal(SparseMatrix &A, Vector &diagonal) {
   MatrixReplaceDiagonalOnDevice(A, diagonal);
 }
 
+// The row arrays are freed the way the generator allocated them
+// (GenerateProblem_ref.cpp:103-117 of the reference): one array per row by
+// default, one block per array under HPCG_CONTIGUOUS_ARRAYS (the xhpcg_amd
+// driver's generator is built that way).
 inline void DeleteMatrix(SparseMatrix &A) {
   MatrixReleaseDevice(A);
-  if (A.matrixValues && A.localNumberOfRows > 0) {
-    // rows live in one contiguous block per array (GenerateProblem.cpp)
-    delete[] A.matrixValues[0];
-    delete[] A.mtxIndG[0];
-    delete[] A.mtxIndL[0];
+#ifndef HPCG_CONTIGUOUS_ARRAYS
+  for (local_int_t i = 0; i < A.localNumberOfRows; ++i) {
+    if (A.matrixValues) delete[] A.matrixValues[i];
+    if (A.mtxIndG) delete[] A.mtxIndG[i];
+    if (A.mtxIndL) delete[] A.mtxIndL[i];
   }
+#else
+  if (A.localNumberOfRows > 0) {
+    if (A.matrixValues) delete[] A.matrixValues[0];
+    if (A.mtxIndG) delete[] A.mtxIndG[0];
+    if (A.mtxIndL) delete[] A.mtxIndL[0];
+  }
+#endif
   delete[] A.title;
   delete[] A.nonzerosInRow;
   delete[] A.mtxIndG;

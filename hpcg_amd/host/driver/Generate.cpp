@@ -57,6 +57,12 @@ T *prefer_huge_pages(T *p, size_t count) {
 }
 }  // namespace
 
+// Rows live in one block per array, which DeleteMatrix frees as such only
+// when built with HPCG_CONTIGUOUS_ARRAYS (SparseMatrix.hpp).
+#ifndef HPCG_CONTIGUOUS_ARRAYS
+#error "the driver's generator allocates contiguous row arrays: build with -DHPCG_CONTIGUOUS_ARRAYS"
+#endif
+
 void GenerateProblem(SparseMatrix &A, Vector *b, Vector *x, Vector *xexact) {
   const Geometry &g = *A.geom;
   const local_int_t nx = g.nx, ny = g.ny, nz = g.nz;
