@@ -15,7 +15,8 @@ KEEP_SCHEDULE = 1
 CG_TIMERS = 1
 CG_GRAPH = 2
 
-ERRORS = {1: "EINVAL", 2: "EHIP", 3: "ENOMEM", 4: "EUNSUP"}
+ERRORS = {1: "EINVAL", 2: "EHIP", 3: "ENOMEM", 4: "EUNSUP", 5: "ECHECK"}
+ECHECK = 5
 
 
 class HpcgxError(RuntimeError):
@@ -52,6 +53,7 @@ SIGNATURES = {
     "hpcgx_abi_version": (I, []),
     "hpcgx_optimize_problem": (I, [ctypes.POINTER(HostLevel), I, I, PP]),
     "hpcgx_generate_problem": (I, [I, I, I, I, I, PP]),
+    "hpcgx_check_problem": (I, [P, P, P, P, P]),
     "hpcgx_problem_destroy": (I, [P]),
     "hpcgx_problem_nlevels": (I, [P]),
     "hpcgx_problem_info": (I, [P, I, ctypes.POINTER(LevelInfo)]),

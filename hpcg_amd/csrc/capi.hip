@@ -177,6 +177,8 @@ int hpcgx_generate_problem(int nx, int ny, int nz, int nlevels, int flags,
     }
     for (int l = 0; l + 1 < nlevels; ++l) hpcgx::link_levels(P, l, nat[l].N, s);
     HPCGX_HIP(hipStreamSynchronize(s));
+    nat.clear();
+    hpcgx::check_problem(P, nullptr, nullptr, nullptr, s);  // main.cpp:150-159
     alloc_solver_state(P);
     *out = h.release();
   });
@@ -692,10 +694,20 @@ int hpcgx_generate_fleet(int count, const int *nx, const int *ny, const int *nz,
     }
     for (int l = 0; l + 1 < nlevels; ++l) hpcgx::link_levels(P, l, nat[l].N, s);
     HPCGX_HIP(hipStreamSynchronize(s));
+    nat.clear();
     alloc_solver_state(P);
     hpcgx::prepare_fleet(P, dims, s);
+    hpcgx::check_problem(P, nullptr, nullptr, nullptr, s);  // main.cpp:150-159, per problem
     HPCGX_HIP(hipStreamSynchronize(s));
     *out = h.release();
+  });
+}
+
+int hpcgx_check_problem(const hpcgx_problem *p, const double *b, const double *x,
+                        const double *xexact, void *stream) {
+  return guard([&] {
+    if (!p) throw Error(HPCGX_EINVAL, "NULL problem");
+    hpcgx::check_problem(p->P, b, x, xexact, static_cast<hipStream_t>(stream));
   });
 }
 

@@ -413,6 +413,14 @@ void build_chunks(Problem &P, Level &L, bool interleave);
 void drop_wave(Problem &P, Level &L);
 void problem_vectors(const Problem &P, double *b, double *x, double *xexact,
                      hipStream_t s);
+// The grids level l's rows come from (empty when unknown).
+std::vector<GridPart> grid_parts(const Problem &P, int l);
+// CheckProblem.cpp:79-148 on the device: every level's matrix and f2c, and
+// any of b, x, xexact given (fine level, device order), against the
+// generated 27-point problem of the level's grids; throws HPCGX_ECHECK on a
+// difference, HPCGX_EINVAL when a level's grid is unknown.  Synchronous.
+void check_problem(const Problem &P, const double *b, const double *x, const double *xexact,
+                   hipStream_t s);
 // ---- host levels (hostload.hip) ----
 // N's col / val / nnz_row (allocated for H.nrow rows) from the reference's
 // row-pointer storage; throws on a row with 0 or more than 27 entries

@@ -966,6 +966,234 @@ void link_levels(Problem &P, int l, const NaturalLevel &Nf, hipStream_t s) {
   }
 }
 
+namespace {
+// CheckProblem.cpp:79-148 on the device, for a level in device order whose
+// rows come from known grids.  Device row i is natural row g = iperm[i] of
+// part p; walking its stencil neighbours in the generator's order (sz, sy,
+// sx ascending, so ascending columns), the k-th neighbour must be the
+// row's k-th stored entry (slots in order, skipping the exact zeros of the
+// aligned ELL's padding): natural column iperm[col] == the neighbour, value
+// 26 on the diagonal and -1 elsewhere; nothing stored after the last one;
+// nonzerosInRow == the neighbour count; the diagonal array holds 26.  With
+// f2c (every level but the coarsest), coarse row c of part p injects from
+// the fine point (2x, 2y, 2z) (GenerateCoarseProblem.cpp:71-82).
+// Failures set bits of *flags (1 value, 2 column, 4 count, 8 extra entry,
+// 16 diagonal, 32 f2c, 64 row outside every grid) and the smallest failing
+// natural row in *first; *nnz sums the neighbour counts.
+struct CheckOut {
+  int flags;
+  int pad;
+  unsigned long long first;
+  unsigned long long nnz;
+  unsigned long long bad_rows;
+};
+
+__device__ inline int part_of(int nparts, const GridPart *__restrict__ parts, int64_t g) {
+  int lo = 0, hi = nparts - 1;  // last part with off <= g
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (parts[mid].off <= g) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void k_check_problem(
+    int64_t n, int nparts, const GridPart *__restrict__ parts, const int32_t *__restrict__ iperm,
+    const int32_t *__restrict__ perm, const int32_t *__restrict__ col,
+    const double *__restrict__ val, const uint8_t *__restrict__ nnz_row,
+    const double *__restrict__ diag, CheckOut *out) {
+  const int64_t i = gtid();
+  int flags = 0;
+  unsigned long long cnt = 0, g = ~0ull;
+  if (i < n) {
+    g = (unsigned long long)iperm[i];
+    const GridPart p = parts[part_of(nparts, parts, (int64_t)g)];
+    const int64_t local = (int64_t)g - p.off;
+    if ((int64_t)g >= n || perm[g] != i || local < 0 ||
+        local >= (int64_t)p.nx * p.ny * p.nz) {
+      flags = 64;
+    } else {
+      const int ix = (int)(local % p.nx);
+      const int64_t t = local / p.nx;
+      const int iy = (int)(t % p.ny), iz = (int)(t / p.ny);
+      const int64_t nxy = (int64_t)p.nx * p.ny;
+      int j = 0;
+      for (int sz = -1; sz <= 1; ++sz) {
+        if (iz + sz < 0 || iz + sz >= p.nz) continue;
+        for (int sy = -1; sy <= 1; ++sy) {
+          if (iy + sy < 0 || iy + sy >= p.ny) continue;
+          for (int sx = -1; sx <= 1; ++sx) {
+            if (ix + sx < 0 || ix + sx >= p.nx) continue;
+            const int64_t want = (int64_t)g + sz * nxy + (int64_t)sy * p.nx + sx;
+            while (j < kSlots && val[(int64_t)j * n + i] == 0.0) ++j;
+            ++cnt;
+            if (j == kSlots) { flags |= 4; continue; }
+            const int64_t k = (int64_t)j * n + i;
+            const int32_t c = col[k];
+            if (c < 0 || c >= n || iperm[c] != want) flags |= 2;
+            if (val[k] != (want == (int64_t)g ? 26.0 : -1.0)) flags |= 1;
+            ++j;
+          }
+        }
+      }
+      for (; j < kSlots; ++j)
+        if (val[(int64_t)j * n + i] != 0.0) flags |= 8;
+      if (nnz_row[i] != cnt) flags |= 4;
+      if (diag[i] != 26.0) flags |= 16;
+    }
+  }
+  if (flags) {
+    atomicOr(&out->flags, flags);
+    atomicMin(&out->first, g);
+    atomicAdd(&out->bad_rows, 1ull);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_down(cnt, off, 64);
+  if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(&out->nnz, cnt);
+}
+
+// f2c of GenerateCoarseProblem.cpp:77-79 for coarse device rows: natural
+// fine row iperm_f[f2c[c]] must be the fine point (2x, 2y, 2z) of natural
+// coarse row iperm_c[c], in the same part (fine part q at foff[q]).
+__global__ __launch_bounds__(kBlock) void k_check_f2c(
+    int64_t nc, int nparts, const GridPart *__restrict__ cparts,
+    const GridPart *__restrict__ fparts, const int32_t *__restrict__ iperm_c,
+    const int32_t *__restrict__ iperm_f, const int32_t *__restrict__ f2c, int64_t nf,
+    CheckOut *out) {
+  const int64_t c = gtid();
+  if (c >= nc) return;
+  const int64_t gc = iperm_c[c];
+  const int q = part_of(nparts, cparts, gc);
+  const GridPart pc = cparts[q], pf = fparts[q];
+  const int64_t local = gc - pc.off;
+  bool ok = local >= 0 && local < (int64_t)pc.nx * pc.ny * pc.nz;
+  if (ok) {
+    const int ix = (int)(local % pc.nx);
+    const int64_t t = local / pc.nx;
+    const int iy = (int)(t % pc.ny), iz = (int)(t / pc.ny);
+    const int64_t want = pf.off + ((int64_t)(2 * iz) * pf.ny + 2 * iy) * pf.nx + 2 * ix;
+    const int32_t f = f2c[c];
+    ok = f >= 0 && f < nf && iperm_f[f] == want;
+  }
+  if (!ok) {
+    atomicOr(&out->flags, 32);
+    atomicMin(&out->first, (unsigned long long)gc);
+    atomicAdd(&out->bad_rows, 1ull);
+  }
+}
+
+// b, x, xexact of GenerateProblem_ref.cpp:178-180 in device order on the
+// fine level: b_i = 26 - (nnz_i - 1) with nnz_i the stencil's neighbour
+// count, x_i = 0, xexact_i = 1 (CheckProblem.cpp:139-141).
+__device__ inline int axis_count(int v, int n) {
+  return 1 + (v > 0) + (v + 1 < n);
+}
+
+__global__ __launch_bounds__(kBlock) void k_check_vectors(
+    int64_t n, int nparts, const GridPart *__restrict__ parts, const int32_t *__restrict__ iperm,
+    const double *b, const double *x, const double *xexact, CheckOut *out) {
+  const int64_t i = gtid();
+  if (i >= n) return;
+  const int64_t g = iperm[i];
+  const GridPart p = parts[part_of(nparts, parts, g)];
+  const int64_t local = g - p.off;
+  const int ix = (int)(local % p.nx);
+  const int64_t t = local / p.nx;
+  const int iy = (int)(t % p.ny), iz = (int)(t / p.ny);
+  const int m = axis_count(ix, p.nx) * axis_count(iy, p.ny) * axis_count(iz, p.nz);
+  int flags = 0;
+  if (b && b[i] != 26.0 - (double)(m - 1)) flags |= 128;
+  if (x && x[i] != 0.0) flags |= 256;
+  if (xexact && xexact[i] != 1.0) flags |= 512;
+  if (flags) {
+    atomicOr(&out->flags, flags);
+    atomicMin(&out->first, (unsigned long long)g);
+    atomicAdd(&out->bad_rows, 1ull);
+  }
+}
+
+std::string check_message(int l, const CheckOut &o) {
+  static const char *what[] = {"a value differs",       "a column differs",
+                               "an entry count differs", "an entry beyond the stencil",
+                               "a diagonal differs",    "f2c differs",
+                               "a row outside the grids", "b differs",
+                               "x is not 0",            "xexact is not 1"};
+  std::string m = "CheckProblem: level " + std::to_string(l) + ": " +
+                  std::to_string(o.bad_rows) + " rows differ from the generated problem (first: "
+                  "natural row " + std::to_string(o.first) + ";";
+  for (int b = 0; b < 10; ++b)
+    if (o.flags & (1 << b)) m += std::string(" ") + what[b] + ";";
+  m.back() = ')';
+  return m;
+}
+}  // namespace
+
+// The grids level l's rows come from: one per problem (fleets: problem q at
+// its natural row offset), or none when a level's grid is unknown.
+std::vector<GridPart> grid_parts(const Problem &P, int l) {
+  std::vector<GridPart> parts;
+  const Level &L = P.L[l];
+  if (P.fleet.count > 0) {
+    for (int q = 0; q < P.fleet.count; ++q)
+      parts.push_back(GridPart{P.fleet.off[l][q], P.fleet.dims[3 * q] >> l,
+                               P.fleet.dims[3 * q + 1] >> l, P.fleet.dims[3 * q + 2] >> l});
+  } else if ((int64_t)L.nx * L.ny * L.nz == L.n && L.n > 0) {
+    parts.push_back(GridPart{0, L.nx, L.ny, L.nz});
+  }
+  return parts;
+}
+
+void check_problem(const Problem &P, const double *b, const double *x, const double *xexact,
+                   hipStream_t s) {
+  std::vector<std::vector<GridPart>> parts(P.nlevels);
+  for (int l = 0; l < P.nlevels; ++l) {
+    parts[l] = grid_parts(P, l);
+    if (parts[l].empty())
+      throw Error(HPCGX_EINVAL, "CheckProblem: the grid of level " + std::to_string(l) +
+                                    " is unknown (its rows do not come from nx*ny*nz)");
+  }
+  CheckOut init{0, 0, ~0ull, 0ull, 0ull};
+  for (int l = 0; l < P.nlevels; ++l) {
+    const Level &L = P.L[l];
+    const int np = (int)parts[l].size();
+    TempBuf dp(sizeof(GridPart) * np), dpc(sizeof(GridPart) * np), dout(sizeof(CheckOut));
+    HPCGX_HIP(hipMemcpyAsync(dp.p, parts[l].data(), sizeof(GridPart) * np, hipMemcpyHostToDevice,
+                             s));
+    HPCGX_HIP(hipMemcpyAsync(dout.p, &init, sizeof init, hipMemcpyHostToDevice, s));
+    k_check_problem<<<grid_for(L.n), kBlock, 0, s>>>(L.n, np, dp.as<GridPart>(), L.iperm, L.perm,
+                                                     L.col, L.val, L.nnz_row, L.diag,
+                                                     dout.as<CheckOut>());
+    HPCGX_LAUNCHED();
+    if (l + 1 < P.nlevels) {
+      const Level &C = P.L[l + 1];
+      HPCGX_REQUIRE(L.f2c && parts[l + 1].size() == parts[l].size(),
+                    "CheckProblem: the levels' grids do not correspond");
+      HPCGX_HIP(hipMemcpyAsync(dpc.p, parts[l + 1].data(), sizeof(GridPart) * np,
+                               hipMemcpyHostToDevice, s));
+      k_check_f2c<<<grid_for(C.n), kBlock, 0, s>>>(C.n, np, dpc.as<GridPart>(), dp.as<GridPart>(),
+                                                   C.iperm, L.iperm, L.f2c, L.n,
+                                                   dout.as<CheckOut>());
+      HPCGX_LAUNCHED();
+    }
+    if (l == 0 && (b || x || xexact)) {
+      k_check_vectors<<<grid_for(L.n), kBlock, 0, s>>>(L.n, np, dp.as<GridPart>(), L.iperm, b, x,
+                                                       xexact, dout.as<CheckOut>());
+      HPCGX_LAUNCHED();
+    }
+    CheckOut o;
+    HPCGX_HIP(hipMemcpyAsync(&o, dout.p, sizeof o, hipMemcpyDeviceToHost, s));
+    HPCGX_HIP(hipStreamSynchronize(s));
+    int64_t want_nnz = 0;  // CheckProblem.cpp:145-148: prod(3 n_a - 2) per grid
+    for (const GridPart &g : parts[l])
+      want_nnz += (int64_t)(3 * g.nx - 2) * (3 * g.ny - 2) * (3 * g.nz - 2);
+    if (o.flags) throw Error(HPCGX_ECHECK, check_message(l, o));
+    if ((int64_t)o.nnz != want_nnz || L.nnz != want_nnz)
+      throw Error(HPCGX_ECHECK, "CheckProblem: level " + std::to_string(l) + ": " +
+                                    std::to_string(L.nnz) + " nonzeros, the grids give " +
+                                    std::to_string(want_nnz));
+  }
+}
+
 void problem_vectors(const Problem &P, double *b, double *x, double *xexact,
                      hipStream_t s) {
   const Level &L = P.L[0];

@@ -144,6 +144,15 @@ class Problem:
              nlevels, _flags(keep_schedule, 0), ctypes.byref(h))
         return cls(h)
 
+    def check_problem(self, b=None, x=None, xexact=None):
+        """CheckProblem (CheckProblem.cpp:50-151): every level's matrix and
+        f2c, and b, x, xexact where given (device order), against the
+        generated 27-point problem of the level's grids.  Raises HpcgxError
+        (code 5, HPCGX_ECHECK) naming the first differing natural row."""
+        n = self.nrow(0)
+        return call("hpcgx_check_problem", self._h, _ptr(b, n, "b"), _ptr(x, n, "x"),
+                    _ptr(xexact, n, "xexact"), _stream())
+
     def fleet_offsets(self, level=0):
         """Natural row offsets (count + 1) of the fleet's problems on `level`,
         or None for a single problem."""

@@ -29,7 +29,9 @@ enum {
   HPCGX_EINVAL = 1,  /* bad argument (sizes, level, ordering, NULL) */
   HPCGX_EHIP = 2,    /* a HIP runtime call or kernel launch failed */
   HPCGX_ENOMEM = 3,  /* device allocation failed */
-  HPCGX_EUNSUP = 4   /* matrix outside what the setup supports */
+  HPCGX_EUNSUP = 4,  /* matrix outside what the setup supports */
+  HPCGX_ECHECK = 5   /* hpcgx_check_problem: the problem differs from the
+                        generated 27-point problem */
 };
 
 /* Gauss-Seidel row orderings for SYMGS / MG / CG */
@@ -116,6 +118,22 @@ int hpcgx_optimize_problem(const hpcgx_host_level *levels, int nlevels,
  * generated directly in device memory, level by level. */
 int hpcgx_generate_problem(int nx, int ny, int nz, int nlevels, int flags,
                            hpcgx_problem **out);
+
+/* replaces CheckProblem (CheckProblem.cpp:50-151 of the reference; main.cpp:
+ * 150-159 runs it on every level after generation): every level's matrix in
+ * HBM -- each row's stored entries, in order, are its 27-point stencil
+ * neighbours with value 26 on the diagonal and -1 elsewhere, nonzerosInRow
+ * and the diagonal array agree, the nonzero count is prod(3 n_a - 2) per
+ * grid -- and each level's f2c (GenerateCoarseProblem.cpp:71-82); and, where
+ * given (device pointers, fine level, device order; NULL to skip), b, x and
+ * xexact of GenerateProblem_ref.cpp:178-180.  A level's grid is its nx, ny,
+ * nz (hpcgx_generate_problem, hpcgx_host_level's nx/ny/nz) or the fleet's
+ * grids.  Returns HPCGX_ECHECK naming the level, the first differing natural
+ * row and what differs; HPCGX_EINVAL when a level's grid is unknown.
+ * hpcgx_generate_problem and hpcgx_generate_fleet run the matrix checks
+ * before returning.  Synchronous. */
+int hpcgx_check_problem(const hpcgx_problem *p, const double *b, const double *x,
+                        const double *xexact, void *stream);
 
 /* replaces DeleteMatrix / DeleteMGData for device state
  * (SparseMatrix.hpp:151-183, MGData.hpp:65-75). */
