@@ -401,11 +401,15 @@ def main():
                     else " (finest-level colour steps; k_gs_rz16 runs the 8 of the last sweep "
                     "of each V-cycle and also leaves the r.z partials)" if c16
                     else " (finest-level colour step)")
-    per_launch = K * 2 * cb["symgs_L0"] / max(ks_n, 1)
+    # one colour step is one (2 x ncolors)-th of a symmetric sweep; the
+    # logged launches are the full colour steps (the zero-x steps of each
+    # V-cycle's first forward sweep are another kernel, k_gs_zero16)
+    nsteps = 2 * info0["ncolors"] if not info0["wave"] else 2
+    per_launch = cb["symgs_L0"] / nsteps
     # the same sweep in the stored format: every ELL slot (padding included)
     # with 8-byte values and 1- (8-bit offsets), 2- (16-bit) or 4-byte ids
     a_st = 8 + (1 if c8 else 2 if c16 else 4)
-    stored = K * 2 * (2 * a_st * 27 * lv[0]["n"] + 6 * 8 * lv[0]["n"]) / max(ks_n, 1)
+    stored = (2 * a_st * 27 * lv[0]["n"] + 6 * 8 * lv[0]["n"]) / nsteps
     launch_ms = ks_ms / max(ks_n, 1)
     achieved = per_launch / (launch_ms * 1e-3) * 1e-9 if ks_n else None
     # ReportResults.cpp:99,351: the flops of ref_iters reference iterations

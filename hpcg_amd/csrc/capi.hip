@@ -18,28 +18,13 @@ struct hpcgx_problem {
 };
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
-template <class F>
-int guard(F &&f) {
-  try {
-    f();
-    return HPCGX_OK;
-  } catch (const hpcgx::Error &e) {
-    g_err = e.what();
-    return e.code;
-  } catch (const std::bad_alloc &) {
-    g_err = "host allocation failed";
-    return HPCGX_ENOMEM;
-  } catch (const std::exception &e) {
-    g_err = e.what();
-    return HPCGX_EHIP;
-  } catch (...) {
-    g_err = "unknown failure";
-    return HPCGX_EHIP;
-  }
-}
+std::string &hpcgx::last_error_slot() { return g_err; }
+
+namespace {
+using hpcgx::guard;
 
 inline hipStream_t S(void *s) { return static_cast<hipStream_t>(s); }
 
@@ -418,6 +403,14 @@ int hpcgx_tune(hpcgx_problem *p, const char *key, double value) {
     }
     if (k == "rz_fused") {
       P.rz_fused = value != 0.0;
+      return;
+    }
+    if (k == "zero_guess") {
+      P.zero_guess = value != 0.0;
+      return;
+    }
+    if (k == "defer_x") {
+      P.defer_x = value != 0.0;
       return;
     }
     if (k == "fp16_pairs") {

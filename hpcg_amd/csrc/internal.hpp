@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <new>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -49,6 +50,31 @@ struct Error : std::runtime_error {
   } while (0)
 
 #define HPCGX_LAUNCHED() HPCGX_HIP(hipGetLastError())
+
+// The calling thread's message for hpcgx_last_error (capi.hip).
+std::string &last_error_slot();
+
+// Runs f for an extern "C" entry point: exceptions never cross the
+// boundary; each maps to an HPCGX_E* code and its message is recorded.
+template <class F>
+int guard(F &&f) {
+  try {
+    f();
+    return HPCGX_OK;
+  } catch (const Error &e) {
+    last_error_slot() = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    last_error_slot() = "host allocation failed";
+    return HPCGX_ENOMEM;
+  } catch (const std::exception &e) {
+    last_error_slot() = e.what();
+    return HPCGX_EHIP;
+  } catch (...) {
+    last_error_slot() = "unknown failure";
+    return HPCGX_EHIP;
+  }
+}
 
 #define HPCGX_REQUIRE(cond, msg)                                  \
   do {                                                            \
@@ -165,6 +191,11 @@ struct Level {
   int32_t *cbase = nullptr;
   int32_t *tbase = nullptr;
   int32_t *tinfo = nullptr;
+  // Per tile, bit j set when some row of the tile stores a nonzero in slot j
+  // whose column lies in a colour visited before the row's own (a device
+  // row below the first row of the row's colour): the only slots the first
+  // forward sweep from x = 0 reads (launch_symgs_rows_zero).
+  uint32_t *zmask = nullptr;
   // transfer to the next level: f2c[coarse device row] = fine device row
   int64_t nc = 0;
   int32_t *f2c = nullptr;
@@ -199,6 +230,8 @@ struct Problem {
   bool use_pairs = true;      // FP32 colour steps of 8-bit tiles read FP16 value pairs
   bool use_affine = true;     // skip the offset words Level::amask marks ("affine8")
   bool stream_p = false;      // p update with non-temporal accesses ("stream_p")
+  bool zero_guess = true;     // V-cycle's first forward sweeps skip the zero x ("zero_guess")
+  bool defer_x = true;        // CG takes x's step with the next p update ("defer_x")
   int spmv_dot_blocks = 4096; // persistent grid of the fused SpMV + dot
   // The fused SpMV + dot with one workgroup per row chunk of the fine level
   // ("spmv_dot_chunks"): the p.Ap partial of each chunk, summed in fixed
@@ -345,6 +378,14 @@ void launch_symgs_rows(const Problem &P, const Level &L, int64_t begin, int64_t 
                        const double *r, double *x, hipStream_t s, double *dpart = nullptr);
 void launch_symgs_list(const Level &L, const int32_t *rows, int64_t count,
                        const double *r, double *x, hipStream_t s);
+// The colour step of a forward sweep whose x was zero when the sweep began
+// (ComputeMG_ref.cpp:42-47): reads only x of device rows below `begin`
+// (rows of colours already updated in this sweep) and takes 0 for every
+// other x, so x needs no zero fill; same bits as launch_symgs_rows on a
+// zero-filled x.  False (nothing launched) where it does not apply (no
+// narrow column ids at begin, no Level::zmask).
+bool launch_symgs_rows_zero(const Problem &P, const Level &L, int64_t begin, int64_t end,
+                            const double *r, double *x, hipStream_t s);
 void launch_restriction(const Level &L, const double *rf, const double *Axf,
                         double *rc, hipStream_t s);
 void launch_residual_restriction(const Problem &P, const Level &L, const double *rf,
@@ -357,6 +398,12 @@ void launch_waxpby(int64_t n, double alpha, const double *x, double beta,
 enum DotFinish { kFinishStore = 0, kFinishBeta = 1, kFinishAlpha = 2 };
 void launch_dot(Problem &P, int64_t n, const double *x, const double *y,
                 double *out, int finish, hipStream_t s);
+// launch_dot with explicit scratch: partials (dot_partials(n) doubles),
+// the completion ticket (0 on entry, 0 again on exit) and the scalars
+void launch_dot_raw(int64_t n, const double *x, const double *y, double *partials,
+                    unsigned *ticket, double *scalars, double *out, int finish, hipStream_t s);
+int64_t dot_partials(int64_t n);
+void launch_zero_u32(unsigned *a, int64_t n, hipStream_t s);
 // Sum of part[0..count) in fixed order, finished like launch_dot.
 void launch_partials_dot(Problem &P, const double *part, int64_t count, int finish,
                          hipStream_t s);
@@ -366,9 +413,16 @@ void launch_spmv_dot(Problem &P, const Level &L, const double *p, double *Ap,
 // and the FP64 diagonal (the FP64 result while those values are exact)
 void launch_spmv_dot_half(Problem &P, const Level &L, const double *p, double *Ap,
                           hipStream_t s);
+// x == nullptr: r and r.r only (x's step deferred to launch_update_xp)
 void launch_update_xr(Problem &P, int64_t n, double *x, double *r,
                       const double *p, const double *Ap, hipStream_t s,
                       float *r32 = nullptr);
+// x += alpha p (the previous iteration's step) and p = z + beta p in one pass
+void launch_update_xp(int64_t n, double *x, const double *z, double *p,
+                      const double *scalars, hipStream_t s);
+// x += alpha p alone (the last deferred step)
+void launch_step_x(int64_t n, double *x, const double *p, const double *scalars,
+                   hipStream_t s);
 // mixed precision: dot with, and p update from, the FP32 V-cycle output
 void launch_dot32(Problem &P, int64_t n, const double *x, const float *y, double *out,
                   int finish, hipStream_t s);
@@ -464,7 +518,8 @@ void cg_mixed(Problem &P, int ordering, const double *b, double *x,
 // With rz (the buffer of rz_partials), the last sweep of level l's
 // smoother also leaves the r.z partials of its rows there.
 void symgs(Problem &P, int l, int ordering, const double *r, double *x,
-           hipStream_t s, double *rz = nullptr);
+           hipStream_t s, double *rz = nullptr, bool zero_x = false);
+bool zero_guess_sweep(const Problem &P, int l, int ordering);
 void mg(Problem &P, int l, int ordering, const double *r, double *x,
         hipStream_t s, double *rz = nullptr);
 // Number of r.z partials the finest level's colour steps leave, or 0 when

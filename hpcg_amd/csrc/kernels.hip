@@ -301,6 +301,101 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
   if ((threadIdx.x & 63) == 0) dpart[b * kRzWaves + (threadIdx.x >> 6)] = s;
 }
 
+// The first forward sweep of a V-cycle level from x = 0 (ComputeMG_ref.cpp:
+// 42-47 zeroes x before the pre-smoother).  In the colour step over device
+// rows [begin, end), the only x values the reference's row update reads
+// that are not 0 are those of rows updated earlier in this sweep: device
+// rows below `begin`.  So a slot's values are read only when Level::zmask
+// marks it for the tile (some row has a nonzero there with a column below
+// its colour's first row; uniform per tile), and x only below `begin`.
+// Skipped terms are v * 0 and the diagonal term is x_i * d = 0 * d:
+// leaving them out changes no nonzero sum (only, for a zero sum, the sign
+// of the zero), so x agrees with the reference's sweep of a zero-filled x,
+// and x needs no fill: every x this sweep reads was written by it.  The
+// diagonal comes from the diagonal array (the stored entry whose column is
+// the row, as matrixDiagonal).
+//
+// Every slot still issues its loads, as the full colour step does, so the
+// loads of a row stay in flight together; the skipping is done by the
+// buffer unit's range check: a load through a buffer view at a byte offset
+// at or beyond the view's size returns 0 and reads no memory.  A dead
+// slot's views have size 0; the x view of a live slot ends at `begin`.
+typedef __amdgpu_buffer_rsrc_t BufView;
+constexpr int kBufWord3 = 0x00020000;  // buffer descriptor word 3 for gfx9 raw loads
+
+__device__ inline BufView buf_view(const void *p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, kBufWord3);
+}
+template <bool STREAM>
+__device__ inline double buf_f64(BufView v, uint32_t off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(v, off, 0, STREAM ? 2 : 0));
+}
+template <bool STREAM>
+__device__ inline uint32_t buf_u32(BufView v, uint32_t off) {
+  return __builtin_amdgcn_raw_buffer_load_b32(v, off, 0, STREAM ? 2 : 0);
+}
+template <bool STREAM>
+__device__ inline uint32_t buf_u16(BufView v, uint32_t off) {
+  return __builtin_amdgcn_raw_buffer_load_b16(v, off, 0, STREAM ? 2 : 0);
+}
+
+// Row i (lane `lane` of tile t, whose first row is tb) of a zero-x colour
+// step; zm is the tile's Level::zmask.  n * 8 < 2^32 (x offsets).
+template <bool STREAM>
+__device__ inline void gs_row_zero(const Col16 &C, uint32_t zm, int64_t begin, int64_t i,
+                                   int64_t n, const int32_t *__restrict__ col,
+                                   const double *__restrict__ val,
+                                   const double *__restrict__ diag,
+                                   const double *__restrict__ r, double *x) {
+  const int t = tile_of(i);
+  const int info = C.info[t];
+  const int64_t tb = (int64_t)t * kTile;
+  const uint32_t lane = (uint32_t)(i - tb);
+  const uint32_t xlim = (uint32_t)begin * 8u;
+  const int32_t *__restrict__ base = C.base + t;
+  const bool affine = info == kTile8 && C.use8 && C.amask && C.amask[t] == (1u << kSlotWords) - 1u;
+  double sum = r[i];
+  uint32_t w = 0;
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const bool live = (zm >> j) & 1u;
+    int32_t c;
+    if (info == kTile32) {
+      c = (int32_t)buf_u32<STREAM>(buf_view(col + (int64_t)j * n + tb, live ? 256u : 0u),
+                                   lane * 4u);
+    } else if (affine) {
+      c = base[(int64_t)j * C.ntiles] + (int32_t)lane;
+    } else if (info == kTile8 && C.use8) {
+      if (j % 4 == 0) {
+        const bool any = (zm >> j) & 0xfu;
+        w = buf_u32<STREAM>(buf_view(C.off8 + (int64_t)(j / 4) * n + tb, any ? 256u : 0u),
+                            lane * 4u);
+      }
+      c = base[(int64_t)j * C.ntiles] + (int32_t)((w >> (8 * (j % 4))) & 0xffu);
+    } else {
+      c = base[(int64_t)j * C.ntiles] +
+          (int32_t)buf_u16<STREAM>(buf_view(C.off + (int64_t)j * n + tb, live ? 128u : 0u),
+                                   lane * 2u);
+    }
+    const double v = buf_f64<STREAM>(buf_view(val + (int64_t)j * n + tb, live ? 512u : 0u),
+                                     lane * 8u);
+    const double xc = buf_f64<false>(buf_view(x, live ? xlim : 0u), (uint32_t)c * 8u);
+    sum -= v * xc;
+  }
+  const double d = mload<STREAM>(diag + i);
+  sum += 0.0 * d;
+  x[i] = sum / d;
+}
+
+template <bool STREAM, bool FINEST>
+__global__ __launch_bounds__(kBlock) void k_gs_zero16(
+    int64_t begin, int64_t end, int64_t n, Col16 C, const uint32_t *__restrict__ zmask,
+    const int32_t *__restrict__ col, const double *__restrict__ val,
+    const double *__restrict__ diag, const double *__restrict__ r, double *x, bool xcd) {
+  const int64_t i = begin + xcd_block(blockIdx.x, gridDim.x, xcd) * kBlock + threadIdx.x;
+  if (i < end) gs_row_zero<STREAM>(C, zmask[tile_of(i)], begin, i, n, col, val, diag, r, x);
+}
+
 // Two adjacent rows per thread: the matrix is read with 16-byte (values) and
 // 8-byte (column ids) loads per lane.  Needs an even level size and an even
 // range start, so that slot j of the pair is 16-byte aligned.
@@ -734,8 +829,9 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot_h(
 }
 
 // x += alpha p; r -= alpha Ap; r.r (CG.cpp:121-124).  R32: also writes r
-// rounded to FP32, the input of the mixed-precision V-cycle.
-template <bool R32, bool STREAM>
+// rounded to FP32, the input of the mixed-precision V-cycle.  WITH_X false:
+// r and r.r only (x's step is taken by the next k_update_xp).
+template <bool R32, bool STREAM, bool WITH_X>
 __global__ __launch_bounds__(kBlock) void k_update_xr(
     int64_t n, double *__restrict__ x, double *__restrict__ r,
     const double *__restrict__ p, const double *__restrict__ Ap,
@@ -750,15 +846,17 @@ __global__ __launch_bounds__(kBlock) void k_update_xr(
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int64_t k = i + u * stride;
-      xv[u] = mload<STREAM>(x + k);
-      pv[u] = mload<STREAM>(p + k);
+      if (WITH_X) {
+        xv[u] = mload<STREAM>(x + k);
+        pv[u] = mload<STREAM>(p + k);
+      }
       rv[u] = mload<STREAM>(r + k);
       av[u] = mload<STREAM>(Ap + k);
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int64_t k = i + u * stride;
-      mstore<STREAM>(x + k, xv[u] + alpha * pv[u]);
+      if (WITH_X) mstore<STREAM>(x + k, xv[u] + alpha * pv[u]);
       const double ri = rv[u] + malpha * av[u];
       mstore<STREAM>(r + k, ri);
       if (R32) mstore<STREAM>(r32 + k, (float)ri);
@@ -766,7 +864,7 @@ __global__ __launch_bounds__(kBlock) void k_update_xr(
     }
   }
   for (; i < n; i += stride) {
-    mstore<STREAM>(x + i, mload<STREAM>(x + i) + alpha * mload<STREAM>(p + i));
+    if (WITH_X) mstore<STREAM>(x + i, mload<STREAM>(x + i) + alpha * mload<STREAM>(p + i));
     const double ri = mload<STREAM>(r + i) + malpha * mload<STREAM>(Ap + i);
     mstore<STREAM>(r + i, ri);
     if (R32) mstore<STREAM>(r32 + i, (float)ri);
@@ -774,6 +872,32 @@ __global__ __launch_bounds__(kBlock) void k_update_xr(
   }
   s = block_sum(s);
   grid_finish(s, partials, ticket, scalars, &scalars[kNormr2], kFinishStore);
+}
+
+// CG.cpp:121's step of the previous iteration, x += alpha p, taken here
+// with CG.cpp:115's p = z + beta p: one pass reads x, p and z and writes x
+// and p, where separate updates read p twice.  alpha is still the previous
+// iteration's (this iteration's comes with its p.Ap, after this kernel).
+template <bool STREAM>
+__global__ __launch_bounds__(kBlock) void k_update_xp(int64_t n, double *__restrict__ x,
+                                                       const double *__restrict__ z,
+                                                       double *__restrict__ p,
+                                                       const double *__restrict__ scalars) {
+  const int64_t i = gtid();
+  if (i < n) {
+    const double pi = mload<STREAM>(p + i);
+    mstore<STREAM>(x + i, mload<STREAM>(x + i) + scalars[kAlpha] * pi);
+    mstore<STREAM>(p + i, mload<STREAM>(z + i) + scalars[kBeta] * pi);
+  }
+}
+
+// x += alpha p alone (the last iteration's deferred step).
+template <bool STREAM>
+__global__ __launch_bounds__(kBlock) void k_step_x(int64_t n, double *__restrict__ x,
+                                                    const double *__restrict__ p,
+                                                    const double *__restrict__ scalars) {
+  const int64_t i = gtid();
+  if (i < n) mstore<STREAM>(x + i, mload<STREAM>(x + i) + scalars[kAlpha] * mload<STREAM>(p + i));
 }
 
 // r.z with z the FP32 V-cycle output, widened exactly (CG.cpp:110, 113).
@@ -941,6 +1065,26 @@ void launch_symgs_rows(const Problem &P, const Level &L, int64_t begin, int64_t 
   HPCGX_LAUNCHED();
 }
 
+bool launch_symgs_rows_zero(const Problem &P, const Level &L, int64_t begin, int64_t end,
+                            const double *r, double *x, hipStream_t s) {
+  // x is read at byte offsets of 32 bits
+  if (!L.zmask || !use_col16(P, L, begin) || L.n >= ((int64_t)1 << 29)) return false;
+  if (end <= begin) return true;
+  const Col16 C = col16_of(P, L);
+  const unsigned g = grid_for(end - begin);
+  if (L.finest)
+    k_gs_zero16<true, true><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.zmask, L.col, L.val,
+                                                 L.diag, r, x, L.xcd);
+  else if (L.stream)
+    k_gs_zero16<true, false><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.zmask, L.col, L.val,
+                                                  L.diag, r, x, L.xcd);
+  else
+    k_gs_zero16<false, false><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.zmask, L.col, L.val,
+                                                   L.diag, r, x, L.xcd);
+  HPCGX_LAUNCHED();
+  return true;
+}
+
 void launch_symgs_list(const Level &L, const int32_t *rows, int64_t count,
                        const double *r, double *x, hipStream_t s) {
   if (count <= 0) return;
@@ -1008,15 +1152,28 @@ void launch_waxpby(int64_t n, double alpha, const double *x, double beta,
   HPCGX_LAUNCHED();
 }
 
-void launch_dot(Problem &P, int64_t n, const double *x, const double *y,
-                double *out, int finish, hipStream_t s) {
+void launch_dot_raw(int64_t n, const double *x, const double *y, double *partials,
+                    unsigned *ticket, double *scalars, double *out, int finish, hipStream_t s) {
   if (stream_vec(n))
-    k_dot<true><<<reduce_grid(n), kBlock, 0, s>>>(n, x, y, P.partials, P.ticket, P.scalars, out,
+    k_dot<true><<<reduce_grid(n), kBlock, 0, s>>>(n, x, y, partials, ticket, scalars, out,
                                                   finish);
   else
-    k_dot<false><<<reduce_grid(n), kBlock, 0, s>>>(n, x, y, P.partials, P.ticket, P.scalars, out,
+    k_dot<false><<<reduce_grid(n), kBlock, 0, s>>>(n, x, y, partials, ticket, scalars, out,
                                                    finish);
   HPCGX_LAUNCHED();
+}
+
+int64_t dot_partials(int64_t n) { return reduce_grid(n); }
+
+void launch_zero_u32(unsigned *a, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  k_zero_u32<<<grid_for(n), kBlock, 0, s>>>(a, n);
+  HPCGX_LAUNCHED();
+}
+
+void launch_dot(Problem &P, int64_t n, const double *x, const double *y,
+                double *out, int finish, hipStream_t s) {
+  launch_dot_raw(n, x, y, P.partials, P.ticket, P.scalars, out, finish, s);
 }
 
 void launch_partials_dot(Problem &P, const double *part, int64_t count, int finish,
@@ -1111,18 +1268,39 @@ void launch_update_xr(Problem &P, int64_t n, double *x, double *r,
                       const double *p, const double *Ap, hipStream_t s, float *r32) {
   const bool st = stream_vec(n);
   const unsigned g = reduce_grid(n);
-  if (r32 && st)
-    k_update_xr<true, true><<<g, kBlock, 0, s>>>(n, x, r, p, Ap, r32, P.partials, P.ticket,
-                                                 P.scalars);
-  else if (r32)
-    k_update_xr<true, false><<<g, kBlock, 0, s>>>(n, x, r, p, Ap, r32, P.partials, P.ticket,
-                                                  P.scalars);
-  else if (st)
-    k_update_xr<false, true><<<g, kBlock, 0, s>>>(n, x, r, p, Ap, nullptr, P.partials,
-                                                  P.ticket, P.scalars);
+#define HPCGX_UXR(R32, ST, WX)                                                                \
+  k_update_xr<R32, ST, WX><<<g, kBlock, 0, s>>>(n, x, r, p, Ap, r32, P.partials, P.ticket,    \
+                                                P.scalars)
+  if (x) {
+    if (r32 && st) HPCGX_UXR(true, true, true);
+    else if (r32) HPCGX_UXR(true, false, true);
+    else if (st) HPCGX_UXR(false, true, true);
+    else HPCGX_UXR(false, false, true);
+  } else {
+    if (r32 && st) HPCGX_UXR(true, true, false);
+    else if (r32) HPCGX_UXR(true, false, false);
+    else if (st) HPCGX_UXR(false, true, false);
+    else HPCGX_UXR(false, false, false);
+  }
+#undef HPCGX_UXR
+  HPCGX_LAUNCHED();
+}
+
+void launch_update_xp(int64_t n, double *x, const double *z, double *p,
+                      const double *scalars, hipStream_t s) {
+  if (stream_vec(n))
+    k_update_xp<true><<<grid_for(n), kBlock, 0, s>>>(n, x, z, p, scalars);
   else
-    k_update_xr<false, false><<<g, kBlock, 0, s>>>(n, x, r, p, Ap, nullptr, P.partials,
-                                                   P.ticket, P.scalars);
+    k_update_xp<false><<<grid_for(n), kBlock, 0, s>>>(n, x, z, p, scalars);
+  HPCGX_LAUNCHED();
+}
+
+void launch_step_x(int64_t n, double *x, const double *p, const double *scalars,
+                   hipStream_t s) {
+  if (stream_vec(n))
+    k_step_x<true><<<grid_for(n), kBlock, 0, s>>>(n, x, p, scalars);
+  else
+    k_step_x<false><<<grid_for(n), kBlock, 0, s>>>(n, x, p, scalars);
   HPCGX_LAUNCHED();
 }
 

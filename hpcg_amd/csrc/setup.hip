@@ -575,6 +575,36 @@ __global__ __launch_bounds__(kBlock) void k_tiles(int C, int S, int64_t W,
   tile_ptr[k] = lo;
 }
 
+// Level::zmask: one wavefront per tile.  Row i's colour begins at the last
+// colour pointer <= i; slot j is live for the tile when some row holds a
+// nonzero there whose column lies below its colour's first row.
+__global__ __launch_bounds__(kBlock) void k_zero_mask(int64_t n, int64_t T, int ncolors,
+                                                       const int64_t *__restrict__ cptr,
+                                                       const int32_t *__restrict__ col,
+                                                       const double *__restrict__ val,
+                                                       uint32_t *__restrict__ zmask) {
+  const int64_t t = gtid() / 64;
+  const int lane = threadIdx.x & 63;
+  if (t >= T) return;  // uniform over the wavefront
+  const int64_t i = t * kTile + lane;
+  int64_t cb = 0;
+  if (i < n) {
+    int c = 0;
+    while (c + 1 < ncolors && cptr[c + 1] <= i) ++c;
+    cb = cptr[c];
+  }
+  uint32_t m = 0;
+  for (int j = 0; j < kSlots; ++j) {
+    bool live = false;
+    if (i < n) {
+      const int64_t k = (int64_t)j * n + i;
+      live = val[k] != 0.0 && col[k] < cb;
+    }
+    if (__ballot(live)) m |= 1u << j;
+  }
+  if (lane == 0) zmask[t] = m;
+}
+
 }  // namespace
 
 // Row chunks of at most kBlock rows.  Interleaved: ordered by their relative
@@ -839,6 +869,14 @@ void finalize_level(Problem &P, int l, NaturalLevel &N, int flags,
     L.wide_tiles = std::count(info.begin(), info.end(), (int32_t)kTile32);
     L.byte_tiles = std::count(info.begin(), info.end(), (int32_t)kTile8);
     L.c16 = true;
+    L.zmask = P.arena.alloc<uint32_t>(T);
+    TempBuf cp(sizeof(int64_t) * (L.ncolors + 1));
+    HPCGX_HIP(hipMemcpyAsync(cp.p, L.color_ptr.data(), sizeof(int64_t) * (L.ncolors + 1),
+                             hipMemcpyHostToDevice, s));
+    k_zero_mask<<<(unsigned)((T + wpb - 1) / wpb), kBlock, 0, s>>>(
+        n, T, L.ncolors, cp.as<int64_t>(), L.col, L.val, L.zmask);
+    HPCGX_LAUNCHED();
+    HPCGX_HIP(hipStreamSynchronize(s));
   }
   HPCGX_HIP(hipStreamSynchronize(s));
   if (P.use_wave && n >= kWaveMinRows)

@@ -64,20 +64,41 @@ void wave_sweep(Problem &P, const Level &L, int dir, const double *r, double *x,
 }
 }  // namespace
 
+// Whether level l's first forward sweep of a V-cycle can start from an
+// unfilled x (launch_symgs_rows_zero for every colour step).
+bool zero_guess_sweep(const Problem &P, int l, int ordering) {
+  const Level &L = P.L[l];
+  if (!P.zero_guess || ordering != HPCGX_ORDER_MULTICOLOR || !L.zmask ||
+      (L.wave && P.use_wave) || !use_col16(P, L, 0))
+    return false;
+  for (int c = 0; c < L.ncolors; ++c)
+    if (L.color_ptr[c] % kTile) return false;
+  return true;
+}
+
 // The backward sweep visits every row once and leaves its final x, so with
 // rz each of its colour steps adds its r.z partials: colour c's at rz + the
-// partial counts of the colours visited before it.
+// partial counts of the colours visited before it.  zero_x: x was zero
+// when the sweep began (only with zero_guess_sweep); the forward colour
+// steps then read nothing but the x they wrote.
 void symgs(Problem &P, int l, int ordering, const double *r, double *x,
-           hipStream_t s, double *rz) {
+           hipStream_t s, double *rz, bool zero_x) {
   const Level &L = P.L[l];
   const bool wave = ordering == HPCGX_ORDER_MULTICOLOR && L.wave && P.use_wave;
   if (rz && (ordering != HPCGX_ORDER_MULTICOLOR || wave))
     throw Error(HPCGX_EINVAL, "r.z partials need the multicolour colour steps");
+  if (zero_x && !zero_guess_sweep(P, l, ordering))
+    throw Error(HPCGX_EINVAL, "zero-x sweep not available on this level");
   if (wave) {
     wave_sweep(P, L, 0, r, x, s);
     wave_sweep(P, L, 1, r, x, s);
   } else if (ordering == HPCGX_ORDER_MULTICOLOR) {
-    for (int c = 0; c < L.ncolors; ++c) colour_step(P, L, c, r, x, s);
+    for (int c = 0; c < L.ncolors; ++c) {
+      if (zero_x)
+        (void)launch_symgs_rows_zero(P, L, L.color_ptr[c], L.color_ptr[c + 1], r, x, s);
+      else
+        colour_step(P, L, c, r, x, s);
+    }
     int64_t off = 0;
     for (int c = L.ncolors - 1; c >= 0; --c) {
       colour_step(P, L, c, r, x, s, rz ? rz + off : nullptr);
@@ -94,19 +115,22 @@ void symgs(Problem &P, int l, int ordering, const double *r, double *x,
 }
 
 // ComputeMG_ref.cpp:39-63 with the fine-level SpMV and the restriction fused
-// (only the injected rows of A*x are computed).
+// (only the injected rows of A*x are computed).  x = 0 (:42) is a fill
+// kernel, or, where zero_guess_sweep allows, implied: the first forward
+// sweep then reads only the x it has written.
 void mg(Problem &P, int l, int ordering, const double *r, double *x,
         hipStream_t s, double *rz) {
   Level &L = P.L[l];
-  launch_fill(x, L.n, 0.0, s);                          // :42
+  const bool zg = zero_guess_sweep(P, l, ordering);
+  if (!zg) launch_fill(x, L.n, 0.0, s);                 // :42
   if (l + 1 < P.nlevels) {
-    symgs(P, l, ordering, r, x, s);                     // :47
+    symgs(P, l, ordering, r, x, s, nullptr, zg);        // :47
     launch_residual_restriction(P, L, r, x, L.rc, s);   // :49-51
     mg(P, l + 1, ordering, L.rc, L.xc, s);              // :52
     launch_prolongation(L, L.xc, x, s);                 // :53
     symgs(P, l, ordering, r, x, s, rz);                 // :55
   } else {
-    symgs(P, l, ordering, r, x, s, rz);                 // :59
+    symgs(P, l, ordering, r, x, s, rz, zg);             // :59
   }
 }
 
@@ -185,9 +209,12 @@ void cg_iteration(Problem &P, int ordering, bool precond, double *x, hipStream_t
     launch_partials_dot(P, P.rz_part, nrz, kFinishBeta, s);
   else
     launch_dot(P, n, P.r, P.z, nullptr, kFinishBeta, s);
-  launch_update_p(n, P.z, P.p, P.scalars, s, P.stream_p);
+  if (P.defer_x)
+    launch_update_xp(n, x, P.z, P.p, P.scalars, s);
+  else
+    launch_update_p(n, P.z, P.p, P.scalars, s, P.stream_p);
   launch_spmv_dot(P, A, P.p, P.Ap, s);
-  launch_update_xr(P, n, x, P.r, P.p, P.Ap, s);
+  launch_update_xr(P, n, P.defer_x ? nullptr : x, P.r, P.p, P.Ap, s);
 }
 
 namespace {
@@ -222,7 +249,10 @@ hipGraphExec_t iteration_graph(Problem &P, int ordering, bool precond, double *x
 //                     a launch that also forms beta = rtz/oldrtz (:112-114);
 //                     a dot kernel where no such partials are made
 //   Ap = A p, p.Ap -> one SpMV kernel that also forms alpha (:118-120)
-//   x, r updates, r.r -> one kernel (:121-124)
+//   x, r updates, r.r -> one kernel (:121-124); with defer_x the r update
+//                     and r.r, x's step (:121) being taken with the next
+//                     p update (:115) in one pass over x and p, and after
+//                     the last iteration on its own
 // With HPCGX_CG_GRAPH (and no timers or wavefront sweeps) iterations 2..
 // replay one captured graph of cg_iteration; with a tolerance the host reads
 // the residual after each replay.
@@ -250,6 +280,7 @@ void cg(Problem &P, int ordering, const double *b, double *x, int max_iter,
   for (int l = 0; l < P.nlevels; ++l) wave = wave || (P.L[l].wave && P.use_wave);
   const bool graphed = (flags & HPCGX_CG_GRAPH) && !T.on && !P.klog.on && !wave;
   const int64_t nrz = precond ? rz_partials(P, ordering) : 0;
+  const bool defer = P.defer_x;
   int k = 1;
   for (; k <= max_iter && nr / *normr0 > tol; ++k) {     // :100
     if (graphed && k >= 2) {
@@ -273,6 +304,8 @@ void cg(Problem &P, int ordering, const double *b, double *x, int max_iter,
     T.tick();
     if (k == 1)
       launch_copy(P.z, P.p, n, s);                       // :109
+    else if (defer)
+      launch_update_xp(n, x, P.z, P.p, P.scalars, s);    // :121 of k-1, :115
     else
       launch_update_p(n, P.z, P.p, P.scalars, s, P.stream_p);        // :115
     T.tock(t2);
@@ -280,10 +313,15 @@ void cg(Problem &P, int ordering, const double *b, double *x, int max_iter,
     launch_spmv_dot(P, A, P.p, P.Ap, s);                 // :118-120
     T.tock(t3);
     T.tick();
-    launch_update_xr(P, n, x, P.r, P.p, P.Ap, s);        // :121-123
+    launch_update_xr(P, n, defer ? nullptr : x, P.r, P.p, P.Ap, s);  // :121-123
     T.tock(t2);
     *niters = k;
     if (check_each || k == max_iter) nr = std::sqrt(read_scalar(P, kNormr2, s));
+  }
+  if (defer && *niters > 0) {  // the last iteration's x += alpha p (:121)
+    T.tick();
+    launch_step_x(n, x, P.p, P.scalars, s);
+    T.tock(t2);
   }
   if (!check_each) nr = std::sqrt(read_scalar(P, kNormr2, s));
   HPCGX_HIP(hipStreamSynchronize(s));

@@ -371,6 +371,16 @@ int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, i
  *                      reads r and z again (default 1; applies to the
  *                      multicolour colour steps with narrow column ids).
  *                      Changes the dot's summation order.
+ *   "zero_guess"       1/0: the V-cycle's first forward sweep of a level
+ *                      (multicolour ordering, narrow column ids) starts from
+ *                      the zero x without filling it: each colour step reads
+ *                      only the slots and x values of colours already
+ *                      updated in that sweep (default 1; the same x as a
+ *                      zero fill followed by the full sweep)
+ *   "defer_x"          1/0: hpcgx_cg takes each iteration's x += alpha p
+ *                      with the next iteration's p update, in one pass over
+ *                      x and p, and the last one after the loop (default 1;
+ *                      the same x and residuals)
  *   "spmv_dot_blocks"  workgroups of the persistent SpMV + dot kernel
  *                      (8..8192, rounded down to a multiple of 8; default
  *                      4096).  Changes the dot's summation order.
