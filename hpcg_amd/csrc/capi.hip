@@ -99,7 +99,7 @@ void alloc_solver_state(hpcgx::Problem &P) {
   P.chunk_partials = P.arena.alloc<double>(P.nchunk_partials);
   // one partial per wavefront of each finest colour step's workgroups: at
   // most one workgroup per colour more than n / kBlock (up to 64 colours)
-  P.nrz_partials = ((n + hpcgx::kBlock - 1) / hpcgx::kBlock + 64) * hpcgx::kRzWaves;
+  P.nrz_partials = (n + hpcgx::kTile - 1) / hpcgx::kTile + 64;  // one per wavefront of rows
   P.rz_part = P.arena.alloc<double>(P.nrz_partials);
 }
 
@@ -413,6 +413,16 @@ int hpcgx_tune(hpcgx_problem *p, const char *key, double value) {
       P.defer_x = value != 0.0;
       return;
     }
+    if (k == "gs_block") {
+      if (value != 64 && value != 128 && value != 256 && value != 512)
+        throw Error(HPCGX_EINVAL, "gs_block must be 64, 128, 256 or 512");
+      P.gs_block = (int)value;
+      return;
+    }
+    if (k == "gs_waves") {
+      P.gs_waves = value != 0.0 ? 1 : 0;
+      return;
+    }
     if (k == "fp16_pairs") {
       P.use_pairs = value != 0.0;
       return;
@@ -522,6 +532,10 @@ int hpcgx_autotune(hpcgx_problem *p, int ordering, int iterations, char *report,
     }
     knobs.push_back({"spmv_order", {0, 1}});
     knobs.push_back({"xcd_remap", {0, 1}});
+    if (P.L[0].c16) {  // the finest colour steps' shape: workgroup size, register cap
+      knobs.push_back({"gs_block", {256, 128, 512, 64}});
+      knobs.push_back({"gs_waves", {1, 0}});
+    }
     std::string log;
     char line[160];
     for (const Knob &k : knobs) {  // one knob at a time, the others at their best

@@ -18,7 +18,6 @@ constexpr int kSlots = 27;        // ELL width: the 27-point stencil's maximum
 constexpr int kMaxLevels = 8;
 constexpr int kMaxColors = 32;    // first-fit colours of a <=27-entry row: <=27
 constexpr int kBlock = 256;       // threads per workgroup for streaming kernels
-constexpr int kRzWaves = kBlock / 64;  // r.z partials per colour-step workgroup
 // A launch holds fewer than 2^32 work-items (the dispatch packet's grid
 // size is 32 bits): kernels over all n*27 ELL entries loop over the slots
 // inside one work-item per row.
@@ -232,6 +231,8 @@ struct Problem {
   bool stream_p = false;      // p update with non-temporal accesses ("stream_p")
   bool zero_guess = true;     // V-cycle's first forward sweeps skip the zero x ("zero_guess")
   bool defer_x = true;        // CG takes x's step with the next p update ("defer_x")
+  int gs_block = kBlock;      // workgroup size of the finest colour steps ("gs_block")
+  int gs_waves = 1;           // 1: register caps on the finest colour steps ("gs_waves")
   int spmv_dot_blocks = 4096; // persistent grid of the fused SpMV + dot
   // The fused SpMV + dot with one workgroup per row chunk of the fine level
   // ("spmv_dot_chunks"): the p.Ap partial of each chunk, summed in fixed
@@ -313,6 +314,13 @@ struct Problem {
     IterGraph graph;             // captured fleet iteration
     double graph_tol = -1.0;
   } fleet;
+  // events of CG's phase timers (HPCGX_CG_TIMERS), reused call after call
+  struct EventPool {
+    std::vector<hipEvent_t> ev;
+    ~EventPool() {
+      for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+  } tlog;
   // event log of the finest level's multicolour Gauss-Seidel launches
   struct KernelLog {
     bool on = false;
@@ -371,9 +379,9 @@ void launch_gather(const int32_t *idx, const double *src, double *dst,
                    int64_t n, hipStream_t s);
 void launch_spmv(const Problem &P, const Level &L, const double *x, double *y,
                  hipStream_t s);
-// With dpart, wavefront w of logical workgroup b of the launch also stores
-// the sum of r_i * (new x_i) over its rows to dpart[b * kRzWaves + w] (16-bit
-// column id path only).
+// With dpart, wavefront k of the launch (rows begin + 64 k ..) also stores
+// the sum of r_i * (new x_i) over its rows to dpart[k] (16-bit column id
+// path only).
 void launch_symgs_rows(const Problem &P, const Level &L, int64_t begin, int64_t end,
                        const double *r, double *x, hipStream_t s, double *dpart = nullptr);
 void launch_symgs_list(const Level &L, const int32_t *rows, int64_t count,

@@ -269,36 +269,40 @@ __global__ __launch_bounds__(kBlock) void k_gs_range(
   if (i < end) gs_row<STREAM>(i, n, col, val, r, x);
 }
 
-// k_gs_range with 16-bit column ids (begin is a multiple of kTile).  Capped
-// at 72 VGPRs, 7 waves per SIMD (the compiler's own choice, 77, allows 6).
-template <bool STREAM, bool FINEST>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_gs_range16(
+// k_gs_range with 16-bit column ids (begin is a multiple of kTile).  BS
+// threads per workgroup; W: the waves-per-EU floor given to the compiler
+// (7 caps it at 72 VGPRs, where its own choice, 77, allows 6 waves per
+// SIMD; 1 leaves it free).  The finest level's workgroup size and cap are
+// tuning knobs ("gs_block", "gs_waves"); other levels use kBlock and 7.
+template <bool STREAM, bool FINEST, int BS, int W>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_gs_range16(
     int64_t begin, int64_t end, int64_t n, Col16 C, const int32_t *__restrict__ col,
     const double *__restrict__ val, const double *__restrict__ r, double *x, bool xcd) {
-  const int64_t i = begin + xcd_block(blockIdx.x, gridDim.x, xcd) * kBlock + threadIdx.x;
+  const int64_t i = begin + xcd_block(blockIdx.x, gridDim.x, xcd) * BS + threadIdx.x;
   if (i < end) gs_row16<STREAM>(C, i, n, col, val, r, x);
 }
 
 // k_gs_range16 that also stores the sum of r_i * (new x_i) over the rows of
-// wavefront w of logical workgroup b to dpart[b * kRzWaves + w].  In the last
-// sweep of a V-cycle every row is updated once, so these partials hold r.z
-// (CG.cpp:110) without reading r and z again.  One partial per wavefront: a
-// 64-lane shuffle tree, with no LDS and no workgroup barrier (a per-workgroup
-// sum cost 40 us per launch at 512x512x640).  Capped at 80 VGPRs: uncapped the
-// compiler takes 120 (4 waves per SIMD); under the cap it fits 69, 7 waves, as
-// k_gs_range16 runs.
-template <bool STREAM>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_gs_rz16(
+// each wavefront with rows: wavefront k of the launch (rows begin + 64 k ..)
+// writes dpart[k], whatever the workgroup size.  In the last sweep of a
+// V-cycle every row is updated once, so these partials hold r.z (CG.cpp:110)
+// without reading r and z again.  One partial per wavefront: a 64-lane
+// shuffle tree, with no LDS and no workgroup barrier (a per-workgroup sum
+// cost 40 us per launch at 512x512x640).  Under W = 7 the compiler fits 70
+// VGPRs (7 waves per SIMD, as k_gs_range16 runs); uncapped it takes 120
+// (4 waves).
+template <bool STREAM, int BS, int W>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_gs_rz16(
     int64_t begin, int64_t end, int64_t n, Col16 C, const int32_t *__restrict__ col,
     const double *__restrict__ val, const double *__restrict__ r, double *x, bool xcd,
     double *__restrict__ dpart) {
-  const int64_t b = xcd_block(blockIdx.x, gridDim.x, xcd);
-  const int64_t i = begin + b * kBlock + threadIdx.x;
+  const int64_t t = xcd_block(blockIdx.x, gridDim.x, xcd) * BS + threadIdx.x;
+  const int64_t i = begin + t;
   double s = 0.0;
   if (i < end) s = r[i] * gs_row16<STREAM>(C, i, n, col, val, r, x);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  if ((threadIdx.x & 63) == 0) dpart[b * kRzWaves + (threadIdx.x >> 6)] = s;
+  if ((threadIdx.x & 63) == 0 && i < end) dpart[t >> 6] = s;
 }
 
 // The first forward sweep of a V-cycle level from x = 0 (ComputeMG_ref.cpp:
@@ -387,12 +391,12 @@ __device__ inline void gs_row_zero(const Col16 &C, uint32_t zm, int64_t begin, i
   x[i] = sum / d;
 }
 
-template <bool STREAM, bool FINEST>
-__global__ __launch_bounds__(kBlock) void k_gs_zero16(
+template <bool STREAM, bool FINEST, int BS>
+__global__ __launch_bounds__(BS) void k_gs_zero16(
     int64_t begin, int64_t end, int64_t n, Col16 C, const uint32_t *__restrict__ zmask,
     const int32_t *__restrict__ col, const double *__restrict__ val,
     const double *__restrict__ diag, const double *__restrict__ r, double *x, bool xcd) {
-  const int64_t i = begin + xcd_block(blockIdx.x, gridDim.x, xcd) * kBlock + threadIdx.x;
+  const int64_t i = begin + xcd_block(blockIdx.x, gridDim.x, xcd) * BS + threadIdx.x;
   if (i < end) gs_row_zero<STREAM>(C, zmask[tile_of(i)], begin, i, n, col, val, diag, r, x);
 }
 
@@ -1022,22 +1026,45 @@ void launch_symgs_rows(const Problem &P, const Level &L, int64_t begin, int64_t 
     throw Error(HPCGX_EINVAL, "colour step with r.z partials needs 16-bit column ids");
   if (use_col16(P, L, begin)) {
     const Col16 C = col16_of(P, L);
-    const unsigned g = grid_for(end - begin);
-    if (dpart && (L.finest || L.stream))
-      k_gs_rz16<true><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col, L.val, r, x, L.xcd,
-                                           dpart);
-    else if (dpart)
-      k_gs_rz16<false><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col, L.val, r, x, L.xcd,
-                                            dpart);
-    else if (L.finest)
-      k_gs_range16<true, true><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col, L.val, r, x,
-                                                    L.xcd);
+    const int64_t rows = end - begin;
+    if (L.finest) {  // tuned workgroup size and register cap
+      const bool cap = P.gs_waves != 0;
+#define HPCGX_GS(BS)                                                                           \
+  do {                                                                                         \
+    const unsigned g = (unsigned)((rows + BS - 1) / BS);                                       \
+    if (dpart && cap)                                                                          \
+      k_gs_rz16<true, BS, 7><<<g, BS, 0, s>>>(begin, end, L.n, C, L.col, L.val, r, x, L.xcd, \
+                                              dpart);                                          \
+    else if (dpart)                                                                            \
+      k_gs_rz16<true, BS, 1><<<g, BS, 0, s>>>(begin, end, L.n, C, L.col, L.val, r, x, L.xcd, \
+                                              dpart);                                          \
+    else if (cap)                                                                              \
+      k_gs_range16<true, true, BS, 7><<<g, BS, 0, s>>>(begin, end, L.n, C, L.col, L.val, r,  \
+                                                       x, L.xcd);                              \
+    else                                                                                       \
+      k_gs_range16<true, true, BS, 1><<<g, BS, 0, s>>>(begin, end, L.n, C, L.col, L.val, r,  \
+                                                       x, L.xcd);                              \
+  } while (0)
+      switch (P.gs_block) {
+        case 64: HPCGX_GS(64); break;
+        case 128: HPCGX_GS(128); break;
+        case 512: HPCGX_GS(512); break;
+        default: HPCGX_GS(256); break;
+      }
+#undef HPCGX_GS
+      HPCGX_LAUNCHED();
+      return;
+    }
+    const unsigned g = grid_for(rows);
+    if (dpart)
+      k_gs_rz16<true, kBlock, 7><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col, L.val, r, x,
+                                                      L.xcd, dpart);
     else if (L.stream)
-      k_gs_range16<true, false><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col, L.val, r, x,
-                                                     L.xcd);
+      k_gs_range16<true, false, kBlock, 7><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col,
+                                                                L.val, r, x, L.xcd);
     else
-      k_gs_range16<false, false><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col, L.val, r, x,
-                                                      L.xcd);
+      k_gs_range16<false, false, kBlock, 7><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col,
+                                                                 L.val, r, x, L.xcd);
     HPCGX_LAUNCHED();
     return;
   }
@@ -1071,16 +1098,25 @@ bool launch_symgs_rows_zero(const Problem &P, const Level &L, int64_t begin, int
   if (!L.zmask || !use_col16(P, L, begin) || L.n >= ((int64_t)1 << 29)) return false;
   if (end <= begin) return true;
   const Col16 C = col16_of(P, L);
-  const unsigned g = grid_for(end - begin);
-  if (L.finest)
-    k_gs_zero16<true, true><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.zmask, L.col, L.val,
-                                                 L.diag, r, x, L.xcd);
-  else if (L.stream)
-    k_gs_zero16<true, false><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.zmask, L.col, L.val,
-                                                  L.diag, r, x, L.xcd);
-  else
-    k_gs_zero16<false, false><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.zmask, L.col, L.val,
-                                                   L.diag, r, x, L.xcd);
+  const int64_t rows = end - begin;
+  if (L.finest) {
+#define HPCGX_GZ(BS)                                                                          \
+  k_gs_zero16<true, true, BS><<<(unsigned)((rows + BS - 1) / BS), BS, 0, s>>>(                \
+      begin, end, L.n, C, L.zmask, L.col, L.val, L.diag, r, x, L.xcd)
+    switch (P.gs_block) {
+      case 64: HPCGX_GZ(64); break;
+      case 128: HPCGX_GZ(128); break;
+      case 512: HPCGX_GZ(512); break;
+      default: HPCGX_GZ(256); break;
+    }
+#undef HPCGX_GZ
+  } else if (L.stream) {
+    k_gs_zero16<true, false, kBlock><<<grid_for(rows), kBlock, 0, s>>>(
+        begin, end, L.n, C, L.zmask, L.col, L.val, L.diag, r, x, L.xcd);
+  } else {
+    k_gs_zero16<false, false, kBlock><<<grid_for(rows), kBlock, 0, s>>>(
+        begin, end, L.n, C, L.zmask, L.col, L.val, L.diag, r, x, L.xcd);
+  }
   HPCGX_LAUNCHED();
   return true;
 }

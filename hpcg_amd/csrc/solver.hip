@@ -49,10 +49,10 @@ void colour_step(Problem &P, const Level &L, int c, const double *r, double *x,
   log.done();
 }
 
-// r.z partials of a colour step over `rows` rows: one per wavefront of its
-// workgroups
+// r.z partials of a colour step over `rows` rows: one per wavefront with
+// rows (kTile rows each)
 int64_t rz_slots(int64_t rows) {
-  return rows > 0 ? (rows + kBlock - 1) / kBlock * kRzWaves : 0;
+  return rows > 0 ? (rows + kTile - 1) / kTile : 0;
 }
 
 // A whole sweep direction as one wavefront launch.
@@ -158,30 +158,43 @@ double read_scalar(const Problem &P, int slot, hipStream_t s) {
   return v;
 }
 
-struct PhaseTimer {  // HIP-event timing of CG phases (times[1..5])
+// HIP-event timing of CG's phases (times[1..5]): an event pair around each
+// phase, recorded on the launch stream; the pairs are read once, when the
+// call has finished, so timing adds no host waits between phases.  The
+// events come from the problem's pool and are reused by later calls.
+struct PhaseTimer {
+  Problem &P;
   bool on;
   hipStream_t s;
-  hipEvent_t a = nullptr, b = nullptr;
-  PhaseTimer(bool on_, hipStream_t s_) : on(on_), s(s_) {
-    if (on) {
-      HPCGX_HIP(hipEventCreate(&a));
-      HPCGX_HIP(hipEventCreate(&b));
+  size_t used = 0;
+  std::vector<std::pair<int, size_t>> spans;  // (times[] slot, index of the start event)
+  PhaseTimer(Problem &P_, bool on_, hipStream_t s_) : P(P_), on(on_), s(s_) {}
+  hipEvent_t next() {
+    auto &pool = P.tlog.ev;
+    if (used == pool.size()) {
+      hipEvent_t e;
+      HPCGX_HIP(hipEventCreate(&e));
+      pool.push_back(e);
     }
-  }
-  ~PhaseTimer() {
-    if (a) (void)hipEventDestroy(a);
-    if (b) (void)hipEventDestroy(b);
+    return pool[used++];
   }
   void tick() {
-    if (on) HPCGX_HIP(hipEventRecord(a, s));
+    if (on) HPCGX_HIP(hipEventRecord(next(), s));
   }
-  void tock(double &acc) {
+  void tock(int slot) {
     if (!on) return;
-    HPCGX_HIP(hipEventRecord(b, s));
-    HPCGX_HIP(hipEventSynchronize(b));
-    float ms = 0.f;
-    HPCGX_HIP(hipEventElapsedTime(&ms, a, b));
-    acc += ms * 1e-3;
+    spans.emplace_back(slot, used - 1);
+    HPCGX_HIP(hipEventRecord(next(), s));
+  }
+  // adds each phase's time to acc[slot] (seconds); waits for the stream
+  void read(double *acc) {
+    if (!on || spans.empty()) return;
+    HPCGX_HIP(hipEventSynchronize(P.tlog.ev[used - 1]));
+    for (const auto &sp : spans) {
+      float ms = 0.f;
+      HPCGX_HIP(hipEventElapsedTime(&ms, P.tlog.ev[sp.second], P.tlog.ev[sp.second + 1]));
+      acc[sp.first] += ms * 1e-3;
+    }
   }
 };
 }  // namespace
@@ -262,17 +275,17 @@ void cg(Problem &P, int ordering, const double *b, double *x, int max_iter,
   const auto t_begin = std::chrono::steady_clock::now();
   const Level &A = P.L[0];
   const int64_t n = A.n;
-  double t1 = 0, t2 = 0, t3 = 0, t5 = 0;
-  PhaseTimer T((flags & HPCGX_CG_TIMERS) != 0, s);
+  double tp[6] = {0, 0, 0, 0, 0, 0};  // times[1..5] of this call
+  PhaseTimer T(P, (flags & HPCGX_CG_TIMERS) != 0, s);
 
   *niters = 0;
   launch_copy(x, P.p, n, s);                             // :86
   T.tick();
   launch_spmv(P, A, P.p, P.Ap, s);                          // :87
-  T.tock(t3);
+  T.tock(3);
   T.tick();
   launch_residual0(P, n, b, P.Ap, P.r, s);               // :88-89
-  T.tock(t2);
+  T.tock(2);
   double nr = std::sqrt(read_scalar(P, kNormr2, s));
   *normr0 = nr;
   const bool check_each = tol > 0.0;
@@ -294,13 +307,13 @@ void cg(Problem &P, int ordering, const double *b, double *x, int max_iter,
       mg(P, 0, ordering, P.r, P.z, s, nrz ? P.rz_part : nullptr);  // :103
     else
       launch_copy(P.r, P.z, n, s);                       // :105
-    T.tock(t5);
+    T.tock(5);
     T.tick();
     if (nrz)                                             // :110 / :113-114
       launch_partials_dot(P, P.rz_part, nrz, kFinishBeta, s);
     else
       launch_dot(P, n, P.r, P.z, nullptr, kFinishBeta, s);
-    T.tock(t1);
+    T.tock(1);
     T.tick();
     if (k == 1)
       launch_copy(P.z, P.p, n, s);                       // :109
@@ -308,31 +321,29 @@ void cg(Problem &P, int ordering, const double *b, double *x, int max_iter,
       launch_update_xp(n, x, P.z, P.p, P.scalars, s);    // :121 of k-1, :115
     else
       launch_update_p(n, P.z, P.p, P.scalars, s, P.stream_p);        // :115
-    T.tock(t2);
+    T.tock(2);
     T.tick();
     launch_spmv_dot(P, A, P.p, P.Ap, s);                 // :118-120
-    T.tock(t3);
+    T.tock(3);
     T.tick();
     launch_update_xr(P, n, defer ? nullptr : x, P.r, P.p, P.Ap, s);  // :121-123
-    T.tock(t2);
+    T.tock(2);
     *niters = k;
     if (check_each || k == max_iter) nr = std::sqrt(read_scalar(P, kNormr2, s));
   }
   if (defer && *niters > 0) {  // the last iteration's x += alpha p (:121)
     T.tick();
     launch_step_x(n, x, P.p, P.scalars, s);
-    T.tock(t2);
+    T.tock(2);
   }
   if (!check_each) nr = std::sqrt(read_scalar(P, kNormr2, s));
   HPCGX_HIP(hipStreamSynchronize(s));
   *normr = nr;
+  T.read(tp);
   if (times) {
     times[0] += std::chrono::duration<double>(std::chrono::steady_clock::now() -
                                               t_begin).count();
-    times[1] += t1;
-    times[2] += t2;
-    times[3] += t3;
-    times[5] += t5;
+    for (int k = 1; k < 6; ++k) times[k] += tp[k];
   }
   check_status(P);
   if (!std::isfinite(nr))

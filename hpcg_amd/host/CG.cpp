@@ -3,8 +3,8 @@
 // every iteration (V-cycle, SpMV, dots, updates) stays on the device, with
 // the reductions fused into the kernels that produce their operands.
 // times[0] covers the whole call, copies included; times[1..5] (dot,
-// WAXPBY, SpMV, all-reduce = 0, MG) are filled only when cg_phase_timers is
-// set, since timing each phase makes the host wait for the device.
+// WAXPBY, SpMV, all-reduce = 0, MG) come from HIP events around each phase
+// when cg_phase_timers is set (the default), read once per call.
 #include <chrono>
 #include <cstdio>
 
@@ -28,7 +28,7 @@ int DeviceCG(const SparseMatrix &A, const Vector &b, Vector &x, int max_iter,
   if (!rc) rc = Upload(*L, x, dx);
   if (rc) return rc;
   double t[6] = {0, 0, 0, 0, 0, 0};
-  // timed sets (tolerance 0) replay a captured graph of the iteration
+  // without phase timers, iterations 2.. replay a captured graph
   rc = hpcgx_cg(L->ctx->problem, ordering, db, dx, max_iter, tolerance,
                 doPreconditioning ? 1 : 0,
                 (cg_phase_timers ? HPCGX_CG_TIMERS : 0) | HPCGX_CG_GRAPH, &niters, &normr,

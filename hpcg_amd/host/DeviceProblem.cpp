@@ -13,7 +13,8 @@
 namespace hpcg_amd {
 
 int optimized_ordering = HPCGX_ORDER_MULTICOLOR;
-bool cg_phase_timers = false;
+bool cg_phase_timers = true;
+bool grid_knowledge = true;
 
 namespace {
 DeviceContext *g_current = nullptr;
@@ -54,9 +55,11 @@ DeviceContext *EnsureDeviceProblem(SparseMatrix &A) {
       return nullptr;
     }
     hpcgx_host_level h{};
-    h.nx = M->geom->nx;
-    h.ny = M->geom->ny;
-    h.nz = M->geom->nz;
+    if (grid_knowledge) {  // lets the setup propose the schedule from the grid
+      h.nx = M->geom->nx;
+      h.ny = M->geom->ny;
+      h.nz = M->geom->nz;
+    }
     h.nrow = M->localNumberOfRows;
     h.nonzerosInRow = M->nonzerosInRow;
     h.mtxIndL = M->mtxIndL;
@@ -70,6 +73,11 @@ DeviceContext *EnsureDeviceProblem(SparseMatrix &A) {
   auto *c = new DeviceContext{};
   int rc = hpcgx_optimize_problem(levels.data(), (int)levels.size(), 0, &c->problem);
   if (rc) {
+    Report("OptimizeProblem", rc);
+    free_context(c);
+    return nullptr;
+  }
+  if (!grid_knowledge && (rc = hpcgx_tune(c->problem, "affine8", 0.0))) {
     Report("OptimizeProblem", rc);
     free_context(c);
     return nullptr;

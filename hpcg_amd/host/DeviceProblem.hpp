@@ -50,9 +50,15 @@ constexpr int64_t kCopyChunk = (int64_t)1 << 23;
 // use the natural (lexicographic) order.
 extern int optimized_ordering;
 
-// Record HIP-event times of CG's phases (times[1..5]); off by default, as
-// each phase then waits for the device.
+// Record HIP-event times of CG's phases (times[1..5]); on by default.  The
+// events are read once per CG call, but timed iterations launch one by one
+// instead of replaying a captured graph.
 extern bool cg_phase_timers;
+// Give OptimizeProblem each level's grid dimensions (default) and keep the
+// affine offset words.  Off: the schedules come from the row-by-row
+// traversal of the matrix alone and every narrow offset word is loaded, so
+// nothing uses what is known of the grid beyond the matrix itself.
+extern bool grid_knowledge;
 
 // Builds the device problem for A's hierarchy unless it exists; returns
 // nullptr (after printing why) on failure.

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../OptimizedKernels.hpp"
+#include "../DeviceProblem.hpp"
 #include "Driver.hpp"
 
 namespace {
@@ -139,7 +140,11 @@ std::string ReportFile(const SparseMatrix &A, const RunSummary &s, std::string *
   const Geometry &g = *A.geom;
 
   KeyValue doc;
-  doc.add("Implementation", "hpcg_amd: HIP kernels for MI355X (gfx950), one rank per GPU");
+  doc.add("Implementation",
+          std::string("hpcg_amd: HIP kernels for MI355X (gfx950), one rank per GPU") +
+              (hpcg_amd::grid_knowledge ? ""
+                                        : "; schedules and column offsets from the matrix "
+                                          "alone (--no-grid-knowledge)"));
   doc.add("Machine Summary").add("Distributed Processes", g.size);
   doc.get("Machine Summary").add("Threads per processes", g.numThreads);
   KeyValue &gp = doc.add("Global Problem Dimensions");

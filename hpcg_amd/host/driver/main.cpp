@@ -12,7 +12,8 @@
 // reported as part of the optimization phase (times[7]).
 //
 // Usage: xhpcg_amd [nx ny nz] [--nx=N --ny=N --nz=N] [--rt=seconds]
-//                  [--ordering=multicolor|lexicographic] [--phase-timers]
+//                  [--ordering=multicolor|lexicographic] [--no-phase-timers]
+//                  [--no-grid-knowledge]
 // --rt=0 (the default) runs one CG set (the reference's quick path).
 #include <hip/hip_runtime_api.h>
 
@@ -39,7 +40,8 @@ double now() {
 int usage(const char *argv0) {
   std::fprintf(stderr,
                "usage: %s [nx ny nz] [--nx=N] [--ny=N] [--nz=N] [--rt=SECONDS]\n"
-               "          [--ordering=multicolor|lexicographic] [--phase-timers]\n",
+               "          [--ordering=multicolor|lexicographic] [--no-phase-timers]\n"
+               "          [--no-grid-knowledge]\n",
                argv0);
   return 2;
 }
@@ -57,6 +59,8 @@ int main(int argc, char *argv[]) {
     else if (!std::strncmp(a, "--nz=", 5)) nz = std::atoi(a + 5);
     else if (!std::strncmp(a, "--rt=", 5)) runningTime = std::atof(a + 5);
     else if (!std::strcmp(a, "--phase-timers")) hpcg_amd::cg_phase_timers = true;
+    else if (!std::strcmp(a, "--no-phase-timers")) hpcg_amd::cg_phase_timers = false;
+    else if (!std::strcmp(a, "--no-grid-knowledge")) hpcg_amd::grid_knowledge = false;
     else if (!std::strcmp(a, "--ordering=multicolor")) {
       hpcg_amd::optimized_ordering = HPCGX_ORDER_MULTICOLOR;
       ordering_name = "multicolor";

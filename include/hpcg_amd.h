@@ -381,6 +381,11 @@ int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, i
  *                      with the next iteration's p update, in one pass over
  *                      x and p, and the last one after the loop (default 1;
  *                      the same x and residuals)
+ *   "gs_block"         threads per workgroup of the finest level's FP64
+ *                      colour steps: 64, 128, 256 or 512 (default 256)
+ *   "gs_waves"         1/0: the finest level's FP64 colour steps run under
+ *                      register caps that allow 7 waves per SIMD (default
+ *                      1); 0 leaves the register count to the compiler
  *   "spmv_dot_blocks"  workgroups of the persistent SpMV + dot kernel
  *                      (8..8192, rounded down to a multiple of 8; default
  *                      4096).  Changes the dot's summation order.
@@ -401,7 +406,8 @@ int hpcgx_tune(hpcgx_problem *p, const char *key, double value);
 
 /* No counterpart in the reference: picks the tuning knobs for this problem
  * and machine by timing `iterations` PCG iterations (plus one warm-up) per
- * candidate, one knob at a time ("col8", "spmv_dot_chunks", "spmv_dot_blocks", "spmv_order", "xcd_remap";
+ * candidate, one knob at a time ("col8", "spmv_dot_chunks", "spmv_dot_blocks", "spmv_order",
+ * "xcd_remap", and, where the finest level has narrow column ids, "gs_block" and "gs_waves";
  * a fleet times hpcgx_cg_fleet and tunes "fleet_blocks" in place of
  * "spmv_dot_blocks"), keeping the fastest value of each.  Writes one line per timing and per
  * choice into `report` (may be NULL).  Uses the default stream; synchronous.

@@ -109,7 +109,7 @@ def test_reference_matrices_pass_and_alterations_fail(hp, g16):
     cases.append(("column", swap_rows(golden_levels(g16), 1234, 2730), 0,
                   "a column differs"))
     lv = golden_levels(g16)
-    lv[1]["vals"][27 * 100 + lv[1]["diag_slot"][100]] = 25.0
+    lv[1]["vals"][27 * 100 + int(lv[1]["diag_slot"][100])] = 25.0
     cases.append(("diagonal", lv, 1, "a value differs"))
     lv = golden_levels(g16)
     lv[0]["f2c"][17] += 1            # an odd-x fine point: injected by no coarse row

@@ -223,6 +223,26 @@ def test_driver_result_file_matches_reference(tmp_path):
     assert int(d["Iteration Count Information::Optimized CG iterations per set"]) >= 50
     assert float(d["Final Summary::HPCG result is VALID with a GFLOP/s rating of"]) > 0
     assert float(d["Memory Use Information::Memory used for OptimizeProblem data (Gbytes)"]) > 0
+    # the phase times and rates are measured (ReportResults.cpp:322-327,346-349)
+    for k in ("DDOT", "WAXPBY", "SpMV", "MG"):
+        assert float(d["Benchmark Time Summary::" + k]) > 0, k
+        assert float(d["GFLOP/s Summary::Raw " + k]) > 0, k
+    t = [float(d["Benchmark Time Summary::" + k]) for k in ("DDOT", "WAXPBY", "SpMV", "MG")]
+    assert sum(t) <= float(d["Benchmark Time Summary::Total"])
+
+
+@pytest.mark.gpu
+def test_driver_without_grid_knowledge(tmp_path):
+    """Schedules from the traversal of the matrix alone and every offset
+    word loaded: the same convergence and a VALID run."""
+    assert os.path.exists(DRIVER), "xhpcg_amd not built (run __graft_entry__.build())"
+    res = _run([DRIVER, "32", "24", "16", "--no-grid-knowledge"], timeout=300)
+    assert res.returncode == 0, res.stderr
+    assert "HPCG result is VALID" in res.stdout
+    ref = _run([DRIVER, "32", "24", "16"], timeout=300)
+    assert ref.returncode == 0, ref.stderr
+    pat = r"Optimized CG: (\d+) iterations"
+    assert _field(res.stdout, pat) == _field(ref.stdout, pat)
 
 
 @pytest.mark.gpu

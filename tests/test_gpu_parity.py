@@ -623,8 +623,9 @@ def test_autotune(hp, torch):
     chosen = [l for l in rep if l.startswith("chosen ")]
     assert {l.split()[1].split("=")[0] for l in chosen} == {"col8", "spmv_dot_chunks",
                                                             "spmv_dot_blocks", "spmv_order",
-                                                            "xcd_remap"}
-    assert len([l for l in rep if "ms/iteration" in l]) == 2 + 2 + 4 + 2 + 2
+                                                            "xcd_remap", "gs_block",
+                                                            "gs_waves"}
+    assert len([l for l in rep if "ms/iteration" in l]) == 2 + 2 + 4 + 2 + 2 + 4 + 2
     x.zero_()
     r1 = P.cg(b, x, max_iter=10, tolerance=0.0)
     assert r1["normr"] == pytest.approx(r0["normr"], rel=1e-10)
