@@ -80,6 +80,15 @@ def parse():
                          "(0 = skip)")
     ap.add_argument("--fleet-only", action="store_true",
                     help="run only the fleet measurement and print its JSON")
+    ap.add_argument("--block-only", action="store_true",
+                    help="run only the block PCG measurement (--block, --block-grid) and "
+                         "print its JSON")
+    ap.add_argument("--block-iters", type=int, default=0,
+                    help="block PCG: this many iterations at tolerance 0 instead of a solve to "
+                         "--tol-compare, without the single-RHS comparison (for profiling)")
+    ap.add_argument("--mixed-steps", type=int, default=0,
+                    help="after the timed FP64 set, this many mixed-precision iterations at "
+                         "tolerance 0 (for profiling the FP32 V-cycle on its own)")
     ap.add_argument("--fleet-compare", type=int, default=1, choices=[0, 1],
                     help="also solve the fleet's problems one at a time (0 = skip)")
     ap.add_argument("--autotune", type=int, default=1, choices=[0, 1],
@@ -204,6 +213,15 @@ def block_solve(P, s, a, log):
     X = torch.zeros_like(B)
     P.cg_block_mixed(B, X, max_iter=2, tolerance=0.0)  # warm
     X.zero_()
+    if a.block_iters > 0:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = P.cg_block_mixed(B, X, max_iter=a.block_iters, tolerance=0.0, recompute=10)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        log("block s=%d: %d iterations in %.3f s" % (s, r["niters"], dt))
+        return {"s": s, "iterations": r["niters"], "seconds": round(dt, 4),
+                "ms_per_iteration": round(dt / max(r["niters"], 1) * 1e3, 4)}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     r = P.cg_block_mixed(B, X, max_iter=a.max_iters, tolerance=a.tol_compare, recompute=10)
@@ -319,6 +337,14 @@ def main():
     if a.fleet_only:
         print(json.dumps({"fleet": fleet_solve(max(a.fleet, 1), a, log)}), flush=True)
         return
+    if a.block_only:
+        g = a.block_grid
+        log("block PCG problem %d^3" % g)
+        PB = hpcg_amd.Problem.generate(g, g, g, nlevels=4)
+        out = [dict(block_solve(PB, s, a, log), grid=[g, g, g]) for s in (a.block or [8])]
+        PB.close()
+        print(json.dumps({"block_pcg": out}), flush=True)
+        return
     log("generate %dx%dx%d on device" % (a.nx, a.ny, a.nz))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -355,6 +381,11 @@ def main():
         P.cg(b, x, max_iter=a.steps, tolerance=0.0, ordering=a.ordering)
     P.kernel_timing(False)
     ks_ms, ks_n = P.kernel_timing_read()
+    if a.mixed_steps > 0:
+        P.prepare_mixed()
+        x.zero_()
+        P.cg_mixed(b, x, max_iter=a.mixed_steps, tolerance=0.0, recompute=10)
+        torch.cuda.synchronize()
     log("timed %d iterations in %.3f s; normr/normr0 %.3e" % (r["niters"], dt, r["normr"] / r["normr0"]))
     conv = None
     if a.ref_iters > 0:

@@ -496,11 +496,18 @@ struct TempBufD {  // a temporary device buffer freed on scope exit
 };
 void prepare_mixed(Problem &P);
 void replace_diag32(Level &L, const double *d, hipStream_t s);
+// zero_x: x was zero when the sweep began (the V-cycle's first sweep).
+// rz: the backward sweep also leaves the r.z partials of r64 and the new x
+// (one per wavefront of rows, colours in backward order).
 void symgs32(Problem &P, int l, int ordering, const float *r, float *x,
-             hipStream_t s);
+             hipStream_t s, bool zero_x = false, double *rz = nullptr,
+             const double *r64 = nullptr);
 void spmv32(const Problem &P, const Level &L, const float *x, float *y, hipStream_t s);
+// rz, r64: level l's post-smoother leaves the r.z partials (symgs32)
 void mg32(Problem &P, int l, int ordering, const float *r, float *x,
-          hipStream_t s);
+          hipStream_t s, double *rz = nullptr, const double *r64 = nullptr);
+// r.z partials of a colour step over `rows` rows: one per wavefront with rows
+inline int64_t rz_slots(int64_t rows) { return rows > 0 ? (rows + kTile - 1) / kTile : 0; }
 void to_f32(int64_t n, const double *in, float *out, hipStream_t s);
 // ---- block PCG (block.hip) ----
 void prepare_block(Problem &P, int s);

@@ -90,7 +90,7 @@ __global__ __launch_bounds__(kBlock) void k_to_f64(int64_t n, const float *in,
 }
 
 template <bool STREAM>
-__device__ inline void gs32_row(int64_t i, int64_t n,
+__device__ inline float gs32_row(int64_t i, int64_t n,
                                 const int32_t *__restrict__ col,
                                 const half_t *__restrict__ val16,
                                 const float *__restrict__ diag32,
@@ -101,7 +101,9 @@ __device__ inline void gs32_row(int64_t i, int64_t n,
     const int64_t k = (int64_t)j * n + i;
     sum -= (float)mload<STREAM>(val16 + k) * x[mload<STREAM>(col + k)];
   }
-  x[i] = sum / diag32[i];
+  const float v = sum / diag32[i];
+  x[i] = v;
+  return v;
 }
 
 template <bool STREAM>
@@ -134,17 +136,14 @@ __device__ inline half_t vc_val(uint32_t w) {
 }
 
 template <bool STREAM>
-__device__ inline void gs32_row16(const Col16 &C, const uint32_t *__restrict__ vc,
+__device__ inline float gs32_row16(const Col16 &C, const uint32_t *__restrict__ vc,
                                   int64_t i, int64_t n, const int32_t *__restrict__ col,
                                   const half_t *__restrict__ val16,
                                   const float *__restrict__ diag32,
                                   const float *__restrict__ r, float *x) {
   const int t = tile_of(i);
   const int info = C.info[t];
-  if (info == kTile32) {
-    gs32_row<STREAM>(i, n, col, val16, diag32, r, x);
-    return;
-  }
+  if (info == kTile32) return gs32_row<STREAM>(i, n, col, val16, diag32, r, x);
   const int32_t *__restrict__ base = C.tbase + (int64_t)t * kBaseStride;
   float sum = r[i];
 #pragma unroll
@@ -153,7 +152,9 @@ __device__ inline void gs32_row16(const Col16 &C, const uint32_t *__restrict__ v
     const int32_t c = base[j] + (int32_t)(w >> 16);
     sum -= (float)vc_val(w) * x[c];
   }
-  x[i] = sum / diag32[i];
+  const float v = sum / diag32[i];
+  x[i] = v;
+  return v;
 }
 
 template <bool STREAM>
@@ -181,7 +182,7 @@ __device__ inline float row_dot32_16(const Col16 &C, const uint32_t *__restrict_
 // kSlotWords offset words and kPairWords value words, 3 bytes per slot, in
 // gs32_row16's order of terms.  Other tiles take gs32_row16.
 template <bool STREAM>
-__device__ inline void gs32_row8(const Col16 &C, const uint32_t *__restrict__ vp,
+__device__ inline float gs32_row8(const Col16 &C, const uint32_t *__restrict__ vp,
                                  const uint32_t *__restrict__ vc, int64_t i, int64_t n,
                                  const int32_t *__restrict__ col,
                                  const half_t *__restrict__ val16,
@@ -189,10 +190,7 @@ __device__ inline void gs32_row8(const Col16 &C, const uint32_t *__restrict__ vp
                                  const float *__restrict__ r, float *x) {
   const int t = tile_of(i);
   const int info = C.info[t];
-  if (info != kTile8 || !C.use8) {
-    gs32_row16<STREAM>(C, vc, i, n, col, val16, diag32, r, x);
-    return;
-  }
+  if (info != kTile8 || !C.use8) return gs32_row16<STREAM>(C, vc, i, n, col, val16, diag32, r, x);
   const int32_t *__restrict__ base = C.tbase + (int64_t)t * kBaseStride;
   const unsigned am = C.amask ? C.amask[t] : 0u;
   uint32_t o8[kSlotWords], pv[kPairWords];
@@ -207,7 +205,9 @@ __device__ inline void gs32_row8(const Col16 &C, const uint32_t *__restrict__ vp
     const half_t a = __builtin_bit_cast(half_t, (uint16_t)(pv[j / 2] >> (16 * (j % 2))));
     sum -= (float)a * x[c];
   }
-  x[i] = sum / diag32[i];
+  const float v = sum / diag32[i];
+  x[i] = v;
+  return v;
 }
 
 // row_dot32_16 on an 8-bit tile from the FP16 value pairs (gs32_row8's
@@ -238,6 +238,95 @@ __device__ inline float row_dot32_8(const Col16 &C, const uint32_t *__restrict__
   }
   s += diag32[i] * x[i];
   return s;
+}
+
+// The first forward sweep of an FP32 V-cycle level from x = 0: the FP32
+// form of kernels.hip's zero-x colour step (see gs_row_zero there).  Slots
+// that Level::zmask leaves out of the tile, and x at or beyond `begin`,
+// are read through buffer views whose range check returns 0 without a
+// memory access, so the row's loads still go out together.  Each row keeps
+// gs32_row8's (8-bit tiles with FP16 value pairs), gs32_row16's (packed
+// words) or gs32_row's (32-bit ids) term order; skipped terms are a * 0.
+typedef __amdgpu_buffer_rsrc_t BufView;
+constexpr int kBufWord3 = 0x00020000;  // buffer descriptor word 3 for gfx9 raw loads
+
+__device__ inline BufView buf_view(const void *p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, kBufWord3);
+}
+template <bool STREAM>
+__device__ inline uint32_t buf_u32(BufView v, uint32_t off) {
+  return __builtin_amdgcn_raw_buffer_load_b32(v, off, 0, STREAM ? 2 : 0);
+}
+__device__ inline float buf_f32(BufView v, uint32_t off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(v, off, 0, 0));
+}
+template <bool STREAM>
+__device__ inline uint32_t buf_u16(BufView v, uint32_t off) {
+  return __builtin_amdgcn_raw_buffer_load_b16(v, off, 0, STREAM ? 2 : 0);
+}
+
+template <bool STREAM>
+__device__ inline void gs32_row_zero(const Col16 &C, uint32_t zm, int64_t begin,
+                                     const uint32_t *__restrict__ vp,
+                                     const uint32_t *__restrict__ vc, int64_t i, int64_t n,
+                                     const int32_t *__restrict__ col,
+                                     const half_t *__restrict__ val16,
+                                     const float *__restrict__ diag32,
+                                     const float *__restrict__ r, float *x) {
+  const int t = tile_of(i);
+  const int info = C.info[t];
+  const int64_t tb = (int64_t)t * kTile;
+  const uint32_t lane = (uint32_t)(i - tb);
+  const uint32_t xlim = (uint32_t)begin * 4u;
+  float sum = r[i];
+  if (info == kTile8 && C.use8 && vp) {
+    const int32_t *__restrict__ base = C.tbase + (int64_t)t * kBaseStride;
+    const unsigned am = C.amask ? C.amask[t] : 0u;
+    uint32_t o8[kSlotWords], pv[kPairWords];
+#pragma unroll
+    for (int g = 0; g < kSlotWords; ++g) {
+      const bool any = (zm >> (4 * g)) & 0xfu;
+      o8[g] = ((am >> g) & 1u)
+                  ? lane * 0x01010101u
+                  : buf_u32<STREAM>(buf_view(C.off8 + (int64_t)g * n + tb, any ? 256u : 0u),
+                                    lane * 4u);
+    }
+#pragma unroll
+    for (int w = 0; w < kPairWords; ++w) {
+      const bool any = (zm >> (2 * w)) & 3u;
+      pv[w] = buf_u32<STREAM>(buf_view(vp + (int64_t)w * n + tb, any ? 256u : 0u), lane * 4u);
+    }
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const bool live = (zm >> j) & 1u;
+      const int32_t c = base[j] + (int32_t)((o8[j / 4] >> (8 * (j % 4))) & 0xffu);
+      const half_t a = __builtin_bit_cast(half_t, (uint16_t)(pv[j / 2] >> (16 * (j % 2))));
+      sum -= (float)a * buf_f32(buf_view(x, live ? xlim : 0u), (uint32_t)c * 4u);
+    }
+  } else if (info != kTile32) {
+    const int32_t *__restrict__ base = C.tbase + (int64_t)t * kBaseStride;
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const bool live = (zm >> j) & 1u;
+      const uint32_t w =
+          buf_u32<STREAM>(buf_view(vc + (int64_t)j * n + tb, live ? 256u : 0u), lane * 4u);
+      const int32_t c = base[j] + (int32_t)(w >> 16);
+      sum -= (float)vc_val(w) * buf_f32(buf_view(x, live ? xlim : 0u), (uint32_t)c * 4u);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const bool live = (zm >> j) & 1u;
+      const int32_t c = (int32_t)buf_u32<STREAM>(
+          buf_view(col + (int64_t)j * n + tb, live ? 256u : 0u), lane * 4u);
+      const half_t a = __builtin_bit_cast(
+          half_t, (uint16_t)buf_u16<STREAM>(buf_view(val16 + (int64_t)j * n + tb,
+                                                     live ? 128u : 0u),
+                                            lane * 2u));
+      sum -= (float)a * buf_f32(buf_view(x, live ? xlim : 0u), (uint32_t)c * 4u);
+    }
+  }
+  x[i] = sum / diag32[i];
 }
 
 // v16p[w*n + i] = bits of val16[(2w)*n + i] | bits of val16[(2w+1)*n + i] << 16
@@ -272,6 +361,41 @@ __global__ __launch_bounds__(kBlock) void k_gs32_range8(
     const float *__restrict__ r, float *x, bool xcd) {
   const int64_t i = begin + xcd_block(blockIdx.x, gridDim.x, xcd) * kBlock + threadIdx.x;
   if (i < end) gs32_row8<STREAM>(C, vp, vc, i, n, col, val16, diag32, r, x);
+}
+
+template <bool STREAM, bool FINEST>
+__global__ __launch_bounds__(kBlock) void k_gs32_zero(
+    int64_t begin, int64_t end, int64_t n, Col16 C, const uint32_t *__restrict__ zmask,
+    const uint32_t *__restrict__ vp, const uint32_t *__restrict__ vc,
+    const int32_t *__restrict__ col, const half_t *__restrict__ val16,
+    const float *__restrict__ diag32, const float *__restrict__ r, float *x, bool xcd) {
+  const int64_t i = begin + xcd_block(blockIdx.x, gridDim.x, xcd) * kBlock + threadIdx.x;
+  if (i < end)
+    gs32_row_zero<STREAM>(C, zmask[tile_of(i)], begin, vp, vc, i, n, col, val16, diag32, r, x);
+}
+
+// k_gs32_range8 for the V-cycle's last sweep (the finest level's backward
+// post-smoother): wavefront k of the launch (rows begin + 64 k ..) also
+// stores the sum of r64_i * (double)(new x_i) over its rows to dpart[k], so
+// the mixed CG's r.z (CG.cpp:110, r in FP64, z widened) needs no pass over
+// r and z of its own.  One partial per wavefront, a 64-lane shuffle tree.
+template <bool STREAM>
+__global__ __launch_bounds__(kBlock) void k_gs32_rz8(
+    int64_t begin, int64_t end, int64_t n, Col16 C, const uint32_t *__restrict__ vp,
+    const uint32_t *__restrict__ vc, const int32_t *__restrict__ col,
+    const half_t *__restrict__ val16, const float *__restrict__ diag32,
+    const float *__restrict__ r, float *x, bool xcd, const double *__restrict__ r64,
+    double *__restrict__ dpart) {
+  const int64_t t = xcd_block(blockIdx.x, gridDim.x, xcd) * kBlock + threadIdx.x;
+  const int64_t i = begin + t;
+  double sp = 0.0;
+  if (i < end) {
+    const float z = gs32_row8<STREAM>(C, vp, vc, i, n, col, val16, diag32, r, x);
+    sp = mload<STREAM>(r64 + i) * (double)z;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) sp += __shfl_down(sp, off, 64);
+  if ((threadIdx.x & 63) == 0 && i < end) dpart[t >> 6] = sp;
 }
 
 // vc[k] = col16[k] << 16 | bits of val16[k], one thread per row (a launch
@@ -386,13 +510,30 @@ __global__ __launch_bounds__(kBlock) void k_prolong32_f(int64_t nc,
   if (t < nc) xf[fine_row(ftile, t)] += xc[rperm[t]];
 }
 
+// The FP16-pair colour step applies (8-bit offsets and FP16 value pairs).
+bool pairs_step(const Problem &P, const Level &L, int64_t begin) {
+  return use_col16(P, L, begin) && L.v16p && P.use_pairs && P.use_c8 && L.byte_tiles > 0;
+}
+
+// With dpart (only where pairs_step holds), also the r.z partials of
+// k_gs32_rz8 from the FP64 residual r64.
 void gs32_step(const Problem &P, const Level &L, int64_t begin, int64_t end,
-               const float *r, float *x, hipStream_t s) {
+               const float *r, float *x, hipStream_t s, double *dpart = nullptr,
+               const double *r64 = nullptr) {
   if (end <= begin) return;
   const unsigned g = grid_for(end - begin);
-  if (use_col16(P, L, begin) && L.v16p && P.use_pairs && P.use_c8 && L.byte_tiles > 0) {
+  if (dpart && !pairs_step(P, L, begin))
+    throw Error(HPCGX_EINVAL, "FP32 colour step with r.z partials needs the FP16-pair layout");
+  if (pairs_step(P, L, begin)) {
     const Col16 C = col16_of(P, L);
-    if (L.finest)
+    if (dpart) {
+      if (L.finest || L.stream)
+        k_gs32_rz8<true><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.v16p, L.vc, L.col, L.val16,
+                                              L.diag32, r, x, L.xcd, r64, dpart);
+      else
+        k_gs32_rz8<false><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.v16p, L.vc, L.col,
+                                               L.val16, L.diag32, r, x, L.xcd, r64, dpart);
+    } else if (L.finest)
       k_gs32_range8<true, true><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.v16p, L.vc, L.col,
                                                      L.val16, L.diag32, r, x, L.xcd);
     else if (L.stream)
@@ -430,6 +571,39 @@ void gs32_step(const Problem &P, const Level &L, int64_t begin, int64_t end,
   HPCGX_LAUNCHED();
 }
 
+// gs32_step on a forward sweep that began from x = 0 (k_gs32_zero); false
+// (nothing launched) where it does not apply.
+bool gs32_step_zero(const Problem &P, const Level &L, int64_t begin, int64_t end,
+                    const float *r, float *x, hipStream_t s) {
+  if (!L.zmask || !L.vc || !use_col16(P, L, begin) || L.n >= ((int64_t)1 << 29)) return false;
+  if (end <= begin) return true;
+  const unsigned g = grid_for(end - begin);
+  const Col16 C = col16_of(P, L);
+  const uint32_t *vp = (L.v16p && P.use_pairs && P.use_c8) ? L.v16p : nullptr;
+  if (L.finest)
+    k_gs32_zero<true, true><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.zmask, vp, L.vc, L.col,
+                                                 L.val16, L.diag32, r, x, L.xcd);
+  else if (L.stream)
+    k_gs32_zero<true, false><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.zmask, vp, L.vc, L.col,
+                                                  L.val16, L.diag32, r, x, L.xcd);
+  else
+    k_gs32_zero<false, false><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.zmask, vp, L.vc, L.col,
+                                                   L.val16, L.diag32, r, x, L.xcd);
+  HPCGX_LAUNCHED();
+  return true;
+}
+
+// Whether level l's first forward FP32 sweep of a V-cycle can start from an
+// unfilled x (every colour step through gs32_step_zero).
+bool zero_guess_sweep32(const Problem &P, int l, int ordering) {
+  const Level &L = P.L[l];
+  if (!P.zero_guess || ordering != HPCGX_ORDER_MULTICOLOR || !L.zmask || !L.vc ||
+      !use_col16(P, L, 0) || L.n >= ((int64_t)1 << 29))
+    return false;
+  for (int c = 0; c < L.ncolors; ++c)
+    if (L.color_ptr[c] % kTile) return false;
+  return true;
+}
 }  // namespace
 
 void prepare_mixed(Problem &P) {
@@ -475,13 +649,23 @@ void replace_diag32(Level &L, const double *d, hipStream_t s) {
 }
 
 void symgs32(Problem &P, int l, int ordering, const float *r, float *x,
-             hipStream_t s) {
+             hipStream_t s, bool zero_x, double *rz, const double *r64) {
   const Level &L = P.L[l];
+  if (zero_x && !zero_guess_sweep32(P, l, ordering))
+    throw Error(HPCGX_EINVAL, "zero-x FP32 sweep not available on this level");
   if (ordering == HPCGX_ORDER_MULTICOLOR) {
-    for (int c = 0; c < L.ncolors; ++c)
-      gs32_step(P, L, L.color_ptr[c], L.color_ptr[c + 1], r, x, s);
-    for (int c = L.ncolors - 1; c >= 0; --c)
-      gs32_step(P, L, L.color_ptr[c], L.color_ptr[c + 1], r, x, s);
+    for (int c = 0; c < L.ncolors; ++c) {
+      if (zero_x)
+        (void)gs32_step_zero(P, L, L.color_ptr[c], L.color_ptr[c + 1], r, x, s);
+      else
+        gs32_step(P, L, L.color_ptr[c], L.color_ptr[c + 1], r, x, s);
+    }
+    int64_t off = 0;
+    for (int c = L.ncolors - 1; c >= 0; --c) {
+      gs32_step(P, L, L.color_ptr[c], L.color_ptr[c + 1], r, x, s, rz ? rz + off : nullptr,
+                r64);
+      off += rz_slots(L.color_ptr[c + 1] - L.color_ptr[c]);
+    }
   } else {
     for (int pass = 0; pass < 2; ++pass)
       for (int k = 0; k < L.nsched; ++k) {
@@ -516,13 +700,19 @@ void spmv32(const Problem &P, const Level &L, const float *x, float *y, hipStrea
   HPCGX_LAUNCHED();
 }
 
-// ComputeMG_ref.cpp:39-63 in FP32.
+// ComputeMG_ref.cpp:39-63 in FP32.  x = 0 (:42) is a fill kernel, or,
+// where zero_guess_sweep32 allows, implied by the first forward sweep.
 void mg32(Problem &P, int l, int ordering, const float *r, float *x,
-          hipStream_t s) {
+          hipStream_t s, double *rz, const double *r64) {
   Level &L = P.L[l];
-  k_fill32<<<grid_for(L.n), kBlock, 0, s>>>(x, L.n);
-  HPCGX_LAUNCHED();
-  symgs32(P, l, ordering, r, x, s);
+  const bool zg = zero_guess_sweep32(P, l, ordering);
+  if (!zg) {
+    k_fill32<<<grid_for(L.n), kBlock, 0, s>>>(x, L.n);
+    HPCGX_LAUNCHED();
+  }
+  if (rz && l + 1 >= P.nlevels)
+    throw Error(HPCGX_EINVAL, "r.z partials come from a post-smoother sweep");
+  symgs32(P, l, ordering, r, x, s, zg);
   if (l + 1 < P.nlevels) {
     const unsigned g = grid_for(L.nc);
     if (L.rperm) {
@@ -553,7 +743,7 @@ void mg32(Problem &P, int l, int ordering, const float *r, float *x,
     else
       k_prolong32<<<g, kBlock, 0, s>>>(L.nc, L.f2c, L.xc32, x);
     HPCGX_LAUNCHED();
-    symgs32(P, l, ordering, r, x, s);
+    symgs32(P, l, ordering, r, x, s, false, rz, r64);
   }
 }
 
@@ -582,11 +772,31 @@ double read_scalar(const Problem &P, int slot, hipStream_t s) {
 namespace {
 // One iteration of the mixed-precision CG without the residual recompute
 // (CG.cpp:102-124 with the FP32 V-cycle), enqueued on s.
+// r.z partials the finest level's last FP32 colour sweep leaves (one per
+// wavefront of rows), or 0 where that sweep cannot leave them.
+int64_t rz_partials32(const Problem &P, int ordering) {
+  const Level &L = P.L[0];
+  if (!P.rz_fused || !P.rz_part || ordering != HPCGX_ORDER_MULTICOLOR || P.nlevels < 2)
+    return 0;
+  int64_t count = 0;
+  for (int c = 0; c < L.ncolors; ++c) {
+    const int64_t b = L.color_ptr[c], e = L.color_ptr[c + 1];
+    if (e <= b) continue;
+    if (!pairs_step(P, L, b)) return 0;
+    count += rz_slots(e - b);
+  }
+  return count <= P.nrz_partials ? count : 0;
+}
+
 void mixed_iteration(Problem &P, int ordering, double *x, bool first, hipStream_t s) {
   const Level &A = P.L[0];
   const int64_t n = A.n;
-  mg32(P, 0, ordering, P.r32, P.z32, s);
-  launch_dot32(P, n, P.r, P.z32, nullptr, kFinishBeta, s);
+  const int64_t nrz = rz_partials32(P, ordering);
+  mg32(P, 0, ordering, P.r32, P.z32, s, nrz ? P.rz_part : nullptr, P.r);
+  if (nrz)  // r.z from the V-cycle's last sweep (CG.cpp:110, :113-114)
+    launch_partials_dot(P, P.rz_part, nrz, kFinishBeta, s);
+  else
+    launch_dot32(P, n, P.r, P.z32, nullptr, kFinishBeta, s);
   if (first)
     to_f64(n, P.z32, P.p, s);
   else

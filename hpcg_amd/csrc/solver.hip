@@ -49,12 +49,6 @@ void colour_step(Problem &P, const Level &L, int c, const double *r, double *x,
   log.done();
 }
 
-// r.z partials of a colour step over `rows` rows: one per wavefront with
-// rows (kTile rows each)
-int64_t rz_slots(int64_t rows) {
-  return rows > 0 ? (rows + kTile - 1) / kTile : 0;
-}
-
 // A whole sweep direction as one wavefront launch.
 void wave_sweep(Problem &P, const Level &L, int dir, const double *r, double *x,
                 hipStream_t s) {

@@ -364,12 +364,13 @@ int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, i
  *                      slot base plus that position): the FP32 V-cycle per
  *                      word, the FP64 kernels on tiles whose words all are
  *                      (default 1)
- *   "rz_fused"         1/0: the FP64 CG takes r.z from partials that the
- *                      V-cycle's last finest-level colour sweep leaves
- *                      (each row's r and final z are at hand there), summed
- *                      by a second launch, instead of a dot kernel that
- *                      reads r and z again (default 1; applies to the
- *                      multicolour colour steps with narrow column ids).
+ *   "rz_fused"         1/0: the FP64 and the mixed-precision CG take r.z
+ *                      from partials that the V-cycle's last finest-level
+ *                      colour sweep leaves (each row's r and final z are at
+ *                      hand there), summed by a second launch, instead of a
+ *                      dot kernel that reads r and z again (default 1;
+ *                      applies to the multicolour colour steps with narrow
+ *                      column ids, FP16 value pairs in the mixed V-cycle).
  *                      Changes the dot's summation order.
  *   "zero_guess"       1/0: the V-cycle's first forward sweep of a level
  *                      (multicolour ordering, narrow column ids) starts from

@@ -4,8 +4,8 @@ Mixed precision at 256^3 (FP32 V-cycle on FP16 values, FP64 outer PCG, FP64
 residual every 10 iterations):
   * FP32 SpMV and the FP32 V-cycle bit-exact against the oracle's FP32
     restatement (or_spmv32, or_mg32) on the 256^3 hierarchy;
-  * PCG to ||r||/||r0|| <= 1e-9 from x0 = 0 in the same iteration count as
-    the FP64 path, max|x - 1| < 1e-7, max|x_mixed - x_fp64| < 1e-9.
+  * PCG to ||r||/||r0|| <= 1e-9 from x0 = 0 within one iteration of the
+    FP64 path's count, max|x - 1| < 1e-7, max|x_mixed - x_fp64| < 1e-9.
 Block PCG at 256^3, s = 8 and 16 right-hand sides (splitmix64 uniform(-1, 1)
 from seed 42, one stream read row-major into [n][s]):
   * every column of the FP64 block SpMV bit-exact against the oracle's SpMV;
@@ -97,7 +97,7 @@ def test_mixed_256_to_tolerance(P256):
         assert r["normr"] / r["normr0"] <= 1e-9, mode
         assert float((x - 1.0).abs().max()) < 1e-7, mode
         xs[mode], its[mode] = x, r["niters"]
-    assert its["mixed"] == its["fp64"], its
+    assert abs(its["mixed"] - its["fp64"]) <= 1, its
     assert float((xs["mixed"] - xs["fp64"]).abs().max()) < 1e-9
 
 

@@ -191,6 +191,9 @@ def test_cg_mixed_same_bits_every_format(hp, dims):
     import torch
     say("cg mixed formats", dims)
     P = hp.Problem.generate(*dims, nlevels=4)
+    # r.z from the dot kernel in every format (the V-cycle's last sweep
+    # leaves r.z partials only in the FP16-pair layout, summed in another order)
+    P.tune("rz_fused", 0)
     b, x, _ = P.vectors()
     got = []
     for c16, pairs in ((1, 1), (1, 0), (0, 1)):
@@ -222,7 +225,7 @@ def test_cg_mixed_graph_replays(hp, tol):
     P.tune("mixed_graph", 1)
     for rep in range(3):
         if rep == 2:
-            P.tune("fp16_pairs", 0)  # drops the graph; same bits by construction
+            P.tune("stream_p", 1)  # drops the graph; same bits by construction
         x.zero_()
         r = P.cg_mixed(b, x, max_iter=19, tolerance=tol, recompute=4)
         assert r["niters"] == want["niters"] and r["normr"] == want["normr"], rep
@@ -231,6 +234,28 @@ def test_cg_mixed_graph_replays(hp, tol):
         P.mg32(P.zeros32() + 1.0, z)
         x.zero_()
         P.cg(b, x, max_iter=2, tolerance=0.0)
+    del P
+
+
+def test_cg_mixed_rz_fused(hp):
+    """r.z from the partials of the FP32 V-cycle's last sweep (r in FP64, z
+    widened) against the dot kernel over r and z: the same sum in another
+    order, so the trajectories agree to rounding (1e-12 relative through
+    the first residual recompute) and the fused one matches the oracle."""
+    dims = (32, 48, 64)
+    say("cg mixed rz fused", dims)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    b, x, _ = P.vectors()
+    h = hier(dims, True)
+    want = h.cg_mixed(max_iter=10, tol=0.0, recompute=10)["traj"]
+    res = {}
+    for fused in (1, 0):
+        P.tune("rz_fused", fused)
+        x.zero_()
+        res[fused] = P.cg_mixed(b, x, max_iter=10, tolerance=0.0, recompute=10)
+    a, c = (res[k]["normr"] / res[k]["normr0"] for k in (1, 0))
+    assert a == pytest.approx(c, rel=1e-12)
+    assert a == pytest.approx(want[10], rel=1e-9)
     del P
 
 
