@@ -813,3 +813,49 @@ def test_256_properties(hp, torch):
     assert r["niters"] == 50 and r["normr"] / r["normr0"] < 2e-2
     assert r["normr"] < r25["normr"]
     del P
+
+
+# (knob, value, default): each changes how the iteration is scheduled or
+# shaped, never its arithmetic
+SAME_BITS_KNOBS = [("zero_guess", 0, 1), ("defer_x", 0, 1), ("gs_block", 64, 256),
+                   ("gs_block", 128, 256), ("gs_block", 512, 256), ("gs_waves", 0, 1)]
+
+
+def test_schedule_knobs_same_bits(hp, torch):
+    """The first forward sweep from the zero x without a fill (zero_guess),
+    x's step taken with the next p update (defer_x), and the finest colour
+    steps' workgroup size and register cap (gs_block, gs_waves) change no
+    bit: the FP64 and FP32 V-cycles equal the oracle's, and 15 CG iterations
+    give the defaults' x and residual.  Colours of 6720 rows end in partial
+    workgroups of every size."""
+    dims = (24, 40, 56)
+    say("schedule knobs", dims)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    P.prepare_mixed()
+    h = hier(dims, color=True)
+    rr = rnd(91, P.nrow(0))
+    want_z = h.mg(rr)
+    r32 = P.to_device_order(rr).float()
+    z32_want = P.zeros32()
+    P.mg32(r32, z32_want)
+    assert np.array_equal(P.to_numpy(z32_want.double()).astype(np.float32),
+                          h.mg32(rr.astype(np.float32)))
+    b, x, _ = P.vectors()
+    r0 = P.cg(b, x, max_iter=15, tolerance=0.0)
+    x0 = x.clone()
+    for knob, value, default in SAME_BITS_KNOBS:
+        say("knob", knob, value)
+        P.tune(knob, value)
+        try:
+            z = P.zeros()
+            P.mg(P.to_device_order(rr), z, ordering="multicolor")
+            assert np.array_equal(P.to_numpy(z), want_z), knob
+            z32 = P.zeros32()
+            P.mg32(r32, z32)
+            assert torch.equal(z32, z32_want), knob
+            x.zero_()
+            r = P.cg(b, x, max_iter=15, tolerance=0.0)
+            assert r["normr"] == r0["normr"] and torch.equal(x, x0), (knob, value)
+        finally:
+            P.tune(knob, default)
+    del P
