@@ -72,6 +72,7 @@ SIGNATURES = {
     "hpcgx_prolongation": (I, [P, I, P, P, P]),
     "hpcgx_dot": (I, [P, I64, P, P, P, P]),
     "hpcgx_dot_host": (I, [P, I64, P, P, P, P]),
+    "hpcgx_dot_noalloc": (I, [I64, P, P, P, P]),
     "hpcgx_waxpby": (I, [I64, D, P, D, P, P, P]),
     "hpcgx_cg": (I, [P, I, P, P, I, D, I, I, P, P, P, P, P]),
     "hpcgx_prepare_mixed": (I, [P]),

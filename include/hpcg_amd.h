@@ -222,6 +222,20 @@ int hpcgx_dot(hpcgx_problem *p, int64_t n, const double *x, const double *y,
 int hpcgx_dot_host(hpcgx_problem *p, int64_t n, const double *x,
                    const double *y, double *result, void *stream);
 
+/* No counterpart in the reference: hpcgx_dot for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc), with no problem handle.  It runs
+ * hpcgx_dot's kernels, so *result_dev is hpcgx_dot's value bit for bit; the
+ * reduction scratch (4096 partials and a ticket, 32.25 KiB) is a static
+ * device array that each GPU holds until the process ends.  Every n >= 0
+ * fits it (the reduction launches at most 4096 workgroups); n < 0 gives
+ * HPCGX_EINVAL.  It calls no hipMalloc or hipFree; the first call on a
+ * device looks up the array's address there (make that call before any
+ * capture).  All calls on one GPU share its array: calls that overlap on one
+ * GPU race. */
+int hpcgx_dot_noalloc(int64_t n, const double *x, const double *y,
+                      double *result_dev, void *stream);
+
 /* replaces ComputeWAXPBY (ComputeWAXPBY.cpp:42-48 -> ComputeWAXPBY_ref.cpp:
  * 42-70): w = alpha x + beta y with the reference's alpha==1 and beta==1
  * forms; w may alias x or y. */
