@@ -233,6 +233,10 @@ struct Problem {
   bool defer_x = true;        // CG takes x's step with the next p update ("defer_x")
   int gs_block = kBlock;      // workgroup size of the finest colour steps ("gs_block")
   int gs_waves = 1;           // 1: register caps on the finest colour steps ("gs_waves")
+  // finest colour steps and SpMV: on affine tiles, slot pairs whose x windows
+  // are one element apart share one gather ("x_pairs"; bit 0: the colour
+  // steps without r.z and the SpMV, bit 1: the r.z colour step)
+  int x_pairs = 3;
   int spmv_dot_blocks = 4096; // persistent grid of the fused SpMV + dot
   // The fused SpMV + dot with one workgroup per row chunk of the fine level
   // ("spmv_dot_chunks"): the p.Ap partial of each chunk, summed in fixed

@@ -135,11 +135,11 @@ __device__ inline double row_dot(const int32_t *__restrict__ col,
 // SpMV over row chunks (<= kBlock rows each).  The chunk list interleaves the
 // colour blocks (chunk k of every colour before chunk k+1 of any), so the x
 // values gathered from all colour blocks stay within a narrow window.
-template <bool STREAM>
+template <bool STREAM, bool PAIRS = false>
 __device__ inline double row_dot16(const Col16 &C, const int32_t *__restrict__ col,
                                    const double *__restrict__ val,
                                    const double *__restrict__ x, int64_t n,
-                                   int64_t i);
+                                   int64_t i, int64_t end = 0);
 
 // C16: 16-bit column ids (chunks start at multiples of kTile).
 template <bool STREAM, bool C16>
@@ -220,36 +220,94 @@ __device__ inline void narrow_row(const Col16 &C, int t, int info, int64_t n, in
   }
 }
 
-template <bool STREAM>
+// On a tile whose offset words are all affine (slot j's column is its base
+// plus the row's position in the tile), slots 3m and 3m + 2 whose bases
+// differ by one read x windows one element apart: on the 27-point grid the
+// x - 1 and x + 1 neighbours, consecutive rows of one colour block.  Where
+// that holds for all nine m, slot 3m + 2's x values are slot 3m's moved down
+// one lane (ds_bpermute), and lane 63's is read on its own (one address for
+// the whole wavefront), in place of nine 64-lane gathers.  Needs all 64
+// lanes of the tile active (a full tile).  f(j, k, c, x_c) for every slot j
+// in order.
+template <class F>
+__device__ inline void affine_pairs_row(const Col16 &C, int t, int64_t n, int64_t i,
+                                        const double *x, F &&f) {
+  const int32_t *__restrict__ base = C.base + t;
+  const int32_t pos = (int32_t)(i % kTile);
+#pragma unroll
+  for (int m = 0; m < kSlots / 3; ++m) {
+    const int32_t b0 = base[(int64_t)(3 * m) * C.ntiles];
+    const int32_t b1 = base[(int64_t)(3 * m + 1) * C.ntiles];
+    const double x0 = x[b0 + pos];
+    const double x1 = x[b1 + pos];
+    const double up = __shfl_down(x0, 1, 64);
+    const double last = x[b0 + kTile];
+    const double x2 = pos == kTile - 1 ? last : up;
+    f(3 * m, (int64_t)(3 * m) * n + i, b0 + pos, x0);
+    f(3 * m + 1, (int64_t)(3 * m + 1) * n + i, b1 + pos, x1);
+    f(3 * m + 2, (int64_t)(3 * m + 2) * n + i, b0 + 1 + pos, x2);
+  }
+}
+
+// Whether the wavefront of row i can take affine_pairs_row: an affine
+// 8-bit tile all of whose 64 rows lie below `end`, with slot 3m + 2's base
+// one above slot 3m's for every m (uniform per wavefront).
+__device__ inline bool pairs_tile(const Col16 &C, int t, int info, int64_t end) {
+  if (!(info == kTile8 && C.use8 && C.amask && C.amask[t] == (1u << kSlotWords) - 1u &&
+        (int64_t)(t + 1) * kTile <= end))
+    return false;
+  const int32_t *__restrict__ base = C.base + t;
+  bool ok = true;
+#pragma unroll
+  for (int m = 0; m < kSlots / 3; ++m)
+    ok = ok && base[(int64_t)(3 * m + 2) * C.ntiles] == base[(int64_t)(3 * m) * C.ntiles] + 1;
+  return ok;
+}
+
+template <bool STREAM, bool PAIRS>
 __device__ inline double row_dot16(const Col16 &C, const int32_t *__restrict__ col,
                                    const double *__restrict__ val,
                                    const double *__restrict__ x, int64_t n,
-                                   int64_t i) {
+                                   int64_t i, int64_t end) {
   const int t = tile_of(i);
   const int info = C.info[t];
   if (info == kTile32) return row_dot<STREAM>(col, val, x, n, i);
   double sum = 0.0;
+  if (PAIRS && pairs_tile(C, t, info, end)) {
+    affine_pairs_row(C, t, n, i, x, [&](int, int64_t k, int32_t, double xc) {
+      sum += mload<STREAM>(val + k) * xc;
+    });
+    return sum;
+  }
   narrow_row<STREAM>(C, t, info, n, i, [&](int, int64_t k, int32_t c) {
     sum += mload<STREAM>(val + k) * x[c];
   });
   return sum;
 }
 
-template <bool STREAM>
+template <bool STREAM, bool PAIRS = false>
 __device__ inline double gs_row16(const Col16 &C, int64_t i, int64_t n,
                                   const int32_t *__restrict__ col,
                                   const double *__restrict__ val,
-                                  const double *__restrict__ r, double *x) {
+                                  const double *__restrict__ r, double *x, int64_t end = 0) {
   const int t = tile_of(i);
   const int info = C.info[t];
   if (info == kTile32) return gs_row<STREAM>(i, n, col, val, r, x);
   double sum = r[i];
   double d = 0.0;
-  narrow_row<STREAM>(C, t, info, n, i, [&](int, int64_t k, int32_t c) {
-    const double v = mload<STREAM>(val + k);
-    sum -= v * x[c];
-    if (c == i && v != 0.0) d = v;
-  });
+  if (PAIRS && pairs_tile(C, t, info, end)) {
+    affine_pairs_row(C, t, n, i, x, [&](int, int64_t k, int32_t c, double xc) {
+      const double v = mload<STREAM>(val + k);
+      sum -= v * xc;
+      if (c == i && v != 0.0) d = v;
+    });
+  } else {
+    narrow_row<STREAM>(C, t, info, n, i, [&](int, int64_t k, int32_t c) {
+      const double v = mload<STREAM>(val + k);
+      sum -= v * x[c];
+      if (c == i && v != 0.0) d = v;
+    });
+  }
   sum += x[i] * d;
   x[i] = sum / d;
   return sum / d;
@@ -274,12 +332,12 @@ __global__ __launch_bounds__(kBlock) void k_gs_range(
 // (7 caps it at 72 VGPRs, where its own choice, 77, allows 6 waves per
 // SIMD; 1 leaves it free).  The finest level's workgroup size and cap are
 // tuning knobs ("gs_block", "gs_waves"); other levels use kBlock and 7.
-template <bool STREAM, bool FINEST, int BS, int W>
+template <bool STREAM, bool FINEST, int BS, int W, bool PAIRS = false>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_gs_range16(
     int64_t begin, int64_t end, int64_t n, Col16 C, const int32_t *__restrict__ col,
     const double *__restrict__ val, const double *__restrict__ r, double *x, bool xcd) {
   const int64_t i = begin + xcd_block(blockIdx.x, gridDim.x, xcd) * BS + threadIdx.x;
-  if (i < end) gs_row16<STREAM>(C, i, n, col, val, r, x);
+  if (i < end) gs_row16<STREAM, PAIRS>(C, i, n, col, val, r, x, end);
 }
 
 // k_gs_range16 that also stores the sum of r_i * (new x_i) over the rows of
@@ -288,21 +346,25 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_
 // V-cycle every row is updated once, so these partials hold r.z (CG.cpp:110)
 // without reading r and z again.  One partial per wavefront: a 64-lane
 // shuffle tree, with no LDS and no workgroup barrier (a per-workgroup sum
-// cost 40 us per launch at 512x512x640).  Under W = 7 the compiler fits 70
-// VGPRs (7 waves per SIMD, as k_gs_range16 runs); uncapped it takes 120
-// (4 waves).
-template <bool STREAM, int BS, int W>
+// cost 40 us per launch at 512x512x640).  Under W = 6 the compiler fits 71
+// VGPRs, which still run 7 waves per SIMD as k_gs_range16 does, and keeps
+// k_gs_range16's memory waits; under W = 7 (70 VGPRs) it waited more often
+// and the step took 12% longer.  Uncapped it takes 120 (4 waves).
+template <bool STREAM, int BS, int W, bool PAIRS = false>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_gs_rz16(
     int64_t begin, int64_t end, int64_t n, Col16 C, const int32_t *__restrict__ col,
     const double *__restrict__ val, const double *__restrict__ r, double *x, bool xcd,
     double *__restrict__ dpart) {
-  const int64_t t = xcd_block(blockIdx.x, gridDim.x, xcd) * BS + threadIdx.x;
-  const int64_t i = begin + t;
+  const int64_t b = xcd_block(blockIdx.x, gridDim.x, xcd);
+  const int64_t i = begin + b * BS + threadIdx.x;
   double s = 0.0;
-  if (i < end) s = r[i] * gs_row16<STREAM>(C, i, n, col, val, r, x);
+  if (i < end) s = r[i] * gs_row16<STREAM, PAIRS>(C, i, n, col, val, r, x, end);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  if ((threadIdx.x & 63) == 0 && i < end) dpart[t >> 6] = s;
+  // the wavefront's index in the launch, formed after the row so that it
+  // holds no register during it
+  const int64_t w = b * (BS / 64) + (threadIdx.x >> 6);
+  if ((threadIdx.x & 63) == 0 && begin + w * 64 < end) dpart[w] = s;
 }
 
 // The first forward sweep of a V-cycle level from x = 0 (ComputeMG_ref.cpp:
@@ -344,7 +406,10 @@ __device__ inline uint32_t buf_u16(BufView v, uint32_t off) {
 }
 
 // Row i (lane `lane` of tile t, whose first row is tb) of a zero-x colour
-// step; zm is the tile's Level::zmask.  n * 8 < 2^32 (x offsets).
+// step; zm is the tile's Level::zmask.  n * 8 < 2^32 (x offsets).  The
+// tile's class (32-bit ids, all-affine 8-bit offsets, 8-bit, 16-bit) is
+// decided once, outside the slot loops, so each class's loop is straight
+// code whose loads the compiler can issue together.
 template <bool STREAM>
 __device__ inline void gs_row_zero(const Col16 &C, uint32_t zm, int64_t begin, int64_t i,
                                    int64_t n, const int32_t *__restrict__ col,
@@ -357,34 +422,49 @@ __device__ inline void gs_row_zero(const Col16 &C, uint32_t zm, int64_t begin, i
   const uint32_t lane = (uint32_t)(i - tb);
   const uint32_t xlim = (uint32_t)begin * 8u;
   const int32_t *__restrict__ base = C.base + t;
-  const bool affine = info == kTile8 && C.use8 && C.amask && C.amask[t] == (1u << kSlotWords) - 1u;
   double sum = r[i];
-  uint32_t w = 0;
-#pragma unroll
-  for (int j = 0; j < kSlots; ++j) {
+  // slot j's term, its value and x read through views that are empty for a
+  // dead slot and end at `begin` for x
+  auto term = [&](int j, int32_t c) {
     const bool live = (zm >> j) & 1u;
-    int32_t c;
-    if (info == kTile32) {
-      c = (int32_t)buf_u32<STREAM>(buf_view(col + (int64_t)j * n + tb, live ? 256u : 0u),
-                                   lane * 4u);
-    } else if (affine) {
-      c = base[(int64_t)j * C.ntiles] + (int32_t)lane;
-    } else if (info == kTile8 && C.use8) {
-      if (j % 4 == 0) {
-        const bool any = (zm >> j) & 0xfu;
-        w = buf_u32<STREAM>(buf_view(C.off8 + (int64_t)(j / 4) * n + tb, any ? 256u : 0u),
-                            lane * 4u);
-      }
-      c = base[(int64_t)j * C.ntiles] + (int32_t)((w >> (8 * (j % 4))) & 0xffu);
-    } else {
-      c = base[(int64_t)j * C.ntiles] +
-          (int32_t)buf_u16<STREAM>(buf_view(C.off + (int64_t)j * n + tb, live ? 128u : 0u),
-                                   lane * 2u);
-    }
-    const double v = buf_f64<STREAM>(buf_view(val + (int64_t)j * n + tb, live ? 512u : 0u),
-                                     lane * 8u);
+    const double v =
+        buf_f64<STREAM>(buf_view(val + (int64_t)j * n + tb, live ? 512u : 0u), lane * 8u);
     const double xc = buf_f64<false>(buf_view(x, live ? xlim : 0u), (uint32_t)c * 8u);
     sum -= v * xc;
+  };
+  if (info == kTile32) {
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const bool live = (zm >> j) & 1u;
+      term(j, (int32_t)buf_u32<STREAM>(buf_view(col + (int64_t)j * n + tb, live ? 256u : 0u),
+                                       lane * 4u));
+    }
+  } else if (info == kTile8 && C.use8 && C.amask &&
+             C.amask[t] == (1u << kSlotWords) - 1u) {
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) term(j, base[(int64_t)j * C.ntiles] + (int32_t)lane);
+  } else if (info == kTile8 && C.use8) {
+#pragma unroll
+    for (int g = 0; g < kSlotWords; ++g) {
+      const bool any = (zm >> (4 * g)) & 0xfu;
+      const uint32_t w = buf_u32<STREAM>(
+          buf_view(C.off8 + (int64_t)g * n + tb, any ? 256u : 0u), lane * 4u);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int j = 4 * g + b;
+        if (j < kSlots)
+          term(j, base[(int64_t)j * C.ntiles] + (int32_t)((w >> (8 * b)) & 0xffu));
+      }
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const bool live = (zm >> j) & 1u;
+      term(j, base[(int64_t)j * C.ntiles] +
+                  (int32_t)buf_u16<STREAM>(buf_view(C.off + (int64_t)j * n + tb,
+                                                    live ? 128u : 0u),
+                                           lane * 2u));
+    }
   }
   const double d = mload<STREAM>(diag + i);
   sum += 0.0 * d;
@@ -578,7 +658,9 @@ __global__ __launch_bounds__(kBlock) void k_resid_restrict(
 // The same in fine order (Level::ftile, rperm): thread t computes fine row
 // fine_row(ftile, t), so a wavefront reads whole tiles of the matrix (narrow
 // column ids when C16), and stores the coarse row rperm[t].
-template <bool STREAM, bool C16>
+// PAIRS: row_dot16's x-pair path on wavefronts whose 64 transfer rows are
+// all present (each is then a whole fine tile).
+template <bool STREAM, bool C16, bool PAIRS = false>
 __global__ __launch_bounds__(kBlock) void k_resid_restrict_f(
     int64_t nc, const int32_t *__restrict__ ftile, const int32_t *__restrict__ rperm, int64_t n,
     Col16 C, const int32_t *__restrict__ col, const double *__restrict__ val,
@@ -586,7 +668,8 @@ __global__ __launch_bounds__(kBlock) void k_resid_restrict_f(
   const int64_t t = gtid();
   if (t >= nc) return;
   const int64_t f = fine_row(ftile, t);
-  const double ax = C16 ? row_dot16<STREAM>(C, col, val, xf, n, f)
+  const int64_t end = (t / kTile + 1) * kTile <= nc ? n : 0;
+  const double ax = C16 ? row_dot16<STREAM, PAIRS>(C, col, val, xf, n, f, end)
                         : row_dot<STREAM>(col, val, xf, n, f);
   rc[rperm[t]] = rf[f] - ax;
 }
@@ -747,16 +830,17 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(
 // k_spmv_dot with one workgroup per row chunk (logical chunk b of the
 // launch's workgroup, XCD-remapped when xcd): chunk b's p.Ap partial goes to
 // cpart[b] for k_chunk_dot.
-template <bool STREAM, bool C16>
+template <bool STREAM, bool C16, bool PAIRS = false>
 __global__ __launch_bounds__(kBlock) void k_spmv_dot_c(
     const int64_t *__restrict__ chunks, int64_t n, Col16 C, const int32_t *__restrict__ col,
     const double *__restrict__ val, const double *__restrict__ p, double *__restrict__ Ap,
     double *__restrict__ cpart, bool xcd) {
   const int64_t b = xcd_block(blockIdx.x, gridDim.x, xcd);
   const int64_t i = chunks[2 * b] + threadIdx.x;
+  const int64_t end = chunks[2 * b + 1];
   double s = 0.0;
-  if (i < chunks[2 * b + 1]) {
-    const double y = C16 ? row_dot16<STREAM>(C, col, val, p, n, i)
+  if (i < end) {
+    const double y = C16 ? row_dot16<STREAM, PAIRS>(C, col, val, p, n, i, end)
                          : row_dot<STREAM>(col, val, p, n, i);
     Ap[i] = y;
     s = p[i] * y;
@@ -1027,13 +1111,26 @@ void launch_symgs_rows(const Problem &P, const Level &L, int64_t begin, int64_t 
   if (use_col16(P, L, begin)) {
     const Col16 C = col16_of(P, L);
     const int64_t rows = end - begin;
+    // the x-pair colour steps (default shape only): bit 0 of x_pairs for the
+    // steps without r.z, bit 1 for the r.z step
+    if (L.finest && (P.x_pairs & (dpart ? 2 : 1)) && P.gs_block == kBlock && P.gs_waves) {
+      const unsigned g = (unsigned)((rows + kBlock - 1) / kBlock);
+      if (dpart)
+        k_gs_rz16<true, kBlock, 6, true><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col, L.val,
+                                                               r, x, L.xcd, dpart);
+      else
+        k_gs_range16<true, true, kBlock, 7, true><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col,
+                                                                        L.val, r, x, L.xcd);
+      HPCGX_LAUNCHED();
+      return;
+    }
     if (L.finest) {  // tuned workgroup size and register cap
       const bool cap = P.gs_waves != 0;
 #define HPCGX_GS(BS)                                                                           \
   do {                                                                                         \
     const unsigned g = (unsigned)((rows + BS - 1) / BS);                                       \
     if (dpart && cap)                                                                          \
-      k_gs_rz16<true, BS, 7><<<g, BS, 0, s>>>(begin, end, L.n, C, L.col, L.val, r, x, L.xcd, \
+      k_gs_rz16<true, BS, 6><<<g, BS, 0, s>>>(begin, end, L.n, C, L.col, L.val, r, x, L.xcd, \
                                               dpart);                                          \
     else if (dpart)                                                                            \
       k_gs_rz16<true, BS, 1><<<g, BS, 0, s>>>(begin, end, L.n, C, L.col, L.val, r, x, L.xcd, \
@@ -1057,7 +1154,7 @@ void launch_symgs_rows(const Problem &P, const Level &L, int64_t begin, int64_t 
     }
     const unsigned g = grid_for(rows);
     if (dpart)
-      k_gs_rz16<true, kBlock, 7><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col, L.val, r, x,
+      k_gs_rz16<true, kBlock, 6><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col, L.val, r, x,
                                                       L.xcd, dpart);
     else if (L.stream)
       k_gs_range16<true, false, kBlock, 7><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col,
@@ -1141,7 +1238,10 @@ void launch_residual_restriction(const Problem &P, const Level &L, const double 
     const Col16 C = col16_of(P, L);
     const bool c16 = use_col16(P, L, 0);
     const unsigned g = grid_for(L.nc);
-    if (L.stream && c16)
+    if (L.stream && c16 && L.finest && (P.x_pairs & 1))
+      k_resid_restrict_f<true, true, true><<<g, kBlock, 0, s>>>(L.nc, L.ftile, L.rperm, L.n, C,
+                                                                L.col, L.val, rf, xf, rc);
+    else if (L.stream && c16)
       k_resid_restrict_f<true, true><<<g, kBlock, 0, s>>>(L.nc, L.ftile, L.rperm, L.n, C, L.col,
                                                           L.val, rf, xf, rc);
     else if (L.stream)
@@ -1238,7 +1338,10 @@ void launch_spmv_dot(Problem &P, const Level &L, const double *p, double *Ap,
 #define HPCGX_SDC(ST, C16)                                                                   \
   k_spmv_dot_c<ST, C16><<<G, kBlock, 0, s>>>(L.chunks, L.n, C, L.col, L.val, p, Ap,           \
                                              P.chunk_partials, L.xcd)
-    if (use_col16(P, L, 0)) {
+    if (use_col16(P, L, 0) && L.finest && L.stream && (P.x_pairs & 1)) {
+      k_spmv_dot_c<true, true, true><<<G, kBlock, 0, s>>>(L.chunks, L.n, C, L.col, L.val, p, Ap,
+                                                         P.chunk_partials, L.xcd);
+    } else if (use_col16(P, L, 0)) {
       if (L.stream) HPCGX_SDC(true, true); else HPCGX_SDC(false, true);
     } else {
       if (L.stream) HPCGX_SDC(true, false); else HPCGX_SDC(false, false);

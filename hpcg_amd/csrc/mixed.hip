@@ -178,16 +178,30 @@ __device__ inline float row_dot32_16(const Col16 &C, const uint32_t *__restrict_
   return s;
 }
 
+// Whether row i's wavefront can share x windows between slots 3m and 3m + 2
+// (kernels.hip's pairs_tile on the tile-major bases): all offset words of the
+// tile affine, slot 3m + 2's base one above slot 3m's for every m, and all
+// 64 rows below `end`.  Uniform per wavefront.
+__device__ inline bool pairs_tile32(const Col16 &C, int t, unsigned am, int64_t end) {
+  if (am != (1u << kSlotWords) - 1u || (int64_t)(t + 1) * kTile > end) return false;
+  const int32_t *__restrict__ base = C.tbase + (int64_t)t * kBaseStride;
+  bool ok = true;
+#pragma unroll
+  for (int m = 0; m < kSlots / 3; ++m) ok = ok && base[3 * m + 2] == base[3 * m] + 1;
+  return ok;
+}
+
 // FP32 colour step of an 8-bit tile (Level::off8, Level::v16p): per row
 // kSlotWords offset words and kPairWords value words, 3 bytes per slot, in
-// gs32_row16's order of terms.  Other tiles take gs32_row16.
-template <bool STREAM>
+// gs32_row16's order of terms.  Other tiles take gs32_row16.  PAIRS: slot
+// 3m + 2's x values from slot 3m's where pairs_tile32 holds.
+template <bool STREAM, bool PAIRS = false>
 __device__ inline float gs32_row8(const Col16 &C, const uint32_t *__restrict__ vp,
                                  const uint32_t *__restrict__ vc, int64_t i, int64_t n,
                                  const int32_t *__restrict__ col,
                                  const half_t *__restrict__ val16,
                                  const float *__restrict__ diag32,
-                                 const float *__restrict__ r, float *x) {
+                                 const float *__restrict__ r, float *x, int64_t end = 0) {
   const int t = tile_of(i);
   const int info = C.info[t];
   if (info != kTile8 || !C.use8) return gs32_row16<STREAM>(C, vc, i, n, col, val16, diag32, r, x);
@@ -199,11 +213,33 @@ __device__ inline float gs32_row8(const Col16 &C, const uint32_t *__restrict__ v
 #pragma unroll
   for (int w = 0; w < kPairWords; ++w) pv[w] = mload<STREAM>(vp + (int64_t)w * n + i);
   float sum = r[i];
+  if (PAIRS && pairs_tile32(C, t, am, end)) {
+    // kernels.hip's affine_pairs_row in FP32: slot 3m + 2's x from slot 3m's,
+    // one lane down; lane 63's from one address for the wavefront
+    const int32_t pos = (int32_t)(i % kTile);
 #pragma unroll
-  for (int j = 0; j < kSlots; ++j) {
-    const int32_t c = base[j] + (int32_t)((o8[j / 4] >> (8 * (j % 4))) & 0xffu);
-    const half_t a = __builtin_bit_cast(half_t, (uint16_t)(pv[j / 2] >> (16 * (j % 2))));
-    sum -= (float)a * x[c];
+    for (int m = 0; m < kSlots / 3; ++m) {
+      const int32_t b0 = base[3 * m];
+      const float x0 = x[b0 + pos];
+      const float x1 = x[base[3 * m + 1] + pos];
+      const float up = __shfl_down(x0, 1, 64);
+      const float last = x[b0 + kTile];
+      const float x2 = pos == kTile - 1 ? last : up;
+      const float xs[3] = {x0, x1, x2};
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int j = 3 * m + q;
+        const half_t a = __builtin_bit_cast(half_t, (uint16_t)(pv[j / 2] >> (16 * (j % 2))));
+        sum -= (float)a * xs[q];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const int32_t c = base[j] + (int32_t)((o8[j / 4] >> (8 * (j % 4))) & 0xffu);
+      const half_t a = __builtin_bit_cast(half_t, (uint16_t)(pv[j / 2] >> (16 * (j % 2))));
+      sum -= (float)a * x[c];
+    }
   }
   const float v = sum / diag32[i];
   x[i] = v;
@@ -353,14 +389,14 @@ __device__ inline int64_t xcd_block(int64_t b, int64_t G, bool on) {
   return (b % 8) * (G8 / 8) + b / 8;
 }
 
-template <bool STREAM, bool FINEST>
+template <bool STREAM, bool FINEST, bool PAIRS = false>
 __global__ __launch_bounds__(kBlock) void k_gs32_range8(
     int64_t begin, int64_t end, int64_t n, Col16 C, const uint32_t *__restrict__ vp,
     const uint32_t *__restrict__ vc, const int32_t *__restrict__ col,
     const half_t *__restrict__ val16, const float *__restrict__ diag32,
     const float *__restrict__ r, float *x, bool xcd) {
   const int64_t i = begin + xcd_block(blockIdx.x, gridDim.x, xcd) * kBlock + threadIdx.x;
-  if (i < end) gs32_row8<STREAM>(C, vp, vc, i, n, col, val16, diag32, r, x);
+  if (i < end) gs32_row8<STREAM, PAIRS>(C, vp, vc, i, n, col, val16, diag32, r, x, end);
 }
 
 template <bool STREAM, bool FINEST>
@@ -379,7 +415,7 @@ __global__ __launch_bounds__(kBlock) void k_gs32_zero(
 // stores the sum of r64_i * (double)(new x_i) over its rows to dpart[k], so
 // the mixed CG's r.z (CG.cpp:110, r in FP64, z widened) needs no pass over
 // r and z of its own.  One partial per wavefront, a 64-lane shuffle tree.
-template <bool STREAM>
+template <bool STREAM, bool PAIRS = false>
 __global__ __launch_bounds__(kBlock) void k_gs32_rz8(
     int64_t begin, int64_t end, int64_t n, Col16 C, const uint32_t *__restrict__ vp,
     const uint32_t *__restrict__ vc, const int32_t *__restrict__ col,
@@ -390,7 +426,7 @@ __global__ __launch_bounds__(kBlock) void k_gs32_rz8(
   const int64_t i = begin + t;
   double sp = 0.0;
   if (i < end) {
-    const float z = gs32_row8<STREAM>(C, vp, vc, i, n, col, val16, diag32, r, x);
+    const float z = gs32_row8<STREAM, PAIRS>(C, vp, vc, i, n, col, val16, diag32, r, x, end);
     sp = mload<STREAM>(r64 + i) * (double)z;
   }
 #pragma unroll
@@ -526,7 +562,14 @@ void gs32_step(const Problem &P, const Level &L, int64_t begin, int64_t end,
     throw Error(HPCGX_EINVAL, "FP32 colour step with r.z partials needs the FP16-pair layout");
   if (pairs_step(P, L, begin)) {
     const Col16 C = col16_of(P, L);
-    if (dpart) {
+    const bool xp = L.finest && (P.x_pairs & 4);  // the x-pair steps (finest level)
+    if (dpart && xp) {
+      k_gs32_rz8<true, true><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.v16p, L.vc, L.col,
+                                                  L.val16, L.diag32, r, x, L.xcd, r64, dpart);
+    } else if (xp) {
+      k_gs32_range8<true, true, true><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.v16p, L.vc,
+                                                           L.col, L.val16, L.diag32, r, x, L.xcd);
+    } else if (dpart) {
       if (L.finest || L.stream)
         k_gs32_rz8<true><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.v16p, L.vc, L.col, L.val16,
                                               L.diag32, r, x, L.xcd, r64, dpart);

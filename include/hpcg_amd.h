@@ -401,6 +401,17 @@ int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, i
  *   "gs_waves"         1/0: the finest level's FP64 colour steps run under
  *                      register caps that allow 7 waves per SIMD (default
  *                      1); 0 leaves the register count to the compiler
+ *   "x_pairs"          0-7: on tiles whose offset words are all affine and
+ *                      whose slot 3m + 2 has the base one above slot 3m's
+ *                      (all m), the finest level's kernels take slot 3m +
+ *                      2's x values from slot 3m's moved one lane
+ *                      (cross-lane permute) instead of a second gather.
+ *                      Bit 0: the FP64 colour steps without r.z partials
+ *                      (at 256 threads per workgroup with the register
+ *                      caps), the chunked SpMV + dot and the fused residual
+ *                      and restriction; bit 1: the FP64 colour step that
+ *                      leaves r.z partials; bit 2: the FP32 V-cycle's colour
+ *                      steps on FP16 value pairs (default 3; the same bits)
  *   "spmv_dot_blocks"  workgroups of the persistent SpMV + dot kernel
  *                      (8..8192, rounded down to a multiple of 8; default
  *                      4096).  Changes the dot's summation order.
@@ -422,10 +433,11 @@ int hpcgx_tune(hpcgx_problem *p, const char *key, double value);
 /* No counterpart in the reference: picks the tuning knobs for this problem
  * and machine by timing `iterations` PCG iterations (plus one warm-up) per
  * candidate, one knob at a time ("col8", "spmv_dot_chunks", "spmv_dot_blocks", "spmv_order",
- * "xcd_remap", and, where the finest level has narrow column ids, "gs_block" and "gs_waves";
- * a fleet times hpcgx_cg_fleet and tunes "fleet_blocks" in place of
- * "spmv_dot_blocks"), keeping the fastest value of each.  Writes one line per timing and per
- * choice into `report` (may be NULL).  Uses the default stream; synchronous.
+ * "xcd_remap", and, where the finest level has narrow column ids, "gs_block", "gs_waves" and
+ * "x_pairs"; a fleet times hpcgx_cg_fleet and tunes "fleet_blocks" in place of
+ * "spmv_dot_blocks"), keeping the fastest value of each.  Each knob's candidates are timed
+ * in two interleaved rounds and keep their faster time.  Writes one line per candidate (that
+ * time) and per choice into `report` (may be NULL).  Uses the default stream; synchronous.
  * Changing "spmv_dot_blocks" changes the dots' summation order. */
 int hpcgx_autotune(hpcgx_problem *p, int ordering, int iterations, char *report,
                    int report_len);

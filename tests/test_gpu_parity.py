@@ -624,8 +624,8 @@ def test_autotune(hp, torch):
     assert {l.split()[1].split("=")[0] for l in chosen} == {"col8", "spmv_dot_chunks",
                                                             "spmv_dot_blocks", "spmv_order",
                                                             "xcd_remap", "gs_block",
-                                                            "gs_waves"}
-    assert len([l for l in rep if "ms/iteration" in l]) == 2 + 2 + 4 + 2 + 2 + 4 + 2
+                                                            "gs_waves", "x_pairs"}
+    assert len([l for l in rep if "ms/iteration" in l]) == 2 + 2 + 4 + 2 + 2 + 4 + 2 + 2
     x.zero_()
     r1 = P.cg(b, x, max_iter=10, tolerance=0.0)
     assert r1["normr"] == pytest.approx(r0["normr"], rel=1e-10)
@@ -817,7 +817,8 @@ def test_256_properties(hp, torch):
 
 # (knob, value, default): each changes how the iteration is scheduled or
 # shaped, never its arithmetic
-SAME_BITS_KNOBS = [("zero_guess", 0, 1), ("defer_x", 0, 1), ("gs_block", 64, 256),
+SAME_BITS_KNOBS = [("zero_guess", 0, 1), ("defer_x", 0, 1), ("x_pairs", 0, 3), ("x_pairs", 7, 3),
+                   ("gs_block", 64, 256),
                    ("gs_block", 128, 256), ("gs_block", 512, 256), ("gs_waves", 0, 1)]
 
 
@@ -858,4 +859,46 @@ def test_schedule_knobs_same_bits(hp, torch):
             assert r["normr"] == r0["normr"] and torch.equal(x, x0), (knob, value)
         finally:
             P.tune(knob, default)
+    del P
+
+
+def test_x_pairs_same_bits(hp, torch):
+    """x_pairs (slot 3m + 2's x values moved one lane from slot 3m's on
+    affine tiles) changes no bit: the FP64 and FP32 V-cycles equal the
+    oracle's, and 12 CG and 12 mixed CG iterations give the x and residual
+    of the run without it.  nx = 256, so half of the finest tiles clear the
+    x boundaries and take that path."""
+    dims = (256, 16, 24)
+    say("x_pairs", dims)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    P.prepare_mixed()
+    h = hier(dims, color=True)
+    rr = rnd(93, P.nrow(0))
+    want_z = h.mg(rr)
+    want_z32 = h.mg32(rr.astype(np.float32))
+    r32 = P.to_device_order(rr).float()
+    b, x, _ = P.vectors()
+    P.tune("x_pairs", 0)
+    r0 = P.cg(b, x, max_iter=12, tolerance=0.0)
+    x0 = x.clone()
+    x.zero_()
+    m0 = P.cg_mixed(b, x, max_iter=12, tolerance=0.0, recompute=5)
+    xm0 = x.clone()
+    P.tune("x_pairs", 7)
+    try:
+        z = P.zeros()
+        P.mg(P.to_device_order(rr), z, ordering="multicolor")
+        assert np.array_equal(P.to_numpy(z), want_z)
+        z32 = P.zeros32()
+        P.mg32(r32, z32)
+        assert np.array_equal(P.to_numpy(z32.double()).astype(np.float32), want_z32)
+        for graph in (False, True):
+            x.zero_()
+            r = P.cg(b, x, max_iter=12, tolerance=0.0, graph=graph)
+            assert r["normr"] == r0["normr"] and torch.equal(x, x0), graph
+        x.zero_()
+        m = P.cg_mixed(b, x, max_iter=12, tolerance=0.0, recompute=5)
+        assert m["normr"] == m0["normr"] and torch.equal(x, xm0)
+    finally:
+        P.tune("x_pairs", 3)
     del P
