@@ -206,8 +206,71 @@ __device__ inline void bgs32_row(int64_t i, int o, int64_t n, const Col16 &C,
   stv<V>(X + i * S + o, sum);
 }
 
+// Whether tile t has all its offset words affine (slot j's column is its
+// base plus the row's position in the tile) and slot 3m + 2's base one
+// above slot 3m's for every m: on the 27-point grid the x - 1 and x + 1
+// neighbours, consecutive rows of one colour block (uniform per tile).
+__device__ inline bool bpairs_tile(const Col16 &C, int t) {
+  if (C.info[t] != kTile8 || !C.amask || C.amask[t] != (1u << kSlotWords) - 1u) return false;
+  const int32_t *__restrict__ base = C.tbase + (int64_t)t * kBaseStride;
+  bool ok = true;
+#pragma unroll
+  for (int m = 0; m < kSlots / 3; ++m) ok = ok && base[3 * m + 2] == base[3 * m] + 1;
+  return ok;
+}
+
+// bgs32_row on a wavefront whose 64 / W rows are all present and whose tile
+// passes bpairs_tile: slot 3m + 2's x rows are slot 3m's one row (W lanes)
+// further on, so they come from slot 3m's values moved down W lanes
+// (ds_bpermute); the wavefront's last row reads its own.  Same terms in the
+// same order.  The loop over the nine slot triples stays rolled: unrolled,
+// the compiler hoists every triple's loads above the lane moves and needs
+// some 180 registers.
+template <bool STREAM, int S>
+__device__ inline void bgs32_row_pairs(int64_t i, int o, int64_t n, const Col16 &C,
+                                       const uint32_t *__restrict__ vc,
+                                       const float *__restrict__ diag32,
+                                       const float *__restrict__ R, float *X) {
+  constexpr int V = 4, W = S / 4, ROWS = 64 / W;
+  const int t = tile_of(i);
+  const int32_t *__restrict__ base = C.tbase + (int64_t)t * kBaseStride;
+  const int lane = (int)(threadIdx.x & 63);
+  const bool last = lane >= 64 - W;
+  const int64_t i0 = i - lane / W;  // the wavefront's first row
+  const int32_t plast = (int32_t)(__builtin_amdgcn_readfirstlane((int)(i0 % kTile)) + ROWS - 1);
+  float sum[V];
+  ldv<V>(R + i * S + o, sum);
+#pragma unroll 1
+  for (int m = 0; m < kSlots / 3; ++m) {
+    const uint32_t w0 = mload<STREAM>(vc + (int64_t)(3 * m) * n + i);
+    const uint32_t w1 = mload<STREAM>(vc + (int64_t)(3 * m + 1) * n + i);
+    const uint32_t w2 = mload<STREAM>(vc + (int64_t)(3 * m + 2) * n + i);
+    const int32_t c0 = base[3 * m] + (int32_t)(w0 >> 16);
+    const int32_t c1 = base[3 * m + 1] + (int32_t)(w1 >> 16);
+    float x0[V], x1[V], xl[V];
+    ldv<V>(X + (int64_t)c0 * S + o, x0);
+    ldv<V>(X + (int64_t)c1 * S + o, x1);
+    ldv<V>(X + (int64_t)(base[3 * m + 2] + plast) * S + o, xl);
+    const float a0 = (float)vc_val(w0), a1 = (float)vc_val(w1), a2 = (float)vc_val(w2);
+#pragma unroll
+    for (int k = 0; k < V; ++k) sum[k] -= a0 * x0[k];
+#pragma unroll
+    for (int k = 0; k < V; ++k) sum[k] -= a1 * x1[k];
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      const float up = __shfl_down(x0[k], W, 64);
+      sum[k] -= a2 * (last ? xl[k] : up);
+    }
+  }
+  const float d = diag32[i];
+#pragma unroll
+  for (int k = 0; k < V; ++k) sum[k] = sum[k] / d;
+  stv<V>(X + i * S + o, sum);
+}
+
 // One colour step, rows [begin, end) (begin a multiple of kTile when C16).
-template <bool STREAM, int S, bool C16>
+// PAIRS: bgs32_row_pairs on the wavefronts where it applies.
+template <bool STREAM, int S, bool C16, bool PAIRS = false>
 __global__ __launch_bounds__(kBlock) void k_bgs32_rows(int64_t begin, int64_t end, int64_t n,
                                                        Col16 C, const uint32_t *__restrict__ vc,
                                                        const int32_t *__restrict__ col,
@@ -217,6 +280,14 @@ __global__ __launch_bounds__(kBlock) void k_bgs32_rows(int64_t begin, int64_t en
   constexpr int W = S / 4;
   const int64_t g = gtid();
   const int64_t i = begin + g / W;
+  if (PAIRS) {
+    // the wavefront's first row; all its 64 / W rows present (uniform)
+    const int64_t i0 = begin + ((int64_t)blockIdx.x * kBlock + (threadIdx.x & ~63u)) / W;
+    if (i0 + 64 / W <= end && bpairs_tile(C, tile_of(i0))) {
+      bgs32_row_pairs<STREAM, S>(i, (int)(g % W) * 4, n, C, vc, diag32, R, X);
+      return;
+    }
+  }
   if (i >= end) return;
   bgs32_row<STREAM, S, C16>(i, (int)(g % W) * 4, n, C, vc, col, val16, diag32, R, X);
 }
@@ -599,7 +670,15 @@ void gs_rows_s(const Problem &P, const Level &L, int64_t b, int64_t e, const flo
   const Col16 C = col16_of(P, L);
   const bool c16 = L.vc && use_col16(P, L, b);
   const unsigned g = grid_for((e - b) * (S / 4));
-  if (L.stream && c16)
+  // the shared neighbour rows: 16.22 -> 15.63 ms per block iteration at s = 8
+  // (256^3), 30.05 -> 30.79 ms at s = 16 (same job), so s = 8 only
+  if (S == 8 && c16 && P.block_pairs && L.finest && L.stream)
+    k_bgs32_rows<true, S, true, true><<<g, kBlock, 0, st>>>(b, e, L.n, C, L.vc, L.col, L.val16,
+                                                            L.diag32, R, X);
+  else if (S == 8 && c16 && P.block_pairs && L.finest)
+    k_bgs32_rows<false, S, true, true><<<g, kBlock, 0, st>>>(b, e, L.n, C, L.vc, L.col,
+                                                             L.val16, L.diag32, R, X);
+  else if (L.stream && c16)
     k_bgs32_rows<true, S, true><<<g, kBlock, 0, st>>>(b, e, L.n, C, L.vc, L.col, L.val16,
                                                       L.diag32, R, X);
   else if (L.stream)

@@ -419,6 +419,14 @@ int hpcgx_tune(hpcgx_problem *p, const char *key, double value) {
       P.gs_block = (int)value;
       return;
     }
+    if (k == "block_pairs") {
+      P.block_pairs = value != 0.0;
+      return;
+    }
+    if (k == "xr_rows") {
+      P.xr_rows = value != 0.0;
+      return;
+    }
     if (k == "x_pairs") {
       if (value < 0 || value > 7 || value != (int)value)
         throw Error(HPCGX_EINVAL, "x_pairs must be an integer from 0 to 7");

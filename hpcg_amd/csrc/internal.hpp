@@ -237,6 +237,12 @@ struct Problem {
   // are one element apart share one gather ("x_pairs"; bit 0: the colour
   // steps without r.z and the SpMV, bit 1: the r.z colour step)
   int x_pairs = 3;
+  // the r (and x) update of CG with one row per thread and a second launch
+  // for r.r ("xr_rows")
+  bool xr_rows = true;
+  // block PCG finest colour steps: slot 3m + 2's x rows from slot 3m's by
+  // W-lane moves on affine tiles ("block_pairs")
+  bool block_pairs = true;
   int spmv_dot_blocks = 4096; // persistent grid of the fused SpMV + dot
   // The fused SpMV + dot with one workgroup per row chunk of the fine level
   // ("spmv_dot_chunks"): the p.Ap partial of each chunk, summed in fixed

@@ -748,14 +748,15 @@ __device__ inline void grid_finish(double s, double *partials, unsigned *ticket,
 __global__ __launch_bounds__(kBlock) void k_chunk_dot(int64_t nchunks,
                                                       const double *__restrict__ cpart,
                                                       double *partials, unsigned *ticket,
-                                                      double *scalars, int finish) {
+                                                      double *scalars, int finish,
+                                                      double *out = nullptr) {
   const int64_t per = (nchunks + gridDim.x - 1) / gridDim.x;
   const int64_t lo = (int64_t)blockIdx.x * per;
   const int64_t hi = lo + per < nchunks ? lo + per : nchunks;
   double s = 0.0;
   for (int64_t q = lo + threadIdx.x; q < hi; q += kBlock) s += cpart[q];
   s = block_sum(s);
-  grid_finish(s, partials, ticket, scalars, nullptr, finish);
+  grid_finish(s, partials, ticket, scalars, out, finish);
 }
 
 constexpr int64_t kStreamRows = (int64_t)1 << 24;
@@ -960,6 +961,31 @@ __global__ __launch_bounds__(kBlock) void k_update_xr(
   }
   s = block_sum(s);
   grid_finish(s, partials, ticket, scalars, &scalars[kNormr2], kFinishStore);
+}
+
+// k_update_xr with one row per thread: each workgroup stores its r.r
+// partial with a plain store (no ticket), and k_chunk_dot sums the partials
+// in a fixed order in a second launch ("xr_rows").  At 512x512x640 the
+// update without x takes 671 us this way against 909 us in the grid-stride
+// form, which finishes r.r in its last workgroup (same job).
+template <bool R32, bool STREAM, bool WITH_X>
+__global__ __launch_bounds__(kBlock) void k_update_xr_rows(
+    int64_t n, double *__restrict__ x, double *__restrict__ r, const double *__restrict__ p,
+    const double *__restrict__ Ap, float *__restrict__ r32, const double *__restrict__ scalars,
+    double *__restrict__ part) {
+  const int64_t i = gtid();
+  double s = 0.0;
+  if (i < n) {
+    const double alpha = scalars[kAlpha];
+    const double malpha = -alpha;
+    if (WITH_X) mstore<STREAM>(x + i, mload<STREAM>(x + i) + alpha * mload<STREAM>(p + i));
+    const double ri = mload<STREAM>(r + i) + malpha * mload<STREAM>(Ap + i);
+    mstore<STREAM>(r + i, ri);
+    if (R32) mstore<STREAM>(r32 + i, (float)ri);
+    s = ri * ri;
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
 // CG.cpp:121's step of the previous iteration, x += alpha p, taken here
@@ -1406,6 +1432,29 @@ void launch_spmv_dot_half(Problem &P, const Level &L, const double *p, double *A
 void launch_update_xr(Problem &P, int64_t n, double *x, double *r,
                       const double *p, const double *Ap, hipStream_t s, float *r32) {
   const bool st = stream_vec(n);
+  if (P.xr_rows && P.chunk_partials && (int64_t)grid_for(n) <= P.nchunk_partials) {
+    const unsigned g = grid_for(n);
+#define HPCGX_UXRR(R32, ST, WX)                                                                 \
+  k_update_xr_rows<R32, ST, WX><<<g, kBlock, 0, s>>>(n, x, r, p, Ap, r32, P.scalars,            \
+                                                     P.chunk_partials)
+    if (x) {
+      if (r32 && st) HPCGX_UXRR(true, true, true);
+      else if (r32) HPCGX_UXRR(true, false, true);
+      else if (st) HPCGX_UXRR(false, true, true);
+      else HPCGX_UXRR(false, false, true);
+    } else {
+      if (r32 && st) HPCGX_UXRR(true, true, false);
+      else if (r32) HPCGX_UXRR(true, false, false);
+      else if (st) HPCGX_UXRR(false, true, false);
+      else HPCGX_UXRR(false, false, false);
+    }
+#undef HPCGX_UXRR
+    HPCGX_LAUNCHED();
+    k_chunk_dot<<<reduce_grid(g), kBlock, 0, s>>>(g, P.chunk_partials, P.partials, P.ticket,
+                                                  P.scalars, kFinishStore, &P.scalars[kNormr2]);
+    HPCGX_LAUNCHED();
+    return;
+  }
   const unsigned g = reduce_grid(n);
 #define HPCGX_UXR(R32, ST, WX)                                                                \
   k_update_xr<R32, ST, WX><<<g, kBlock, 0, s>>>(n, x, r, p, Ap, r32, P.partials, P.ticket,    \

@@ -412,6 +412,17 @@ int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, i
  *                      and restriction; bit 1: the FP64 colour step that
  *                      leaves r.z partials; bit 2: the FP32 V-cycle's colour
  *                      steps on FP16 value pairs (default 3; the same bits)
+ *   "block_pairs"      1/0: the block PCG's finest colour steps at s = 8,
+ *                      on tiles where "x_pairs" applies, take slot 3m + 2's
+ *                      neighbour rows from slot 3m's moved one row (2
+ *                      lanes) instead of a second gather (default 1; the
+ *                      same bits)
+ *   "xr_rows"          1/0: CG's r update (with x's step where it is not
+ *                      deferred) runs one row per thread and sums r.r from
+ *                      per-workgroup partials in a second launch, instead
+ *                      of a grid-stride kernel that finishes r.r in its
+ *                      last workgroup (default 1).  Changes r.r's summation
+ *                      order.
  *   "spmv_dot_blocks"  workgroups of the persistent SpMV + dot kernel
  *                      (8..8192, rounded down to a multiple of 8; default
  *                      4096).  Changes the dot's summation order.

@@ -155,3 +155,38 @@ def test_cg_block_graph_replays(hp):
         x = P.zeros()
         P.cg_mixed(B[:, 0].contiguous(), x, max_iter=2, tolerance=0.0)
     del P
+
+
+@pytest.mark.parametrize("s", [8])
+def test_block_pairs_same_bits(hp, s):
+    """block_pairs (slot 3m + 2's x rows moved W lanes from slot 3m's on
+    affine tiles) changes no bit: the block V-cycle equals the oracle's per
+    column, and 12 block CG iterations give the x of the run without it.
+    nx = 256, so half of the finest tiles clear the x boundaries."""
+    import torch
+    dims = (256, 16, 24)
+    say("block pairs", dims, s)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    h = O.Hierarchy(*dims)
+    h.use_color_order()
+    n = h.nrow(0)
+    Rn = rhs(n, s, 13).astype(np.float32)
+    R = P.to_device_order_multi(Rn, 0, "float32")
+    B = P.to_device_order_multi(rhs(n, s, 42), 0)
+    X = torch.zeros_like(B)
+    P.tune("block_pairs", 0)
+    P.cg_block_mixed(B, X, max_iter=12, tolerance=0.0, recompute=5)
+    X0 = X.clone()
+    P.tune("block_pairs", 1)
+    try:
+        Z = torch.zeros_like(R)
+        P.mg32_block(R, Z, ordering="multicolor")
+        got = P.to_numpy_multi(Z).astype(np.float32)
+        for k in range(s):
+            assert np.array_equal(got[:, k], h.mg32(Rn[:, k])), k
+        X.zero_()
+        P.cg_block_mixed(B, X, max_iter=12, tolerance=0.0, recompute=5)
+        assert torch.equal(X, X0)
+    finally:
+        P.tune("block_pairs", 1)
+    del P

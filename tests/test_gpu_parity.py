@@ -902,3 +902,28 @@ def test_x_pairs_same_bits(hp, torch):
     finally:
         P.tune("x_pairs", 3)
     del P
+
+
+def test_xr_rows(hp, torch):
+    """The one-row r update (xr_rows) sums r.r in another order and changes
+    nothing else: x keeps its bits in the FP64 and the mixed CG (alpha and
+    beta do not use r.r), and the residual norm agrees to rounding."""
+    dims = (24, 40, 56)
+    say("xr_rows", dims)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    b, x, _ = P.vectors()
+    res = {}
+    for v in (0, 1):
+        P.tune("xr_rows", v)
+        x.zero_()
+        r = P.cg(b, x, max_iter=15, tolerance=0.0)
+        xf = x.clone()
+        x.zero_()
+        m = P.cg_mixed(b, x, max_iter=15, tolerance=0.0, recompute=4)
+        res[v] = (r, xf, m, x.clone())
+    P.tune("xr_rows", 1)
+    (r0, x0, m0, xm0), (r1, x1, m1, xm1) = res[0], res[1]
+    assert torch.equal(x0, x1) and torch.equal(xm0, xm1)
+    assert r1["normr"] == pytest.approx(r0["normr"], rel=1e-13)
+    assert m1["normr"] == pytest.approx(m0["normr"], rel=1e-13)
+    del P
