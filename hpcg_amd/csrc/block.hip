@@ -184,9 +184,12 @@ __device__ inline void slots32(const Col16 &C, const uint32_t *__restrict__ vc,
   }
 }
 
-// Row i, columns [o, o + 4): mixed.hip's gs32_row for each column.
+// Row i, columns [o, o + 4): mixed.hip's gs32_row for each column.  Always
+// inlined: with the x-sharing kernels calling it too the compiler made it a
+// function of its own, and a call per row took the colour step from 99 to
+// 31 registers and from about 190 to 264 us (s = 8, 256^3).
 template <bool STREAM, int S, bool C16>
-__device__ inline void bgs32_row(int64_t i, int o, int64_t n, const Col16 &C,
+__device__ __attribute__((always_inline)) inline void bgs32_row(int64_t i, int o, int64_t n, const Col16 &C,
                                  const uint32_t *__restrict__ vc,
                                  const int32_t *__restrict__ col,
                                  const half_t *__restrict__ val16,
@@ -227,7 +230,7 @@ __device__ inline bool bpairs_tile(const Col16 &C, int t) {
 // the compiler hoists every triple's loads above the lane moves and needs
 // some 180 registers.
 template <bool STREAM, int S>
-__device__ inline void bgs32_row_pairs(int64_t i, int o, int64_t n, const Col16 &C,
+__device__ __attribute__((always_inline)) inline void bgs32_row_pairs(int64_t i, int o, int64_t n, const Col16 &C,
                                        const uint32_t *__restrict__ vc,
                                        const float *__restrict__ diag32,
                                        const float *__restrict__ R, float *X) {

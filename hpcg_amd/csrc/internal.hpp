@@ -242,7 +242,7 @@ struct Problem {
   bool xr_rows = true;
   // block PCG finest colour steps: slot 3m + 2's x rows from slot 3m's by
   // W-lane moves on affine tiles ("block_pairs")
-  bool block_pairs = true;
+  bool block_pairs = false;
   int spmv_dot_blocks = 4096; // persistent grid of the fused SpMV + dot
   // The fused SpMV + dot with one workgroup per row chunk of the fine level
   // ("spmv_dot_chunks"): the p.Ap partial of each chunk, summed in fixed

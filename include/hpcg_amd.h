@@ -415,7 +415,7 @@ int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, i
  *   "block_pairs"      1/0: the block PCG's finest colour steps at s = 8,
  *                      on tiles where "x_pairs" applies, take slot 3m + 2's
  *                      neighbour rows from slot 3m's moved one row (2
- *                      lanes) instead of a second gather (default 1; the
+ *                      lanes) instead of a second gather (default 0; the
  *                      same bits)
  *   "xr_rows"          1/0: CG's r update (with x's step where it is not
  *                      deferred) runs one row per thread and sums r.r from

@@ -188,5 +188,5 @@ def test_block_pairs_same_bits(hp, s):
         P.cg_block_mixed(B, X, max_iter=12, tolerance=0.0, recompute=5)
         assert torch.equal(X, X0)
     finally:
-        P.tune("block_pairs", 1)
+        P.tune("block_pairs", 0)
     del P
