@@ -209,6 +209,93 @@ __device__ __attribute__((always_inline)) inline void bgs32_row(int64_t i, int o
   stv<V>(X + i * S + o, sum);
 }
 
+// ---- the first forward sweep of a block V-cycle level from X = 0 ----------
+// kernels.hip's zero-x colour step for W lanes per row: the slots that
+// Level::zmask leaves out of the tile, and the X rows at or beyond `begin`
+// (colours not yet updated in this sweep), are read through buffer views
+// whose range check returns 0 without a memory access, so every row's loads
+// still go out together and X needs no fill.  Skipped terms are a * 0.
+// Needs n * S * 4 < 2^32 (byte offsets into X).
+typedef __amdgpu_buffer_rsrc_t BufView;
+constexpr int kBufWord3 = 0x00020000;  // buffer descriptor word 3 for gfx9 raw loads
+__device__ inline BufView buf_view(const void *p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, kBufWord3);
+}
+template <bool STREAM>
+__device__ inline uint32_t buf_u32(BufView v, uint32_t off) {
+  return __builtin_amdgcn_raw_buffer_load_b32(v, off, 0, STREAM ? 2 : 0);
+}
+template <bool STREAM>
+__device__ inline uint32_t buf_u16(BufView v, uint32_t off) {
+  return __builtin_amdgcn_raw_buffer_load_b16(v, off, 0, STREAM ? 2 : 0);
+}
+__device__ inline float4 buf_f32x4(BufView v, uint32_t off) {
+  return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(v, off, 0, 0));
+}
+
+template <bool STREAM, int S>
+__device__ __attribute__((always_inline)) inline void bgs32_row_zero(
+    uint32_t zm, int64_t begin, int64_t i, int o, int64_t n, const Col16 &C,
+    const uint32_t *__restrict__ vc, const int32_t *__restrict__ col,
+    const half_t *__restrict__ val16, const float *__restrict__ diag32,
+    const float *__restrict__ R, float *X) {
+  constexpr int V = 4;
+  const int t = tile_of(i);
+  const int64_t tb = (int64_t)t * kTile;
+  const uint32_t row = (uint32_t)(i - tb);
+  const uint32_t xlim = (uint32_t)(begin * S * 4);
+  float sum[V];
+  ldv<V>(R + i * S + o, sum);
+  auto term = [&](bool live, float a, int32_t c) {
+    const float4 xv = buf_f32x4(buf_view(X, live ? xlim : 0u), (uint32_t)(c * S + o) * 4u);
+    sum[0] -= a * xv.x;
+    sum[1] -= a * xv.y;
+    sum[2] -= a * xv.z;
+    sum[3] -= a * xv.w;
+  };
+  if (C.info[t] != kTile32) {
+    const int32_t *__restrict__ base = C.tbase + (int64_t)t * kBaseStride;
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const bool live = (zm >> j) & 1u;
+      const uint32_t w =
+          buf_u32<STREAM>(buf_view(vc + (int64_t)j * n + tb, live ? 256u : 0u), row * 4u);
+      term(live, (float)vc_val(w), base[j] + (int32_t)(w >> 16));
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const bool live = (zm >> j) & 1u;
+      const int32_t c = (int32_t)buf_u32<STREAM>(
+          buf_view(col + (int64_t)j * n + tb, live ? 256u : 0u), row * 4u);
+      const half_t a = __builtin_bit_cast(
+          half_t, (uint16_t)buf_u16<STREAM>(buf_view(val16 + (int64_t)j * n + tb,
+                                                     live ? 128u : 0u), row * 2u));
+      term(live, (float)a, c);
+    }
+  }
+  const float d = diag32[i];
+#pragma unroll
+  for (int k = 0; k < V; ++k) sum[k] = sum[k] / d;
+  stv<V>(X + i * S + o, sum);
+}
+
+// A colour step of the first forward sweep from X = 0, rows [begin, end)
+// (begin a multiple of kTile).
+template <bool STREAM, int S>
+__global__ __launch_bounds__(kBlock) void k_bgs32_rows_zero(
+    int64_t begin, int64_t end, int64_t n, Col16 C, const uint32_t *__restrict__ zmask,
+    const uint32_t *__restrict__ vc, const int32_t *__restrict__ col,
+    const half_t *__restrict__ val16, const float *__restrict__ diag32,
+    const float *__restrict__ R, float *X) {
+  constexpr int W = S / 4;
+  const int64_t g = gtid();
+  const int64_t i = begin + g / W;
+  if (i >= end) return;
+  bgs32_row_zero<STREAM, S>(zmask[tile_of(i)], begin, i, (int)(g % W) * 4, n, C, vc, col, val16,
+                            diag32, R, X);
+}
+
 // Whether tile t has all its offset words affine (slot j's column is its
 // base plus the row's position in the tile) and slot 3m + 2's base one
 // above slot 3m's for every m: on the 27-point grid the x - 1 and x + 1
@@ -696,6 +783,29 @@ void gs_rows_s(const Problem &P, const Level &L, int64_t b, int64_t e, const flo
   HPCGX_LAUNCHED();
 }
 
+template <int S>
+void gs_rows_zero_s(const Problem &P, const Level &L, int64_t b, int64_t e, const float *R,
+                    float *X, hipStream_t st) {
+  const Col16 C = col16_of(P, L);
+  const unsigned g = grid_for((e - b) * (S / 4));
+  if (L.stream)
+    k_bgs32_rows_zero<true, S><<<g, kBlock, 0, st>>>(b, e, L.n, C, L.zmask, L.vc, L.col,
+                                                     L.val16, L.diag32, R, X);
+  else
+    k_bgs32_rows_zero<false, S><<<g, kBlock, 0, st>>>(b, e, L.n, C, L.zmask, L.vc, L.col,
+                                                      L.val16, L.diag32, R, X);
+  HPCGX_LAUNCHED();
+}
+
+// The colour step of a forward sweep that began from X = 0 (only where
+// zero_guess_block holds).
+void gs_rows_zero(const Problem &P, const Level &L, int s, int64_t b, int64_t e, const float *R,
+                  float *X, hipStream_t st) {
+  if (s == 4) gs_rows_zero_s<4>(P, L, b, e, R, X, st);
+  else if (s == 8) gs_rows_zero_s<8>(P, L, b, e, R, X, st);
+  else gs_rows_zero_s<16>(P, L, b, e, R, X, st);
+}
+
 void gs_rows(const Problem &P, const Level &L, int s, int64_t b, int64_t e, const float *R,
              float *X, hipStream_t st) {
   if (s == 4) gs_rows_s<4>(P, L, b, e, R, X, st);
@@ -794,15 +904,32 @@ void spmv_rows(const Problem &P, const Level &L, int s, const double *X, double 
   else spmv_rows_s<16>(P, L, X, Y, dot, g, st);
 }
 
+// Whether level l's first forward block sweep of a V-cycle can start from an
+// unfilled X (every colour step through gs_rows_zero).
+bool zero_guess_block(const Problem &P, int l, int ordering, int s) {
+  const Level &L = P.L[l];
+  if (!P.zero_guess || ordering != HPCGX_ORDER_MULTICOLOR || !rows_kernels(s) || !L.zmask ||
+      !L.vc || !L.tbase || !use_col16(P, L, 0) || L.n * s * 4 >= ((int64_t)1 << 32))
+    return false;
+  for (int c = 0; c < L.ncolors; ++c)
+    if (L.color_ptr[c] % kTile) return false;
+  return true;
+}
+
+// zero_x: X was zero when the sweep began (only with zero_guess_block).
 void bsymgs32(Problem &P, int l, int ordering, int s, const float *R, float *X,
-              hipStream_t st) {
+              hipStream_t st, bool zero_x = false) {
   const Level &L = P.L[l];
   if (ordering == HPCGX_ORDER_MULTICOLOR && rows_kernels(s)) {
     for (int pass = 0; pass < 2; ++pass)
       for (int q = 0; q < L.ncolors; ++q) {
         const int c = pass == 0 ? q : L.ncolors - 1 - q;
         const int64_t b = L.color_ptr[c], e = L.color_ptr[c + 1];
-        if (e > b) gs_rows(P, L, s, b, e, R, X, st);
+        if (e <= b) continue;
+        if (pass == 0 && zero_x)
+          gs_rows_zero(P, L, s, b, e, R, X, st);
+        else
+          gs_rows(P, L, s, b, e, R, X, st);
       }
   } else if (ordering == HPCGX_ORDER_MULTICOLOR) {
     for (int pass = 0; pass < 2; ++pass)
@@ -897,9 +1024,14 @@ void Problem::BlockState::release(Problem &P) {
 void mg32_block(Problem &P, int l, int ordering, int s, const float *R, float *X,
                 hipStream_t st) {
   Level &L = P.L[l];
-  k_bfill32<<<grid_for(L.n * s), kBlock, 0, st>>>(X, L.n * s);
-  HPCGX_LAUNCHED();
-  bsymgs32(P, l, ordering, s, R, X, st);
+  // X = 0 (ComputeMG_ref.cpp:42): a fill, or implied by a first forward
+  // sweep that reads only the X it has written
+  const bool zg = zero_guess_block(P, l, ordering, s);
+  if (!zg) {
+    k_bfill32<<<grid_for(L.n * s), kBlock, 0, st>>>(X, L.n * s);
+    HPCGX_LAUNCHED();
+  }
+  bsymgs32(P, l, ordering, s, R, X, st, zg);
   if (l + 1 < P.nlevels) {
     float *rc = P.blk.rc32[l], *xc = P.blk.xc32[l];
     const bool rows = rows_kernels(s) && L.rperm;

@@ -387,11 +387,12 @@ int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, i
  *                      column ids, FP16 value pairs in the mixed V-cycle).
  *                      Changes the dot's summation order.
  *   "zero_guess"       1/0: the V-cycle's first forward sweep of a level
- *                      (multicolour ordering, narrow column ids) starts from
- *                      the zero x without filling it: each colour step reads
- *                      only the slots and x values of colours already
- *                      updated in that sweep (default 1; the same x as a
- *                      zero fill followed by the full sweep)
+ *                      (multicolour ordering, narrow column ids; FP64, FP32
+ *                      and block V-cycles) starts from the zero x without
+ *                      filling it: each colour step reads only the slots
+ *                      and x values of colours already updated in that
+ *                      sweep (default 1; the same x as a zero fill followed
+ *                      by the full sweep)
  *   "defer_x"          1/0: hpcgx_cg takes each iteration's x += alpha p
  *                      with the next iteration's p update, in one pass over
  *                      x and p, and the last one after the loop (default 1;

@@ -190,3 +190,33 @@ def test_block_pairs_same_bits(hp, s):
     finally:
         P.tune("block_pairs", 0)
     del P
+
+
+@pytest.mark.parametrize("s", [4, 8, 16])
+def test_block_zero_guess_same_bits(hp, s):
+    """The block V-cycle's first forward sweep from X = 0 without a fill
+    (zero_guess) gives the bits of the fill followed by the full sweep: the
+    block V-cycle and 12 block CG iterations, with zero_guess on and off."""
+    import torch
+    dims = (32, 48, 64)
+    say("block zero guess", dims, s)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    n = P.nrow(0)
+    R = P.to_device_order_multi(rhs(n, s, 17).astype(np.float32), 0, "float32")
+    B = P.to_device_order_multi(rhs(n, s, 42), 0)
+    out = {}
+    try:
+        for zg in (0, 1):
+            P.tune("zero_guess", zg)
+            Z = torch.full_like(R, float("nan"))  # the zero-x sweep must not read it
+            if not zg:
+                Z.zero_()
+            P.mg32_block(R, Z, ordering="multicolor")
+            X = torch.zeros_like(B)
+            P.cg_block_mixed(B, X, max_iter=12, tolerance=0.0, recompute=5)
+            out[zg] = (Z, X)
+    finally:
+        P.tune("zero_guess", 1)
+    assert torch.equal(out[0][0], out[1][0])
+    assert torch.equal(out[0][1], out[1][1])
+    del P
