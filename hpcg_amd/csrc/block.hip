@@ -280,9 +280,96 @@ __device__ __attribute__((always_inline)) inline void bgs32_row_zero(
   stv<V>(X + i * S + o, sum);
 }
 
+// bgs32_row_zero over the live slots only (tune key "zero_live"): Level::
+// zmask's set bits in ascending slot order, CH per chunk, the chunk's
+// unused entries read through empty views (0 * 0), so the loads of dead
+// slots are not issued.  NARROW: packed words (else 32-bit ids), decided
+// once per tile outside the loop.
+template <bool STREAM, int S, bool NARROW, int CH>
+__device__ __attribute__((always_inline)) inline void bzero_live_sum(
+    uint32_t zm, int t, int64_t tb, uint32_t row, uint32_t xlim, int o, int64_t n,
+    const Col16 &C, const uint32_t *__restrict__ vc, const int32_t *__restrict__ col,
+    const half_t *__restrict__ val16, const float *X, float *sum) {
+  const int32_t *__restrict__ base = C.tbase + (int64_t)t * kBaseStride;
+  uint32_t mm = zm;
+  while (mm) {
+    int js[CH];
+    bool ok[CH];
+#pragma unroll
+    for (int q = 0; q < CH; ++q) {
+      ok[q] = mm != 0u;
+      js[q] = ok[q] ? __builtin_ctz(mm) : 0;
+      mm &= mm - 1u;
+    }
+    // the chunk's index and value words, then its X rows, then the sums
+    uint32_t wa[CH], wb[CH];
+#pragma unroll
+    for (int q = 0; q < CH; ++q) {
+      const int j = js[q];
+      if (NARROW) {
+        wa[q] = buf_u32<STREAM>(buf_view(vc + (int64_t)j * n + tb, ok[q] ? 256u : 0u), row * 4u);
+      } else {
+        wa[q] = buf_u32<STREAM>(buf_view(col + (int64_t)j * n + tb, ok[q] ? 256u : 0u), row * 4u);
+        wb[q] = buf_u16<STREAM>(buf_view(val16 + (int64_t)j * n + tb, ok[q] ? 128u : 0u),
+                                row * 2u);
+      }
+    }
+    float a[CH];
+    float4 xv[CH];
+#pragma unroll
+    for (int q = 0; q < CH; ++q) {
+      const int j = js[q];
+      int32_t c;
+      if (NARROW) {
+        a[q] = (float)vc_val(wa[q]);
+        c = base[j] + (int32_t)(wa[q] >> 16);
+      } else {
+        c = (int32_t)wa[q];
+        a[q] = (float)__builtin_bit_cast(half_t, (uint16_t)wb[q]);
+      }
+      xv[q] = buf_f32x4(buf_view(X, ok[q] ? xlim : 0u), (uint32_t)(c * S + o) * 4u);
+    }
+#pragma unroll
+    for (int q = 0; q < CH; ++q) {
+      sum[0] -= a[q] * xv[q].x;
+      sum[1] -= a[q] * xv[q].y;
+      sum[2] -= a[q] * xv[q].z;
+      sum[3] -= a[q] * xv[q].w;
+    }
+  }
+}
+
+template <bool STREAM, int S, int CH>
+__device__ __attribute__((always_inline)) inline void bgs32_row_zero_live(
+    uint32_t zm, int64_t begin, int64_t i, int o, int64_t n, const Col16 &C,
+    const uint32_t *__restrict__ vc, const int32_t *__restrict__ col,
+    const half_t *__restrict__ val16, const float *__restrict__ diag32,
+    const float *__restrict__ R, float *X) {
+  constexpr int V = 4;
+  // a wavefront's 64 / W rows lie in one tile (begin is a multiple of
+  // kTile), so the tile and its mask are uniform: the slot loop runs on
+  // scalar registers
+  const int t = __builtin_amdgcn_readfirstlane(tile_of(i));
+  zm = __builtin_amdgcn_readfirstlane(zm);
+  const int64_t tb = (int64_t)t * kTile;
+  const uint32_t row = (uint32_t)(i - tb);
+  const uint32_t xlim = (uint32_t)(begin * S * 4);
+  float sum[V];
+  ldv<V>(R + i * S + o, sum);
+  if (C.info[t] != kTile32)
+    bzero_live_sum<STREAM, S, true, CH>(zm, t, tb, row, xlim, o, n, C, vc, col, val16, X, sum);
+  else
+    bzero_live_sum<STREAM, S, false, CH>(zm, t, tb, row, xlim, o, n, C, vc, col, val16, X, sum);
+  const float d = diag32[i];
+#pragma unroll
+  for (int k = 0; k < V; ++k) sum[k] = sum[k] / d;
+  stv<V>(X + i * S + o, sum);
+}
+
 // A colour step of the first forward sweep from X = 0, rows [begin, end)
 // (begin a multiple of kTile).
-template <bool STREAM, int S>
+// LIVE: 0 every slot (bgs32_row_zero), else the live slots in chunks of LIVE
+template <bool STREAM, int S, int LIVE = 0>
 __global__ __launch_bounds__(kBlock) void k_bgs32_rows_zero(
     int64_t begin, int64_t end, int64_t n, Col16 C, const uint32_t *__restrict__ zmask,
     const uint32_t *__restrict__ vc, const int32_t *__restrict__ col,
@@ -292,8 +379,12 @@ __global__ __launch_bounds__(kBlock) void k_bgs32_rows_zero(
   const int64_t g = gtid();
   const int64_t i = begin + g / W;
   if (i >= end) return;
-  bgs32_row_zero<STREAM, S>(zmask[tile_of(i)], begin, i, (int)(g % W) * 4, n, C, vc, col, val16,
-                            diag32, R, X);
+  if (LIVE)
+    bgs32_row_zero_live<STREAM, S, LIVE ? LIVE : 8>(zmask[tile_of(i)], begin, i, (int)(g % W) * 4,
+                                                    n, C, vc, col, val16, diag32, R, X);
+  else
+    bgs32_row_zero<STREAM, S>(zmask[tile_of(i)], begin, i, (int)(g % W) * 4, n, C, vc, col,
+                              val16, diag32, R, X);
 }
 
 // Whether tile t has all its offset words affine (slot j's column is its
@@ -788,12 +879,17 @@ void gs_rows_zero_s(const Problem &P, const Level &L, int64_t b, int64_t e, cons
                     float *X, hipStream_t st) {
   const Col16 C = col16_of(P, L);
   const unsigned g = grid_for((e - b) * (S / 4));
-  if (L.stream)
-    k_bgs32_rows_zero<true, S><<<g, kBlock, 0, st>>>(b, e, L.n, C, L.zmask, L.vc, L.col,
-                                                     L.val16, L.diag32, R, X);
-  else
-    k_bgs32_rows_zero<false, S><<<g, kBlock, 0, st>>>(b, e, L.n, C, L.zmask, L.vc, L.col,
-                                                      L.val16, L.diag32, R, X);
+#define HPCGX_BZ(ST, LI)                                                                      \
+  k_bgs32_rows_zero<ST, S, LI><<<g, kBlock, 0, st>>>(b, e, L.n, C, L.zmask, L.vc, L.col,        \
+                                                     L.val16, L.diag32, R, X)
+  const int live = !(P.zero_live & 8) ? 0 : P.zero_chunk == 16 ? 16 : 8;
+  if (L.stream && live == 16) HPCGX_BZ(true, 16);
+  else if (L.stream && live) HPCGX_BZ(true, 8);
+  else if (L.stream) HPCGX_BZ(true, 0);
+  else if (live == 16) HPCGX_BZ(false, 16);
+  else if (live) HPCGX_BZ(false, 8);
+  else HPCGX_BZ(false, 0);
+#undef HPCGX_BZ
   HPCGX_LAUNCHED();
 }
 

@@ -471,13 +471,108 @@ __device__ inline void gs_row_zero(const Col16 &C, uint32_t zm, int64_t begin, i
   x[i] = sum / d;
 }
 
-template <bool STREAM, bool FINEST, int BS>
+// gs_row_zero over the live slots only ("zero_live"): Level::zmask's set
+// bits in ascending slot order, CH per chunk, with the chunk's unused
+// entries read through empty views (a * 0).  The dead slots' loads are not
+// issued at all: gs_row_zero issues every slot's loads, and a step with few
+// live slots then cost as many instructions as a full one.  CLS: the tile
+// class (0 32-bit ids, 1 all-affine 8-bit, 2 8-bit, 3 16-bit), decided once
+// per tile outside the loop.
+template <bool STREAM, int CLS, int CH>
+__device__ inline double zero_live_sum(const Col16 &C, uint32_t zm, int t, int64_t tb,
+                                       uint32_t lane, uint32_t xlim, int64_t n,
+                                       const int32_t *__restrict__ col,
+                                       const double *__restrict__ val, const double *x,
+                                       double sum) {
+  const int32_t *__restrict__ base = C.base + t;
+  uint32_t mm = zm;
+  while (mm) {
+    int js[CH];
+    bool ok[CH];
+#pragma unroll
+    for (int q = 0; q < CH; ++q) {
+      ok[q] = mm != 0u;
+      js[q] = ok[q] ? __builtin_ctz(mm) : 0;
+      mm &= mm - 1u;
+    }
+    // the chunk's index words and values, then its x values, then the sums
+    uint32_t w[CH];
+    double v[CH], xc[CH];
+#pragma unroll
+    for (int q = 0; q < CH; ++q) {
+      const int j = js[q];
+      if (CLS == 0)
+        w[q] = buf_u32<STREAM>(buf_view(col + (int64_t)j * n + tb, ok[q] ? 256u : 0u), lane * 4u);
+      else if (CLS == 2)
+        w[q] = buf_u32<STREAM>(buf_view(C.off8 + (int64_t)(j >> 2) * n + tb, ok[q] ? 256u : 0u),
+                               lane * 4u);
+      else if (CLS == 3)
+        w[q] = buf_u16<STREAM>(buf_view(C.off + (int64_t)j * n + tb, ok[q] ? 128u : 0u),
+                               lane * 2u);
+      v[q] = buf_f64<STREAM>(buf_view(val + (int64_t)j * n + tb, ok[q] ? 512u : 0u), lane * 8u);
+    }
+#pragma unroll
+    for (int q = 0; q < CH; ++q) {
+      const int j = js[q];
+      int32_t c;
+      if (CLS == 0)
+        c = (int32_t)w[q];
+      else if (CLS == 1)
+        c = base[(int64_t)j * C.ntiles] + (int32_t)lane;
+      else if (CLS == 2)
+        c = base[(int64_t)j * C.ntiles] + (int32_t)((w[q] >> (8 * (j & 3))) & 0xffu);
+      else
+        c = base[(int64_t)j * C.ntiles] + (int32_t)w[q];
+      xc[q] = buf_f64<false>(buf_view(x, ok[q] ? xlim : 0u), (uint32_t)c * 8u);
+    }
+#pragma unroll
+    for (int q = 0; q < CH; ++q) sum -= v[q] * xc[q];
+  }
+  return sum;
+}
+
+template <bool STREAM, int CH>
+__device__ inline void gs_row_zero_live(const Col16 &C, uint32_t zm, int64_t begin, int64_t i,
+                                        int64_t n, const int32_t *__restrict__ col,
+                                        const double *__restrict__ val,
+                                        const double *__restrict__ diag,
+                                        const double *__restrict__ r, double *x) {
+  // every wavefront's rows lie in one tile (colour blocks start on tiles in
+  // this layout, and workgroups are whole wavefronts), so the tile and its
+  // mask are uniform: the slot loop runs on scalar registers
+  const int t = __builtin_amdgcn_readfirstlane(tile_of(i));
+  zm = __builtin_amdgcn_readfirstlane(zm);
+  const int info = C.info[t];
+  const int64_t tb = (int64_t)t * kTile;
+  const uint32_t lane = (uint32_t)(i - tb);
+  const uint32_t xlim = (uint32_t)begin * 8u;
+  double sum = r[i];
+  if (info == kTile32)
+    sum = zero_live_sum<STREAM, 0, CH>(C, zm, t, tb, lane, xlim, n, col, val, x, sum);
+  else if (info == kTile8 && C.use8 && C.amask && C.amask[t] == (1u << kSlotWords) - 1u)
+    sum = zero_live_sum<STREAM, 1, CH>(C, zm, t, tb, lane, xlim, n, col, val, x, sum);
+  else if (info == kTile8 && C.use8)
+    sum = zero_live_sum<STREAM, 2, CH>(C, zm, t, tb, lane, xlim, n, col, val, x, sum);
+  else
+    sum = zero_live_sum<STREAM, 3, CH>(C, zm, t, tb, lane, xlim, n, col, val, x, sum);
+  const double d = mload<STREAM>(diag + i);
+  sum += 0.0 * d;
+  x[i] = sum / d;
+}
+
+// LIVE: 0 every slot (gs_row_zero), else the live slots in chunks of LIVE
+template <bool STREAM, bool FINEST, int BS, int LIVE = 0>
 __global__ __launch_bounds__(BS) void k_gs_zero16(
     int64_t begin, int64_t end, int64_t n, Col16 C, const uint32_t *__restrict__ zmask,
     const int32_t *__restrict__ col, const double *__restrict__ val,
     const double *__restrict__ diag, const double *__restrict__ r, double *x, bool xcd) {
   const int64_t i = begin + xcd_block(blockIdx.x, gridDim.x, xcd) * BS + threadIdx.x;
-  if (i < end) gs_row_zero<STREAM>(C, zmask[tile_of(i)], begin, i, n, col, val, diag, r, x);
+  if (i >= end) return;
+  if (LIVE)
+    gs_row_zero_live<STREAM, LIVE ? LIVE : 8>(C, zmask[tile_of(i)], begin, i, n, col, val, diag,
+                                              r, x);
+  else
+    gs_row_zero<STREAM>(C, zmask[tile_of(i)], begin, i, n, col, val, diag, r, x);
 }
 
 // Two adjacent rows per thread: the matrix is read with 16-byte (values) and
@@ -1222,7 +1317,20 @@ bool launch_symgs_rows_zero(const Problem &P, const Level &L, int64_t begin, int
   if (end <= begin) return true;
   const Col16 C = col16_of(P, L);
   const int64_t rows = end - begin;
-  if (L.finest) {
+  const bool live = P.zero_live & (L.finest ? 1 : 2);
+  const bool ch16 = P.zero_chunk == 16;
+#define HPCGX_GZL(ST, FI, CH)                                                                 \
+  k_gs_zero16<ST, FI, kBlock, CH><<<grid_for(rows), kBlock, 0, s>>>(begin, end, L.n, C, L.zmask, \
+                                                                    L.col, L.val, L.diag, r, x, \
+                                                                    L.xcd)
+  if (live) {
+    if (L.finest && ch16) HPCGX_GZL(true, true, 16);
+    else if (L.finest) HPCGX_GZL(true, true, 8);
+    else if (L.stream && ch16) HPCGX_GZL(true, false, 16);
+    else if (L.stream) HPCGX_GZL(true, false, 8);
+    else if (ch16) HPCGX_GZL(false, false, 16);
+    else HPCGX_GZL(false, false, 8);
+  } else if (L.finest) {
 #define HPCGX_GZ(BS)                                                                          \
   k_gs_zero16<true, true, BS><<<(unsigned)((rows + BS - 1) / BS), BS, 0, s>>>(                \
       begin, end, L.n, C, L.zmask, L.col, L.val, L.diag, r, x, L.xcd)
@@ -1240,6 +1348,7 @@ bool launch_symgs_rows_zero(const Problem &P, const Level &L, int64_t begin, int
     k_gs_zero16<false, false, kBlock><<<grid_for(rows), kBlock, 0, s>>>(
         begin, end, L.n, C, L.zmask, L.col, L.val, L.diag, r, x, L.xcd);
   }
+#undef HPCGX_GZL
   HPCGX_LAUNCHED();
   return true;
 }

@@ -365,6 +365,111 @@ __device__ inline void gs32_row_zero(const Col16 &C, uint32_t zm, int64_t begin,
   x[i] = sum / diag32[i];
 }
 
+// gs32_row_zero over the live slots only (tune key "zero_live"): Level::
+// zmask's set bits in ascending slot order, CH per chunk, with the
+// chunk's unused entries read through empty views (0 * 0).  Each live slot
+// loads its own offset word and value word, so the loads of dead slots are
+// not issued.  CLS: 0 32-bit ids, 1 packed words, 2 8-bit offsets with FP16
+// value pairs, 3 the same with every offset word affine (no offset loads);
+// the tile class is decided once outside the loop, and the chunk is
+// straight code (an affine word is a select, not a branch), so its loads
+// go out together.
+template <bool STREAM, int CLS, int CH>
+__device__ inline float zero_live_sum32(const Col16 &C, uint32_t zm, int t, int64_t tb,
+                                        uint32_t lane, uint32_t xlim, int64_t n,
+                                        const uint32_t *__restrict__ vp,
+                                        const uint32_t *__restrict__ vc,
+                                        const int32_t *__restrict__ col,
+                                        const half_t *__restrict__ val16, const float *x,
+                                        float sum) {
+  const int32_t *__restrict__ base = C.tbase + (int64_t)t * kBaseStride;
+  const unsigned am = (CLS == 2 && C.amask) ? C.amask[t] : 0u;  // affine offset words
+  uint32_t mm = zm;
+  while (mm) {
+    int js[CH];
+    bool ok[CH];
+#pragma unroll
+    for (int q = 0; q < CH; ++q) {
+      ok[q] = mm != 0u;
+      js[q] = ok[q] ? __builtin_ctz(mm) : 0;
+      mm &= mm - 1u;
+    }
+    // the chunk's index and value words, then its x values, then the sums
+    uint32_t wa[CH], wb[CH];
+#pragma unroll
+    for (int q = 0; q < CH; ++q) {
+      const int j = js[q];
+      if (CLS >= 2) {
+        // an affine offset word (every one for CLS 3) is not loaded
+        const bool aff = CLS == 3 || ((am >> (j >> 2)) & 1u);
+        wa[q] = CLS == 3 ? 0u
+                         : buf_u32<STREAM>(buf_view(C.off8 + (int64_t)(j >> 2) * n + tb,
+                                                    ok[q] && !aff ? 256u : 0u),
+                                           lane * 4u);
+        wb[q] = buf_u32<STREAM>(buf_view(vp + (int64_t)(j >> 1) * n + tb, ok[q] ? 256u : 0u),
+                                lane * 4u);
+      } else if (CLS == 1) {
+        wa[q] = buf_u32<STREAM>(buf_view(vc + (int64_t)j * n + tb, ok[q] ? 256u : 0u), lane * 4u);
+      } else {
+        wa[q] = buf_u32<STREAM>(buf_view(col + (int64_t)j * n + tb, ok[q] ? 256u : 0u), lane * 4u);
+        wb[q] = buf_u16<STREAM>(buf_view(val16 + (int64_t)j * n + tb, ok[q] ? 128u : 0u),
+                                lane * 2u);
+      }
+    }
+    float a[CH], xc[CH];
+#pragma unroll
+    for (int q = 0; q < CH; ++q) {
+      const int j = js[q];
+      int32_t c;
+      if (CLS >= 2) {
+        const bool aff = CLS == 3 || ((am >> (j >> 2)) & 1u);
+        const uint32_t o = aff ? lane * 0x01010101u : wa[q];
+        c = base[j] + (int32_t)((o >> (8 * (j & 3))) & 0xffu);
+        a[q] = (float)__builtin_bit_cast(half_t, (uint16_t)(wb[q] >> (16 * (j & 1))));
+      } else if (CLS == 1) {
+        c = base[j] + (int32_t)(wa[q] >> 16);
+        a[q] = (float)vc_val(wa[q]);
+      } else {
+        c = (int32_t)wa[q];
+        a[q] = (float)__builtin_bit_cast(half_t, (uint16_t)wb[q]);
+      }
+      xc[q] = buf_f32(buf_view(x, ok[q] ? xlim : 0u), (uint32_t)c * 4u);
+    }
+#pragma unroll
+    for (int q = 0; q < CH; ++q) sum -= a[q] * xc[q];
+  }
+  return sum;
+}
+
+template <bool STREAM, int CH>
+__device__ inline void gs32_row_zero_live(const Col16 &C, uint32_t zm, int64_t begin,
+                                          const uint32_t *__restrict__ vp,
+                                          const uint32_t *__restrict__ vc, int64_t i, int64_t n,
+                                          const int32_t *__restrict__ col,
+                                          const half_t *__restrict__ val16,
+                                          const float *__restrict__ diag32,
+                                          const float *__restrict__ r, float *x) {
+  // every wavefront's rows lie in one tile (colour blocks start on tiles in
+  // this layout, and workgroups are whole wavefronts), so the tile and its
+  // mask are uniform: the slot loop runs on scalar registers
+  const int t = __builtin_amdgcn_readfirstlane(tile_of(i));
+  zm = __builtin_amdgcn_readfirstlane(zm);
+  const int info = C.info[t];
+  const int64_t tb = (int64_t)t * kTile;
+  const uint32_t lane = (uint32_t)(i - tb);
+  const uint32_t xlim = (uint32_t)begin * 4u;
+  float sum = r[i];
+  if (info == kTile8 && C.use8 && vp && C.amask && C.amask[t] == (1u << kSlotWords) - 1u)
+    sum = zero_live_sum32<STREAM, 3, CH>(C, zm, t, tb, lane, xlim, n, vp, vc, col, val16, x, sum);
+  else if (info == kTile8 && C.use8 && vp)
+    sum = zero_live_sum32<STREAM, 2, CH>(C, zm, t, tb, lane, xlim, n, vp, vc, col, val16, x, sum);
+  else if (info != kTile32)
+    sum = zero_live_sum32<STREAM, 1, CH>(C, zm, t, tb, lane, xlim, n, vp, vc, col, val16, x, sum);
+  else
+    sum = zero_live_sum32<STREAM, 0, CH>(C, zm, t, tb, lane, xlim, n, vp, vc, col, val16, x, sum);
+  x[i] = sum / diag32[i];
+}
+
 // v16p[w*n + i] = bits of val16[(2w)*n + i] | bits of val16[(2w+1)*n + i] << 16
 // (0 above the last slot), one thread per row.
 __global__ __launch_bounds__(kBlock) void k_pack_v16p(int64_t n, const half_t *__restrict__ val16,
@@ -399,14 +504,19 @@ __global__ __launch_bounds__(kBlock) void k_gs32_range8(
   if (i < end) gs32_row8<STREAM, PAIRS>(C, vp, vc, i, n, col, val16, diag32, r, x, end);
 }
 
-template <bool STREAM, bool FINEST>
+// LIVE: 0 every slot (gs32_row_zero), else the live slots in chunks of LIVE
+template <bool STREAM, bool FINEST, int LIVE = 0>
 __global__ __launch_bounds__(kBlock) void k_gs32_zero(
     int64_t begin, int64_t end, int64_t n, Col16 C, const uint32_t *__restrict__ zmask,
     const uint32_t *__restrict__ vp, const uint32_t *__restrict__ vc,
     const int32_t *__restrict__ col, const half_t *__restrict__ val16,
     const float *__restrict__ diag32, const float *__restrict__ r, float *x, bool xcd) {
   const int64_t i = begin + xcd_block(blockIdx.x, gridDim.x, xcd) * kBlock + threadIdx.x;
-  if (i < end)
+  if (i >= end) return;
+  if (LIVE)
+    gs32_row_zero_live<STREAM, LIVE ? LIVE : 8>(C, zmask[tile_of(i)], begin, vp, vc, i, n, col,
+                                                val16, diag32, r, x);
+  else
     gs32_row_zero<STREAM>(C, zmask[tile_of(i)], begin, vp, vc, i, n, col, val16, diag32, r, x);
 }
 
@@ -623,15 +733,20 @@ bool gs32_step_zero(const Problem &P, const Level &L, int64_t begin, int64_t end
   const unsigned g = grid_for(end - begin);
   const Col16 C = col16_of(P, L);
   const uint32_t *vp = (L.v16p && P.use_pairs && P.use_c8) ? L.v16p : nullptr;
-  if (L.finest)
-    k_gs32_zero<true, true><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.zmask, vp, L.vc, L.col,
-                                                 L.val16, L.diag32, r, x, L.xcd);
-  else if (L.stream)
-    k_gs32_zero<true, false><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.zmask, vp, L.vc, L.col,
-                                                  L.val16, L.diag32, r, x, L.xcd);
-  else
-    k_gs32_zero<false, false><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.zmask, vp, L.vc, L.col,
-                                                   L.val16, L.diag32, r, x, L.xcd);
+#define HPCGX_GZ32(ST, FI, LI)                                                                \
+  k_gs32_zero<ST, FI, LI><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.zmask, vp, L.vc, L.col,    \
+                                               L.val16, L.diag32, r, x, L.xcd)
+  const int live = !(P.zero_live & 4) ? 0 : P.zero_chunk == 16 ? 16 : 8;
+  if (L.finest && live == 16) HPCGX_GZ32(true, true, 16);
+  else if (L.finest && live) HPCGX_GZ32(true, true, 8);
+  else if (L.finest) HPCGX_GZ32(true, true, 0);
+  else if (L.stream && live == 16) HPCGX_GZ32(true, false, 16);
+  else if (L.stream && live) HPCGX_GZ32(true, false, 8);
+  else if (L.stream) HPCGX_GZ32(true, false, 0);
+  else if (live == 16) HPCGX_GZ32(false, false, 16);
+  else if (live) HPCGX_GZ32(false, false, 8);
+  else HPCGX_GZ32(false, false, 0);
+#undef HPCGX_GZ32
   HPCGX_LAUNCHED();
   return true;
 }

@@ -393,6 +393,13 @@ int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, i
  *                      and x values of colours already updated in that
  *                      sweep (default 1; the same x as a zero fill followed
  *                      by the full sweep)
+ *   "zero_live"        0..15: the zero-x colour steps (see "zero_guess")
+ *                      issue loads for the live slots only, eight per chunk,
+ *                      instead of every slot through empty views; bit 0 the
+ *                      FP64 finest level, bit 1 the FP64 coarser levels,
+ *                      bit 2 the FP32 V-cycle, bit 3 the block V-cycle
+ *                      (default 11; the same x)
+ *   "zero_chunk"       8 or 16: live slots per chunk of those steps (default 8)
  *   "defer_x"          1/0: hpcgx_cg takes each iteration's x += alpha p
  *                      with the next iteration's p update, in one pass over
  *                      x and p, and the last one after the loop (default 1;

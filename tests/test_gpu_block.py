@@ -195,8 +195,9 @@ def test_block_pairs_same_bits(hp, s):
 @pytest.mark.parametrize("s", [4, 8, 16])
 def test_block_zero_guess_same_bits(hp, s):
     """The block V-cycle's first forward sweep from X = 0 without a fill
-    (zero_guess) gives the bits of the fill followed by the full sweep: the
-    block V-cycle and 12 block CG iterations, with zero_guess on and off."""
+    (zero_guess), over every slot and over the live slots only (zero_live),
+    gives the bits of the fill followed by the full sweep: the block V-cycle
+    and 12 block CG iterations."""
     import torch
     dims = (32, 48, 64)
     say("block zero guess", dims, s)
@@ -206,17 +207,25 @@ def test_block_zero_guess_same_bits(hp, s):
     B = P.to_device_order_multi(rhs(n, s, 42), 0)
     out = {}
     try:
-        for zg in (0, 1):
+        # (zero_guess, zero_live): the filled sweep, the zero-x sweep over
+        # every slot, and the zero-x sweep over the live slots only
+        for zg, live, chunk in ((0, 0, 8), (1, 0, 8), (1, 8, 8), (1, 8, 16)):
+            say("zero_guess", zg, "zero_live", live, "zero_chunk", chunk)
             P.tune("zero_guess", zg)
+            P.tune("zero_live", live)
+            P.tune("zero_chunk", chunk)
             Z = torch.full_like(R, float("nan"))  # the zero-x sweep must not read it
             if not zg:
                 Z.zero_()
             P.mg32_block(R, Z, ordering="multicolor")
             X = torch.zeros_like(B)
             P.cg_block_mixed(B, X, max_iter=12, tolerance=0.0, recompute=5)
-            out[zg] = (Z, X)
+            out[(zg, live, chunk)] = (Z, X)
     finally:
         P.tune("zero_guess", 1)
-    assert torch.equal(out[0][0], out[1][0])
-    assert torch.equal(out[0][1], out[1][1])
+        P.tune("zero_live", 11)
+        P.tune("zero_chunk", 8)
+    for k in ((1, 0, 8), (1, 8, 8), (1, 8, 16)):
+        assert torch.equal(out[(0, 0, 8)][0], out[k][0]), k
+        assert torch.equal(out[(0, 0, 8)][1], out[k][1]), k
     del P

@@ -818,6 +818,7 @@ def test_256_properties(hp, torch):
 # (knob, value, default): each changes how the iteration is scheduled or
 # shaped, never its arithmetic
 SAME_BITS_KNOBS = [("zero_guess", 0, 1), ("defer_x", 0, 1), ("x_pairs", 0, 3), ("x_pairs", 7, 3),
+                   ("zero_live", 0, 11), ("zero_live", 15, 11), ("zero_chunk", 16, 8),
                    ("gs_block", 64, 256),
                    ("gs_block", 128, 256), ("gs_block", 512, 256), ("gs_waves", 0, 1)]
 
@@ -901,6 +902,57 @@ def test_x_pairs_same_bits(hp, torch):
         assert m["normr"] == m0["normr"] and torch.equal(x, xm0)
     finally:
         P.tune("x_pairs", 3)
+    del P
+
+
+def test_zero_live_same_bits(hp, torch):
+    """zero_live (the zero-x first sweeps' loads issued for the live slots
+    only, 8 or 16 per chunk) changes no bit with 8-bit affine, 8-bit and
+    16-bit column offsets: the FP64 and FP32 V-cycles equal the oracle's,
+    and 12 CG and 12 mixed CG iterations give the x and residual of the run
+    without it."""
+    dims = (256, 16, 24)
+    say("zero_live", dims)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    P.prepare_mixed()
+    h = hier(dims, color=True)
+    rr = rnd(97, P.nrow(0))
+    want_z = h.mg(rr)
+    want_z32 = h.mg32(rr.astype(np.float32))
+    r32 = P.to_device_order(rr).float()
+    b, x, _ = P.vectors()
+    try:
+        for c8, a8 in ((1, 1), (1, 0), (0, 1)):
+            P.tune("col8", c8)
+            P.tune("affine8", a8)
+            # the mixed outer iteration's bits depend on the column format,
+            # so each format's runs are compared with zero_live off in it
+            ref = None
+            for live, chunk in ((0, 8), (15, 8), (15, 16)):
+                say("zero_live", live, "zero_chunk", chunk, "col8", c8, "affine8", a8)
+                P.tune("zero_live", live)
+                P.tune("zero_chunk", chunk)
+                z = P.zeros()
+                P.mg(P.to_device_order(rr), z, ordering="multicolor")
+                assert np.array_equal(P.to_numpy(z), want_z), (c8, a8, live)
+                z32 = P.zeros32()
+                P.mg32(r32, z32)
+                assert np.array_equal(P.to_numpy(z32.double()).astype(np.float32),
+                                      want_z32), (c8, a8, live)
+                x.zero_()
+                r = P.cg(b, x, max_iter=12, tolerance=0.0)
+                xr = x.clone()
+                x.zero_()
+                m = P.cg_mixed(b, x, max_iter=12, tolerance=0.0, recompute=5)
+                got = (r["normr"], xr, m["normr"], x.clone())
+                if ref is None:
+                    ref = got
+                    continue
+                assert got[0] == ref[0] and torch.equal(got[1], ref[1]), (c8, a8, live)
+                assert got[2] == ref[2] and torch.equal(got[3], ref[3]), (c8, a8, live)
+    finally:
+        for k, v in (("col8", 1), ("affine8", 1), ("zero_live", 11), ("zero_chunk", 8)):
+            P.tune(k, v)
     del P
 
 
