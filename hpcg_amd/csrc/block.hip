@@ -656,14 +656,14 @@ __global__ __launch_bounds__(kBlock) void k_bspmv_chunks(
 __global__ __launch_bounds__(kBlock) void k_chunk_bdot(int64_t nparts, int s,
                                                        const double *__restrict__ cpart,
                                                        double *partials, unsigned *ticket,
-                                                       double *sc) {
+                                                       double *sc, int finish) {
   const int64_t per = (nparts + gridDim.x - 1) / gridDim.x;
   const int64_t lo = (int64_t)blockIdx.x * per;
   const int64_t hi = lo + per < nparts ? lo + per : nparts;
   const int k = threadIdx.x % s, step = kBlock / s;
   double t = 0.0;
   for (int64_t p = lo + threadIdx.x / s; p < hi; p += step) t += cpart[p * s + k];
-  block_finish_s(t, s, partials, ticket, sc, kBAlpha);
+  block_finish_s(t, s, partials, ticket, sc, finish);
 }
 
 // ---- FP64 outer kernels ----------------------------------------------------
@@ -755,6 +755,76 @@ __global__ __launch_bounds__(kBlock) void k_bupdate_xr(int64_t total, int s,
     acc += r * r;
   }
   block_finish_s(acc, s, partials, ticket, sc, kBStore);
+}
+
+// ---- the vector kernels, kBE elements per thread ("xr_rows") -------------
+// Workgroup w takes elements [w * kBlock * kBE, ...) of the [n][s]
+// multivectors, thread q the elements w * kBlock * kBE + q + e * kBlock; its
+// column is q % s (s divides 64).  Each workgroup stores its column sums to
+// cpart[w * s + k] with a plain store, and k_chunk_bdot, a launch of its
+// own, sums them in a fixed order and finishes.  The grid-stride forms above
+// keep a small grid and the last workgroup's finish; these launch a
+// workgroup per kBlock * kBE elements, so more loads are in flight.
+constexpr int kBE = 4;
+
+// Column sums of a workgroup in a fixed order: within each wavefront a
+// shuffle tree over the lanes of a column (lanes k, k + s, ...), then the
+// wavefronts in order.  Valid in threads k < s.
+__device__ inline double bwave_column_sum(double v, int s) {
+  __shared__ double red[kBlock / 64][64];
+  for (int off = 32; off >= s; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane < s) red[w][lane] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x < (unsigned)s) {
+#pragma unroll
+    for (int q = 0; q < kBlock / 64; ++q) t += red[q][threadIdx.x];
+  }
+  return t;
+}
+
+// r_k . z_k per workgroup (k_bdot32's sums)
+__global__ __launch_bounds__(kBlock) void k_bdot32_rows(int64_t total, int s,
+                                                        const double *__restrict__ R,
+                                                        const float *__restrict__ Z32,
+                                                        double *__restrict__ cpart) {
+  const int64_t t0 = (int64_t)blockIdx.x * (kBlock * kBE) + threadIdx.x;
+  double acc = 0.0;
+#pragma unroll
+  for (int e = 0; e < kBE; ++e) {
+    const int64_t t = t0 + (int64_t)e * kBlock;
+    if (t < total) acc += R[t] * (double)Z32[t];
+  }
+  const double c = bwave_column_sum(acc, s);
+  if (threadIdx.x < (unsigned)s) cpart[(int64_t)blockIdx.x * s + threadIdx.x] = c;
+}
+
+// k_bupdate_xr's update, |r|^2 per workgroup
+__global__ __launch_bounds__(kBlock) void k_bupdate_xr_rows(int64_t total, int s,
+                                                            double *__restrict__ X,
+                                                            double *__restrict__ R,
+                                                            const double *__restrict__ Pv,
+                                                            const double *__restrict__ AP,
+                                                            float *__restrict__ R32,
+                                                            const double *__restrict__ sc,
+                                                            double *__restrict__ cpart) {
+  const int64_t t0 = (int64_t)blockIdx.x * (kBlock * kBE) + threadIdx.x;
+  const double alpha = sc[2 * s + (int)(threadIdx.x % s)];
+  double acc = 0.0;
+#pragma unroll
+  for (int e = 0; e < kBE; ++e) {
+    const int64_t t = t0 + (int64_t)e * kBlock;
+    if (t < total) {
+      X[t] = X[t] + alpha * Pv[t];
+      const double r = R[t] + (-alpha) * AP[t];
+      R[t] = r;
+      R32[t] = (float)r;
+      acc += r * r;
+    }
+  }
+  const double c = bwave_column_sum(acc, s);
+  if (threadIdx.x < (unsigned)s) cpart[(int64_t)blockIdx.x * s + threadIdx.x] = c;
 }
 
 // ---- FP32 V-cycle kernels (mixed.hip's arithmetic, s vectors) ------------
@@ -969,7 +1039,8 @@ void spmv_rows_s(const Problem &P, const Level &L, const double *X, double *Y, b
     if (dot) {
       const int64_t nparts = L.nchunks * W;
       const unsigned R = (unsigned)std::min<int64_t>(256, nparts);
-      k_chunk_bdot<<<R, kBlock, 0, st>>>(nparts, S, B.cpart, B.partials, B.ticket, B.scalars);
+      k_chunk_bdot<<<R, kBlock, 0, st>>>(nparts, S, B.cpart, B.partials, B.ticket, B.scalars,
+                                         kBAlpha);
       HPCGX_LAUNCHED();
     }
     return;
@@ -1209,14 +1280,32 @@ void block_iteration(Problem &P, int ordering, int s, double *X, bool first, hip
   auto &B = P.blk;
   const int64_t total = P.L[0].n * s;
   const unsigned gr = block_reduce_grid(total, s);
+  // the kBE-per-thread vector kernels where s divides 64 and cpart holds
+  // their column sums
+  const int64_t nb = (total + kBlock * kBE - 1) / (kBlock * kBE);
+  const bool rows = P.xr_rows && B.cpart && 64 % s == 0 && nb * s <= B.ncpart;
+  const unsigned R = (unsigned)std::min<int64_t>(256, nb);
   mg32_block(P, 0, ordering, s, B.R32, B.Z32, st);
-  k_bdot32<<<gr, kBlock, 0, st>>>(total, s, B.R, B.Z32, B.partials, B.ticket, B.scalars);
+  if (rows) {
+    k_bdot32_rows<<<(unsigned)nb, kBlock, 0, st>>>(total, s, B.R, B.Z32, B.cpart);
+    HPCGX_LAUNCHED();
+    k_chunk_bdot<<<R, kBlock, 0, st>>>(nb, s, B.cpart, B.partials, B.ticket, B.scalars, kBBeta);
+  } else {
+    k_bdot32<<<gr, kBlock, 0, st>>>(total, s, B.R, B.Z32, B.partials, B.ticket, B.scalars);
+  }
   HPCGX_LAUNCHED();
   k_bupdate_p32<<<grid_for(total), kBlock, 0, st>>>(total, s, B.Z32, B.Pv, B.scalars, first);
   HPCGX_LAUNCHED();
   block_spmv_dot(P, s, B.Pv, B.AP, true, st);
-  k_bupdate_xr<<<gr, kBlock, 0, st>>>(total, s, X, B.R, B.Pv, B.AP, B.R32, B.partials, B.ticket,
-                                      B.scalars);
+  if (rows) {
+    k_bupdate_xr_rows<<<(unsigned)nb, kBlock, 0, st>>>(total, s, X, B.R, B.Pv, B.AP, B.R32,
+                                                       B.scalars, B.cpart);
+    HPCGX_LAUNCHED();
+    k_chunk_bdot<<<R, kBlock, 0, st>>>(nb, s, B.cpart, B.partials, B.ticket, B.scalars, kBStore);
+  } else {
+    k_bupdate_xr<<<gr, kBlock, 0, st>>>(total, s, X, B.R, B.Pv, B.AP, B.R32, B.partials, B.ticket,
+                                        B.scalars);
+  }
   HPCGX_LAUNCHED();
 }
 

@@ -430,7 +430,10 @@ int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, i
  *                      per-workgroup partials in a second launch, instead
  *                      of a grid-stride kernel that finishes r.r in its
  *                      last workgroup (default 1).  Changes r.r's summation
- *                      order.
+ *                      order.  In the block PCG (s dividing 64) it also
+ *                      selects the form for r_k.z_k and the r update, four
+ *                      elements per thread, which changes the order of
+ *                      those sums.
  *   "spmv_dot_blocks"  workgroups of the persistent SpMV + dot kernel
  *                      (8..8192, rounded down to a multiple of 8; default
  *                      4096).  Changes the dot's summation order.

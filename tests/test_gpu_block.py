@@ -229,3 +229,33 @@ def test_block_zero_guess_same_bits(hp, s):
         assert torch.equal(out[(0, 0, 8)][0], out[k][0]), k
         assert torch.equal(out[(0, 0, 8)][1], out[k][1]), k
     del P
+
+
+@pytest.mark.parametrize("s", [4, 16])
+def test_block_xr_rows(hp, s):
+    """The block r.z and r update with four elements per thread and
+    per-workgroup column sums (xr_rows) follow the grid-stride forms'
+    trajectory to rounding (the sums are reassociated), and the captured
+    iteration gives the eager launches' bits."""
+    import torch
+    dims = (32, 48, 64)
+    say("block xr_rows", dims, s)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    B = P.to_device_order_multi(rhs(P.nrow(0), s, 42), 0)
+    out = {}
+    try:
+        for v, graph in ((0, 0), (1, 0), (1, 1)):
+            P.tune("xr_rows", v)
+            P.tune("mixed_graph", graph)
+            X = torch.zeros_like(B)
+            r = P.cg_block_mixed(B, X, max_iter=15, tolerance=0.0, recompute=5)
+            out[(v, graph)] = (r, X)
+    finally:
+        P.tune("xr_rows", 1)
+        P.tune("mixed_graph", 1)
+    (r0, X0), (r1, X1), (rg, Xg) = out[(0, 0)], out[(1, 0)], out[(1, 1)]
+    for k in range(s):
+        assert r1["normr"][k] == pytest.approx(r0["normr"][k], rel=1e-9), k
+    assert float((X1 - X0).abs().max()) <= 1e-9 * float(X0.abs().max())
+    assert rg["normr"] == r1["normr"] and torch.equal(Xg, X1)
+    del P
