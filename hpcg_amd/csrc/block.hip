@@ -368,8 +368,8 @@ __device__ __attribute__((always_inline)) inline void bgs32_row_zero_live(
 
 // A colour step of the first forward sweep from X = 0, rows [begin, end)
 // (begin a multiple of kTile).
-// LIVE: 0 every slot (bgs32_row_zero), else the live slots in chunks of LIVE
-template <bool STREAM, int S, int LIVE = 0>
+// LIVE: the live slots only (bgs32_row_zero_live), else every slot (bgs32_row_zero)
+template <bool STREAM, int S, bool LIVE = false>
 __global__ __launch_bounds__(kBlock) void k_bgs32_rows_zero(
     int64_t begin, int64_t end, int64_t n, Col16 C, const uint32_t *__restrict__ zmask,
     const uint32_t *__restrict__ vc, const int32_t *__restrict__ col,
@@ -380,8 +380,8 @@ __global__ __launch_bounds__(kBlock) void k_bgs32_rows_zero(
   const int64_t i = begin + g / W;
   if (i >= end) return;
   if (LIVE)
-    bgs32_row_zero_live<STREAM, S, LIVE ? LIVE : 8>(zmask[tile_of(i)], begin, i, (int)(g % W) * 4,
-                                                    n, C, vc, col, val16, diag32, R, X);
+    bgs32_row_zero_live<STREAM, S, 8>(zmask[tile_of(i)], begin, i, (int)(g % W) * 4, n, C, vc,
+                                      col, val16, diag32, R, X);
   else
     bgs32_row_zero<STREAM, S>(zmask[tile_of(i)], begin, i, (int)(g % W) * 4, n, C, vc, col,
                               val16, diag32, R, X);
@@ -952,13 +952,11 @@ void gs_rows_zero_s(const Problem &P, const Level &L, int64_t b, int64_t e, cons
 #define HPCGX_BZ(ST, LI)                                                                      \
   k_bgs32_rows_zero<ST, S, LI><<<g, kBlock, 0, st>>>(b, e, L.n, C, L.zmask, L.vc, L.col,        \
                                                      L.val16, L.diag32, R, X)
-  const int live = !(P.zero_live & 8) ? 0 : P.zero_chunk == 16 ? 16 : 8;
-  if (L.stream && live == 16) HPCGX_BZ(true, 16);
-  else if (L.stream && live) HPCGX_BZ(true, 8);
-  else if (L.stream) HPCGX_BZ(true, 0);
-  else if (live == 16) HPCGX_BZ(false, 16);
-  else if (live) HPCGX_BZ(false, 8);
-  else HPCGX_BZ(false, 0);
+  const bool live = P.zero_live & 8;
+  if (L.stream && live) HPCGX_BZ(true, true);
+  else if (L.stream) HPCGX_BZ(true, false);
+  else if (live) HPCGX_BZ(false, true);
+  else HPCGX_BZ(false, false);
 #undef HPCGX_BZ
   HPCGX_LAUNCHED();
 }

@@ -419,11 +419,6 @@ int hpcgx_tune(hpcgx_problem *p, const char *key, double value) {
       P.gs_block = (int)value;
       return;
     }
-    if (k == "zero_chunk") {
-      if (value != 8 && value != 16) throw Error(HPCGX_EINVAL, "zero_chunk must be 8 or 16");
-      P.zero_chunk = (int)value;
-      return;
-    }
     if (k == "zero_live") {
       if (value < 0 || value > 15 || value != (int)value)
         throw Error(HPCGX_EINVAL, "zero_live must be an integer from 0 to 15");

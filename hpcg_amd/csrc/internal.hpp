@@ -244,7 +244,6 @@ struct Problem {
   // bit 0 the FP64 finest level, bit 1 the FP64 coarser levels, bit 2 the
   // FP32 V-cycle, bit 3 the block V-cycle
   int zero_live = 11;
-  int zero_chunk = 8;  // live slots per chunk of those steps ("zero_chunk", 8 or 16)
   // block PCG finest colour steps: slot 3m + 2's x rows from slot 3m's by
   // W-lane moves on affine tiles ("block_pairs")
   bool block_pairs = false;

@@ -504,8 +504,8 @@ __global__ __launch_bounds__(kBlock) void k_gs32_range8(
   if (i < end) gs32_row8<STREAM, PAIRS>(C, vp, vc, i, n, col, val16, diag32, r, x, end);
 }
 
-// LIVE: 0 every slot (gs32_row_zero), else the live slots in chunks of LIVE
-template <bool STREAM, bool FINEST, int LIVE = 0>
+// LIVE: the live slots only (gs32_row_zero_live), else every slot (gs32_row_zero)
+template <bool STREAM, bool FINEST, bool LIVE = false>
 __global__ __launch_bounds__(kBlock) void k_gs32_zero(
     int64_t begin, int64_t end, int64_t n, Col16 C, const uint32_t *__restrict__ zmask,
     const uint32_t *__restrict__ vp, const uint32_t *__restrict__ vc,
@@ -514,8 +514,8 @@ __global__ __launch_bounds__(kBlock) void k_gs32_zero(
   const int64_t i = begin + xcd_block(blockIdx.x, gridDim.x, xcd) * kBlock + threadIdx.x;
   if (i >= end) return;
   if (LIVE)
-    gs32_row_zero_live<STREAM, LIVE ? LIVE : 8>(C, zmask[tile_of(i)], begin, vp, vc, i, n, col,
-                                                val16, diag32, r, x);
+    gs32_row_zero_live<STREAM, 8>(C, zmask[tile_of(i)], begin, vp, vc, i, n, col, val16, diag32,
+                                  r, x);
   else
     gs32_row_zero<STREAM>(C, zmask[tile_of(i)], begin, vp, vc, i, n, col, val16, diag32, r, x);
 }
@@ -736,16 +736,13 @@ bool gs32_step_zero(const Problem &P, const Level &L, int64_t begin, int64_t end
 #define HPCGX_GZ32(ST, FI, LI)                                                                \
   k_gs32_zero<ST, FI, LI><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.zmask, vp, L.vc, L.col,    \
                                                L.val16, L.diag32, r, x, L.xcd)
-  const int live = !(P.zero_live & 4) ? 0 : P.zero_chunk == 16 ? 16 : 8;
-  if (L.finest && live == 16) HPCGX_GZ32(true, true, 16);
-  else if (L.finest && live) HPCGX_GZ32(true, true, 8);
-  else if (L.finest) HPCGX_GZ32(true, true, 0);
-  else if (L.stream && live == 16) HPCGX_GZ32(true, false, 16);
-  else if (L.stream && live) HPCGX_GZ32(true, false, 8);
-  else if (L.stream) HPCGX_GZ32(true, false, 0);
-  else if (live == 16) HPCGX_GZ32(false, false, 16);
-  else if (live) HPCGX_GZ32(false, false, 8);
-  else HPCGX_GZ32(false, false, 0);
+  const bool live = P.zero_live & 4;
+  if (L.finest && live) HPCGX_GZ32(true, true, true);
+  else if (L.finest) HPCGX_GZ32(true, true, false);
+  else if (L.stream && live) HPCGX_GZ32(true, false, true);
+  else if (L.stream) HPCGX_GZ32(true, false, false);
+  else if (live) HPCGX_GZ32(false, false, true);
+  else HPCGX_GZ32(false, false, false);
 #undef HPCGX_GZ32
   HPCGX_LAUNCHED();
   return true;

@@ -399,7 +399,6 @@ int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, i
  *                      FP64 finest level, bit 1 the FP64 coarser levels,
  *                      bit 2 the FP32 V-cycle, bit 3 the block V-cycle
  *                      (default 11; the same x)
- *   "zero_chunk"       8 or 16: live slots per chunk of those steps (default 8)
  *   "defer_x"          1/0: hpcgx_cg takes each iteration's x += alpha p
  *                      with the next iteration's p update, in one pass over
  *                      x and p, and the last one after the loop (default 1;

@@ -818,7 +818,7 @@ def test_256_properties(hp, torch):
 # (knob, value, default): each changes how the iteration is scheduled or
 # shaped, never its arithmetic
 SAME_BITS_KNOBS = [("zero_guess", 0, 1), ("defer_x", 0, 1), ("x_pairs", 0, 3), ("x_pairs", 7, 3),
-                   ("zero_live", 0, 11), ("zero_live", 15, 11), ("zero_chunk", 16, 8),
+                   ("zero_live", 0, 11), ("zero_live", 15, 11),
                    ("gs_block", 64, 256),
                    ("gs_block", 128, 256), ("gs_block", 512, 256), ("gs_waves", 0, 1)]
 
@@ -907,7 +907,7 @@ def test_x_pairs_same_bits(hp, torch):
 
 def test_zero_live_same_bits(hp, torch):
     """zero_live (the zero-x first sweeps' loads issued for the live slots
-    only, 8 or 16 per chunk) changes no bit with 8-bit affine, 8-bit and
+    only, eight per chunk) changes no bit with 8-bit affine, 8-bit and
     16-bit column offsets: the FP64 and FP32 V-cycles equal the oracle's,
     and 12 CG and 12 mixed CG iterations give the x and residual of the run
     without it."""
@@ -928,10 +928,9 @@ def test_zero_live_same_bits(hp, torch):
             # the mixed outer iteration's bits depend on the column format,
             # so each format's runs are compared with zero_live off in it
             ref = None
-            for live, chunk in ((0, 8), (15, 8), (15, 16)):
-                say("zero_live", live, "zero_chunk", chunk, "col8", c8, "affine8", a8)
+            for live in (0, 15):
+                say("zero_live", live, "col8", c8, "affine8", a8)
                 P.tune("zero_live", live)
-                P.tune("zero_chunk", chunk)
                 z = P.zeros()
                 P.mg(P.to_device_order(rr), z, ordering="multicolor")
                 assert np.array_equal(P.to_numpy(z), want_z), (c8, a8, live)
@@ -951,7 +950,7 @@ def test_zero_live_same_bits(hp, torch):
                 assert got[0] == ref[0] and torch.equal(got[1], ref[1]), (c8, a8, live)
                 assert got[2] == ref[2] and torch.equal(got[3], ref[3]), (c8, a8, live)
     finally:
-        for k, v in (("col8", 1), ("affine8", 1), ("zero_live", 11), ("zero_chunk", 8)):
+        for k, v in (("col8", 1), ("affine8", 1), ("zero_live", 11)):
             P.tune(k, v)
     del P
 
