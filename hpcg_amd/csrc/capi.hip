@@ -434,8 +434,8 @@ int hpcgx_tune(hpcgx_problem *p, const char *key, double value) {
       return;
     }
     if (k == "x_pairs") {
-      if (value < 0 || value > 7 || value != (int)value)
-        throw Error(HPCGX_EINVAL, "x_pairs must be an integer from 0 to 7");
+      if (value < 0 || value > 15 || value != (int)value)
+        throw Error(HPCGX_EINVAL, "x_pairs must be an integer from 0 to 15");
       P.x_pairs = (int)value;
       return;
     }

@@ -1278,6 +1278,9 @@ void launch_symgs_rows(const Problem &P, const Level &L, int64_t begin, int64_t 
     if (dpart)
       k_gs_rz16<true, kBlock, 6><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col, L.val, r, x,
                                                       L.xcd, dpart);
+    else if (L.stream && (P.x_pairs & 8))  // bit 3: the coarser levels' steps
+      k_gs_range16<true, false, kBlock, 7, true><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col,
+                                                                      L.val, r, x, L.xcd);
     else if (L.stream)
       k_gs_range16<true, false, kBlock, 7><<<g, kBlock, 0, s>>>(begin, end, L.n, C, L.col,
                                                                 L.val, r, x, L.xcd);

@@ -408,7 +408,7 @@ int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, i
  *   "gs_waves"         1/0: the finest level's FP64 colour steps run under
  *                      register caps that allow 7 waves per SIMD (default
  *                      1); 0 leaves the register count to the compiler
- *   "x_pairs"          0-7: on tiles whose offset words are all affine and
+ *   "x_pairs"          0-15: on tiles whose offset words are all affine and
  *                      whose slot 3m + 2 has the base one above slot 3m's
  *                      (all m), the finest level's kernels take slot 3m +
  *                      2's x values from slot 3m's moved one lane
@@ -418,7 +418,8 @@ int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, i
  *                      caps), the chunked SpMV + dot and the fused residual
  *                      and restriction; bit 1: the FP64 colour step that
  *                      leaves r.z partials; bit 2: the FP32 V-cycle's colour
- *                      steps on FP16 value pairs (default 3; the same bits)
+ *                      steps on FP16 value pairs; bit 3: the FP64 colour
+ *                      steps of the coarser levels (default 3; the same bits)
  *   "block_pairs"      1/0: the block PCG's finest colour steps at s = 8,
  *                      on tiles where "x_pairs" applies, take slot 3m + 2's
  *                      neighbour rows from slot 3m's moved one row (2

@@ -817,7 +817,7 @@ def test_256_properties(hp, torch):
 
 # (knob, value, default): each changes how the iteration is scheduled or
 # shaped, never its arithmetic
-SAME_BITS_KNOBS = [("zero_guess", 0, 1), ("defer_x", 0, 1), ("x_pairs", 0, 3), ("x_pairs", 7, 3),
+SAME_BITS_KNOBS = [("zero_guess", 0, 1), ("defer_x", 0, 1), ("x_pairs", 0, 3), ("x_pairs", 15, 3),
                    ("zero_live", 0, 11), ("zero_live", 15, 11),
                    ("gs_block", 64, 256),
                    ("gs_block", 128, 256), ("gs_block", 512, 256), ("gs_waves", 0, 1)]
@@ -885,7 +885,7 @@ def test_x_pairs_same_bits(hp, torch):
     x.zero_()
     m0 = P.cg_mixed(b, x, max_iter=12, tolerance=0.0, recompute=5)
     xm0 = x.clone()
-    P.tune("x_pairs", 7)
+    P.tune("x_pairs", 15)
     try:
         z = P.zeros()
         P.mg(P.to_device_order(rr), z, ordering="multicolor")
