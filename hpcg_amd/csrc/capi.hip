@@ -101,6 +101,7 @@ void alloc_solver_state(hpcgx::Problem &P) {
   // most one workgroup per colour more than n / kBlock (up to 64 colours)
   P.nrz_partials = (n + hpcgx::kTile - 1) / hpcgx::kTile + 64;  // one per wavefront of rows
   P.rz_part = P.arena.alloc<double>(P.nrz_partials);
+  P.dot_parts = P.arena.alloc<double>(hpcgx::kDotParts);
 }
 
 // shared tail of both setup paths: schedules, reordering, links

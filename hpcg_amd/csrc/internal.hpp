@@ -259,6 +259,7 @@ struct Problem {
   bool rz_fused = true;
   int64_t nrz_partials = 0;
   double *rz_part = nullptr;          // [nrz_partials]
+  double *dot_parts = nullptr;        // [kDotParts]: launch_dot's first pass
   bool mixed_ready = false;   // FP16/FP32 copies built
   bool half_exact = false;    // every off-diagonal value is exact in FP16
   float *r32 = nullptr, *z32 = nullptr;  // FP32 input / output of the V-cycle
@@ -420,11 +421,18 @@ void launch_waxpby(int64_t n, double alpha, const double *x, double beta,
 enum DotFinish { kFinishStore = 0, kFinishBeta = 1, kFinishAlpha = 2 };
 void launch_dot(Problem &P, int64_t n, const double *x, const double *y,
                 double *out, int finish, hipStream_t s);
-// launch_dot with explicit scratch: partials (dot_partials(n) doubles),
-// the completion ticket (0 on entry, 0 again on exit) and the scalars
-void launch_dot_raw(int64_t n, const double *x, const double *y, double *partials,
-                    unsigned *ticket, double *scalars, double *out, int finish, hipStream_t s);
-int64_t dot_partials(int64_t n);
+// The dot's first pass: one workgroup per kDotPer elements, kDotE per
+// thread, one partial per workgroup; vectors of up to kDotParts * kDotPer
+// elements take it, longer ones the grid-stride reduction.
+constexpr int kDotE = 8;
+constexpr int64_t kDotPer = (int64_t)kBlock * kDotE;
+constexpr int64_t kDotParts = 131072;
+// launch_dot with explicit scratch: parts (kDotParts doubles), partials
+// (kReduceBlocks doubles), the completion ticket (0 on entry, 0 again on
+// exit) and the scalars
+void launch_dot_raw(int64_t n, const double *x, const double *y, double *parts,
+                    double *partials, unsigned *ticket, double *scalars, double *out, int finish,
+                    hipStream_t s);
 void launch_zero_u32(unsigned *a, int64_t n, hipStream_t s);
 // Sum of part[0..count) in fixed order, finished like launch_dot.
 void launch_partials_dot(Problem &P, const double *part, int64_t count, int finish,

@@ -49,21 +49,41 @@ __global__ __launch_bounds__(kBlock) void k_gather(
 // form is a template parameter chosen on the host exactly as the reference
 // chooses it.  w may alias x or y (each element is read before it is written
 // by the same thread).
-template <int FORM>
+// Workgroup b takes elements [b * kWaxE * kBlock, ...), thread q the
+// elements b * kWaxE * kBlock + q + e * kBlock: every load of a thread is
+// issued before its stores.  STREAM (vectors far larger than the last-level
+// cache): non-temporal loads and stores.
+constexpr int kWaxE = 4;
+template <int FORM, bool STREAM>
 __global__ __launch_bounds__(kBlock) void k_waxpby(int64_t n, double alpha,
                                                     const double *x,
                                                     double beta,
                                                     const double *y,
                                                     double *w) {
-  const int64_t i = gtid();
-  if (i >= n) return;
-  const double xi = x[i], yi = y[i];
-  if (FORM == 0)
-    w[i] = xi + beta * yi;
-  else if (FORM == 1)
-    w[i] = alpha * xi + yi;
-  else
-    w[i] = alpha * xi + beta * yi;
+  const int64_t t0 = (int64_t)blockIdx.x * (kBlock * kWaxE) + threadIdx.x;
+  double xv[kWaxE], yv[kWaxE];
+#pragma unroll
+  for (int e = 0; e < kWaxE; ++e) {
+    const int64_t i = t0 + (int64_t)e * kBlock;
+    xv[e] = i >= n ? 0.0 : STREAM ? __builtin_nontemporal_load(x + i) : x[i];
+    yv[e] = i >= n ? 0.0 : STREAM ? __builtin_nontemporal_load(y + i) : y[i];
+  }
+#pragma unroll
+  for (int e = 0; e < kWaxE; ++e) {
+    const int64_t i = t0 + (int64_t)e * kBlock;
+    if (i >= n) break;
+    double v;
+    if (FORM == 0)
+      v = xv[e] + beta * yv[e];
+    else if (FORM == 1)
+      v = alpha * xv[e] + yv[e];
+    else
+      v = alpha * xv[e] + beta * yv[e];
+    if (STREAM)
+      __builtin_nontemporal_store(v, w + i);
+    else
+      w[i] = v;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -892,6 +912,33 @@ __global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const double *x,
   grid_finish(s, partials, ticket, scalars, out, finish);
 }
 
+// The dot's first pass (ComputeDotProduct_ref.cpp:46-80 restated with a
+// fixed tree): workgroup b takes elements [b * kDotPer, ...), its thread q
+// the elements b * kDotPer + q + e * kBlock (e < kDotE, all loads issued
+// before the sum), in that order, then block_sum; the workgroup's partial is
+// stored plainly and k_chunk_dot sums the partials in a second launch.  The
+// grid-stride k_dot above finishes in its last workgroup, behind a
+// device-scope fence in each of its 4096 workgroups: 0.47 ms of loads plus
+// the fences took 0.62 ms on two vectors of 512x512x640 doubles, this
+// pass and k_chunk_dot 0.39 ms.
+template <bool STREAM>
+__global__ __launch_bounds__(kBlock) void k_dot_rows(int64_t n, const double *x, const double *y,
+                                                     double *__restrict__ parts) {
+  const int64_t t0 = (int64_t)blockIdx.x * kDotPer + threadIdx.x;
+  double a[kDotE], b[kDotE];
+#pragma unroll
+  for (int e = 0; e < kDotE; ++e) {
+    const int64_t t = t0 + (int64_t)e * kBlock;
+    a[e] = t < n ? mload<STREAM>(x + t) : 0.0;
+    b[e] = t < n ? mload<STREAM>(y + t) : 0.0;
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int e = 0; e < kDotE; ++e) s += a[e] * b[e];
+  s = block_sum(s);
+  if (threadIdx.x == 0) parts[blockIdx.x] = s;
+}
+
 // Ap = A p and p.Ap (CG.cpp:118-120) in one pass.  A persistent grid (a
 // multiple of 8 workgroups): XCD x (= blockIdx % 8) owns the x-th eighth of
 // the colour-interleaved chunk list and its workgroups stride through it, so
@@ -1408,27 +1455,45 @@ void launch_prolongation(const Level &L, const double *xc, double *xf,
 void launch_waxpby(int64_t n, double alpha, const double *x, double beta,
                    const double *y, double *w, hipStream_t s) {
   if (n <= 0) return;
-  if (alpha == 1.0)
-    k_waxpby<0><<<grid_for(n), kBlock, 0, s>>>(n, alpha, x, beta, y, w);
-  else if (beta == 1.0)
-    k_waxpby<1><<<grid_for(n), kBlock, 0, s>>>(n, alpha, x, beta, y, w);
-  else
-    k_waxpby<2><<<grid_for(n), kBlock, 0, s>>>(n, alpha, x, beta, y, w);
+  const unsigned g = (unsigned)((n + kBlock * kWaxE - 1) / (kBlock * kWaxE));
+  const bool st = stream_vec(n);
+#define HPCGX_WAX(F)                                                                           \
+  if (st)                                                                                      \
+    k_waxpby<F, true><<<g, kBlock, 0, s>>>(n, alpha, x, beta, y, w);                           \
+  else                                                                                         \
+    k_waxpby<F, false><<<g, kBlock, 0, s>>>(n, alpha, x, beta, y, w)
+  if (alpha == 1.0) {
+    HPCGX_WAX(0);
+  } else if (beta == 1.0) {
+    HPCGX_WAX(1);
+  } else {
+    HPCGX_WAX(2);
+  }
+#undef HPCGX_WAX
   HPCGX_LAUNCHED();
 }
 
-void launch_dot_raw(int64_t n, const double *x, const double *y, double *partials,
-                    unsigned *ticket, double *scalars, double *out, int finish, hipStream_t s) {
-  if (stream_vec(n))
+void launch_dot_raw(int64_t n, const double *x, const double *y, double *parts,
+                    double *partials, unsigned *ticket, double *scalars, double *out, int finish,
+                    hipStream_t s) {
+  const int64_t np = (n + kDotPer - 1) / kDotPer;
+  if (parts && np <= kDotParts) {
+    if (np > 0 && stream_vec(n))
+      k_dot_rows<true><<<(unsigned)np, kBlock, 0, s>>>(n, x, y, parts);
+    else if (np > 0)
+      k_dot_rows<false><<<(unsigned)np, kBlock, 0, s>>>(n, x, y, parts);
+    if (np > 0) HPCGX_LAUNCHED();
+    k_chunk_dot<<<reduce_grid(np), kBlock, 0, s>>>(np, parts, partials, ticket, scalars, finish,
+                                                   out);
+  } else if (stream_vec(n)) {
     k_dot<true><<<reduce_grid(n), kBlock, 0, s>>>(n, x, y, partials, ticket, scalars, out,
                                                   finish);
-  else
+  } else {
     k_dot<false><<<reduce_grid(n), kBlock, 0, s>>>(n, x, y, partials, ticket, scalars, out,
                                                    finish);
+  }
   HPCGX_LAUNCHED();
 }
-
-int64_t dot_partials(int64_t n) { return reduce_grid(n); }
 
 void launch_zero_u32(unsigned *a, int64_t n, hipStream_t s) {
   if (n <= 0) return;
@@ -1438,7 +1503,7 @@ void launch_zero_u32(unsigned *a, int64_t n, hipStream_t s) {
 
 void launch_dot(Problem &P, int64_t n, const double *x, const double *y,
                 double *out, int finish, hipStream_t s) {
-  launch_dot_raw(n, x, y, P.partials, P.ticket, P.scalars, out, finish, s);
+  launch_dot_raw(n, x, y, P.dot_parts, P.partials, P.ticket, P.scalars, out, finish, s);
 }
 
 void launch_partials_dot(Problem &P, const double *part, int64_t count, int finish,

@@ -2,8 +2,8 @@
 // that cannot allocate device memory (code under HIP graph capture, tight
 // serving loops, harnesses that forbid hipMalloc), with no problem handle.
 //
-// hpcgx_dot keeps its reduction scratch (one partial per workgroup and the
-// completion ticket of the last-workgroup finish) in the problem's arena.
+// hpcgx_dot keeps its reduction scratch (the partials of its two passes and
+// the completion ticket of the last-workgroup finish) in the problem's arena.
 // Here the same scratch is a file-scope device array, one copy per GPU for
 // the life of the process.  Its address is looked up once per device with
 // hipGetSymbolAddress (the array's name is only a host-side stand-in in host
@@ -17,11 +17,13 @@
 #include "internal.hpp"
 
 namespace {
-// The grid-stride reduction launches at most kReduceBlocks workgroups, so
-// this many partials cover every n.
-constexpr int64_t kScratchPartials = hpcgx::kReduceBlocks;
-constexpr size_t kTicketOffset = (size_t)kScratchPartials * sizeof(double);
-constexpr size_t kScratchBytes = kTicketOffset + 256;  // 32 KiB of partials + the ticket
+// hpcgx_dot's scratch: the first pass's partials (kDotParts), the partials
+// of the last-workgroup finish (at most kReduceBlocks workgroups) and the
+// ticket.  Those bounds hold for every n (longer vectors take the
+// grid-stride reduction, as in hpcgx_dot), so no length is refused.
+constexpr size_t kPartialsOffset = (size_t)hpcgx::kDotParts * sizeof(double);
+constexpr size_t kTicketOffset = kPartialsOffset + (size_t)hpcgx::kReduceBlocks * sizeof(double);
+constexpr size_t kScratchBytes = kTicketOffset + 256;  // 1 MiB + 32 KiB + the ticket
 constexpr int kMaxDevices = 64;
 }  // namespace
 
@@ -55,18 +57,16 @@ extern "C" int hpcgx_dot_noalloc(int64_t n, const double *x, const double *y,
     if (!result_dev) throw hpcgx::Error(HPCGX_EINVAL, "NULL argument");
     if (n < 0) throw hpcgx::Error(HPCGX_EINVAL, "negative length");
     if (n > 0 && (!x || !y)) throw hpcgx::Error(HPCGX_EINVAL, "NULL vector");
-    if (hpcgx::dot_partials(n) > kScratchPartials)
-      throw hpcgx::Error(HPCGX_EINVAL,
-                         "length needs more reduction partials than the static scratch holds");
     const hipStream_t s = static_cast<hipStream_t>(stream);
     unsigned char *base = device_scratch();
-    double *partials = reinterpret_cast<double *>(base);
+    double *parts = reinterpret_cast<double *>(base);
+    double *partials = reinterpret_cast<double *>(base + kPartialsOffset);
     unsigned *ticket = reinterpret_cast<unsigned *>(base + kTicketOffset);
     // The last-workgroup finish needs the ticket at 0.  Each launch leaves it
     // at 0, but a launch that never ran to its end would not: clear it with a
     // kernel (no memset node, so the call can be captured in a graph).
     hpcgx::launch_zero_u32(ticket, 1, s);
-    hpcgx::launch_dot_raw(n, x, y, partials, ticket, nullptr, result_dev, hpcgx::kFinishStore,
-                          s);
+    hpcgx::launch_dot_raw(n, x, y, parts, partials, ticket, nullptr, result_dev,
+                          hpcgx::kFinishStore, s);
   });
 }

@@ -226,10 +226,11 @@ int hpcgx_dot_host(hpcgx_problem *p, int64_t n, const double *x,
  * allocate device memory (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc), with no problem handle.  It runs
  * hpcgx_dot's kernels, so *result_dev is hpcgx_dot's value bit for bit; the
- * reduction scratch (4096 partials and a ticket, 32.25 KiB) is a static
- * device array that each GPU holds until the process ends.  Every n >= 0
- * fits it (the reduction launches at most 4096 workgroups); n < 0 gives
- * HPCGX_EINVAL.  It calls no hipMalloc or hipFree; the first call on a
+ * reduction scratch (131072 first-pass partials, 4096 finishing partials
+ * and a ticket, 1.03 MiB) is a static device array that each GPU holds
+ * until the process ends.  Every n >= 0 fits it (vectors longer than
+ * 131072 * 2048 elements take the grid-stride reduction, whose 4096
+ * workgroups fit the finishing partials); n < 0 gives HPCGX_EINVAL.  It calls no hipMalloc or hipFree; the first call on a
  * device looks up the array's address there (make that call before any
  * capture).  All calls on one GPU share its array: calls that overlap on one
  * GPU race. */

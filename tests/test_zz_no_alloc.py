@@ -25,8 +25,8 @@ def test_dot_noalloc_matches_dot():
     P = hpcg_amd.Problem.generate(16, 16, 16, nlevels=4)
     try:
         rng = np.random.default_rng(11)
-        # first call: a length whose reduction fills the 4096-workgroup grid
-        # (grid-stride loop); second: a smaller one, a partial grid
+        # first call: 2442 first-pass workgroups of 2048 elements, the last
+        # one partial; second: a smaller length, 35 workgroups
         for n in (5_000_003, 70_001):
             x = torch.from_numpy(rng.uniform(-1, 1, n)).cuda()
             y = torch.from_numpy(rng.uniform(-1, 1, n)).cuda()
@@ -62,3 +62,33 @@ def test_dot_noalloc_edges():
     assert "negative" in _lib.lib().hpcgx_last_error().decode()
     rc = _lib.lib().hpcgx_dot_noalloc(4, _ptr(x), _ptr(x), ctypes.c_void_p(0), _stream())
     assert rc == 1
+
+
+def test_dot_noalloc_long_vectors():
+    """Past 131072 first-pass workgroups (2^28 elements) both entry points
+    take the grid-stride reduction: still the same bits, and the sum within
+    the dot's bound of a float64 reference."""
+    import torch
+    import hpcg_amd
+    from hpcg_amd import _lib
+
+    torch.cuda.set_device(0)
+    P = hpcg_amd.Problem.generate(16, 16, 16, nlevels=4)
+    try:
+        n = 131072 * 2048 + 1
+        g = torch.Generator(device="cuda").manual_seed(5)
+        x = torch.rand(n, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+        y = torch.rand(n, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+        want = torch.full((1,), np.nan, dtype=torch.float64, device="cuda")
+        got = torch.full((1,), np.nan, dtype=torch.float64, device="cuda")
+        P.dot_device(x, y, want, n=n)
+        rc = _noalloc(n, x, y, got)
+        assert rc == 0, _lib.lib().hpcgx_last_error().decode()
+        torch.cuda.synchronize()
+        w, gv = float(want.item()), float(got.item())
+        assert np.float64(w).tobytes() == np.float64(gv).tobytes(), (w, gv)
+        ref = float(torch.dot(x, y))
+        assert abs(gv - ref) <= 2e-14 * float((x * y).abs().sum()), (gv, ref)
+        del x, y
+    finally:
+        P.close()
