@@ -457,6 +457,8 @@ void launch_step_x(int64_t n, double *x, const double *p, const double *scalars,
 void launch_dot32(Problem &P, int64_t n, const double *x, const float *y, double *out,
                   int finish, hipStream_t s);
 // stream: non-temporal vector accesses on the fine level (Problem::stream_p)
+void launch_update_xp32(int64_t n, double *x, const float *z, double *p,
+                        const double *scalars, hipStream_t s);
 void launch_update_p32(int64_t n, const float *z, double *p, const double *scalars,
                        hipStream_t s, bool stream);
 void launch_update_p(int64_t n, const double *z, double *p,

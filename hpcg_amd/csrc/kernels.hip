@@ -1148,6 +1148,21 @@ __global__ __launch_bounds__(kBlock) void k_update_xp(int64_t n, double *__restr
   }
 }
 
+// k_update_xp with z in FP32, widened exactly: the mixed CG's p update
+// (CG.cpp:115) with the previous iteration's deferred x step (:121).
+template <bool STREAM>
+__global__ __launch_bounds__(kBlock) void k_update_xp32(int64_t n, double *__restrict__ x,
+                                                         const float *__restrict__ z,
+                                                         double *__restrict__ p,
+                                                         const double *__restrict__ scalars) {
+  const int64_t i = gtid();
+  if (i < n) {
+    const double pi = mload<STREAM>(p + i);
+    mstore<STREAM>(x + i, mload<STREAM>(x + i) + scalars[kAlpha] * pi);
+    mstore<STREAM>(p + i, (double)mload<STREAM>(z + i) + scalars[kBeta] * pi);
+  }
+}
+
 // x += alpha p alone (the last iteration's deferred step).
 template <bool STREAM>
 __global__ __launch_bounds__(kBlock) void k_step_x(int64_t n, double *__restrict__ x,
@@ -1668,6 +1683,15 @@ void launch_dot32(Problem &P, int64_t n, const double *x, const float *y, double
   else
     k_dot32<false><<<reduce_grid(n), kBlock, 0, s>>>(n, x, y, P.partials, P.ticket, P.scalars,
                                                      out, finish);
+  HPCGX_LAUNCHED();
+}
+
+void launch_update_xp32(int64_t n, double *x, const float *z, double *p,
+                        const double *scalars, hipStream_t s) {
+  if (stream_vec(n))
+    k_update_xp32<true><<<grid_for(n), kBlock, 0, s>>>(n, x, z, p, scalars);
+  else
+    k_update_xp32<false><<<grid_for(n), kBlock, 0, s>>>(n, x, z, p, scalars);
   HPCGX_LAUNCHED();
 }
 

@@ -954,10 +954,12 @@ void mixed_iteration(Problem &P, int ordering, double *x, bool first, hipStream_
     launch_dot32(P, n, P.r, P.z32, nullptr, kFinishBeta, s);
   if (first)
     to_f64(n, P.z32, P.p, s);
+  else if (P.defer_x)  // with the previous iteration's x step (defer_x)
+    launch_update_xp32(n, x, P.z32, P.p, P.scalars, s);
   else
     launch_update_p32(n, P.z32, P.p, P.scalars, s, P.stream_p);
   launch_spmv_dot_half(P, A, P.p, P.Ap, s);
-  launch_update_xr(P, n, x, P.r, P.p, P.Ap, s, P.r32);
+  launch_update_xr(P, n, P.defer_x ? nullptr : x, P.r, P.p, P.Ap, s, P.r32);
 }
 
 // The captured body of iterations 2.. (rebuilt when its key changes; fills
@@ -1010,6 +1012,12 @@ void cg_mixed(Problem &P, int ordering, const double *b, double *x,
     else
       mixed_iteration(P, ordering, x, k == 1, s);
     if (recompute > 0 && k % recompute == 0) {
+      if (P.defer_x) {
+        // x's deferred step now; alpha then reads 0 in the next p update,
+        // whose x step (x + 0 p) leaves x as it is
+        launch_step_x(n, x, P.p, P.scalars, s);
+        launch_fill(P.scalars + kAlpha, 1, 0.0, s);
+      }
       // r = b - A x in FP64 (replaces the recursively updated r)
       launch_spmv(P, A, x, P.Ap, s);
       launch_residual0(P, n, b, P.Ap, P.r, s);
@@ -1018,6 +1026,10 @@ void cg_mixed(Problem &P, int ordering, const double *b, double *x,
     *niters = k;
     if (check_each || k == max_iter) nr = std::sqrt(read_scalar(P, kNormr2, s));
   }
+  // the last iteration's deferred x step (none after a recompute, which took
+  // it, or before the first iteration)
+  if (P.defer_x && *niters > 0 && !(recompute > 0 && *niters % recompute == 0))
+    launch_step_x(n, x, P.p, P.scalars, s);
   if (!check_each) nr = std::sqrt(read_scalar(P, kNormr2, s));
   HPCGX_HIP(hipStreamSynchronize(s));
   *normr = nr;

@@ -400,10 +400,12 @@ int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, i
  *                      FP64 finest level, bit 1 the FP64 coarser levels,
  *                      bit 2 the FP32 V-cycle, bit 3 the block V-cycle
  *                      (default 11; the same x)
- *   "defer_x"          1/0: hpcgx_cg takes each iteration's x += alpha p
- *                      with the next iteration's p update, in one pass over
- *                      x and p, and the last one after the loop (default 1;
- *                      the same x and residuals)
+ *   "defer_x"          1/0: hpcgx_cg and hpcgx_cg_mixed take each
+ *                      iteration's x += alpha p with the next iteration's p
+ *                      update, in one pass over x and p, and the last one
+ *                      after the loop (the mixed CG also before each FP64
+ *                      residual recompute) (default 1; the same x and
+ *                      residuals)
  *   "gs_block"         threads per workgroup of the finest level's FP64
  *                      colour steps: 64, 128, 256 or 512 (default 256)
  *   "gs_waves"         1/0: the finest level's FP64 colour steps run under

@@ -237,6 +237,36 @@ def test_cg_mixed_graph_replays(hp, tol):
     del P
 
 
+@pytest.mark.parametrize("iters,tol", [(12, 0.0), (10, 0.0), (19, 1e-6)])
+def test_cg_mixed_defer_x(hp, iters, tol):
+    """x's step taken with the next p update (defer_x) gives the bits of the
+    step in the r update, eager and captured: runs that end between
+    residual recomputes (12 iterations, recompute every 5), right after one
+    (10), and at a tolerance."""
+    import torch
+    dims = (32, 48, 64)
+    say("cg mixed defer_x", iters, tol)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    b, x, _ = P.vectors()
+    out = {}
+    try:
+        for defer in (0, 1):
+            P.tune("defer_x", defer)
+            for graph in (0, 1):
+                P.tune("mixed_graph", graph)
+                x.zero_()
+                r = P.cg_mixed(b, x, max_iter=iters, tolerance=tol, recompute=5)
+                out[(defer, graph)] = (r["niters"], r["normr"], x.clone())
+    finally:
+        P.tune("defer_x", 1)
+        P.tune("mixed_graph", 1)
+    ref = out[(0, 0)]
+    for k, v in out.items():
+        assert v[0] == ref[0] and v[1] == ref[1], k
+        assert torch.equal(v[2], ref[2]), k
+    del P
+
+
 def test_cg_mixed_rz_fused(hp):
     """r.z from the partials of the FP32 V-cycle's last sweep (r in FP64, z
     widened) against the dot kernel over r and z: the same sum in another
