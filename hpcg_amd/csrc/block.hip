@@ -735,6 +735,32 @@ __global__ __launch_bounds__(kBlock) void k_bupdate_p32(int64_t total, int s,
   Pv[t] = first ? (double)Z32[t] : (double)Z32[t] + sc[3 * s + k] * Pv[t];
 }
 
+// p = z + beta_k p with the previous iteration's x += alpha_k p taken in
+// the same pass (CG.cpp:115, :121; "defer_x"): alpha is still the previous
+// iteration's when this runs, before the SpMV's p.Ap sets the new one.
+__global__ __launch_bounds__(kBlock) void k_bupdate_xp32(int64_t total, int s,
+                                                         double *__restrict__ X,
+                                                         const float *__restrict__ Z32,
+                                                         double *__restrict__ Pv,
+                                                         const double *__restrict__ sc) {
+  const int64_t t = gtid();
+  if (t >= total) return;
+  const int k = (int)(t % s);
+  const double pv = Pv[t];
+  X[t] = X[t] + sc[2 * s + k] * pv;
+  Pv[t] = (double)Z32[t] + sc[3 * s + k] * pv;
+}
+
+// x += alpha_k p alone: a deferred step before the residual recompute and
+// after the last iteration
+__global__ __launch_bounds__(kBlock) void k_bstep_x(int64_t total, int s, double *__restrict__ X,
+                                                    const double *__restrict__ Pv,
+                                                    const double *__restrict__ sc) {
+  const int64_t t = gtid();
+  if (t >= total) return;
+  X[t] = X[t] + sc[2 * s + (int)(t % s)] * Pv[t];
+}
+
 // x += alpha_k p; r -= alpha_k Ap; r32 = (float)r; |r|^2 (CG.cpp:121-124)
 __global__ __launch_bounds__(kBlock) void k_bupdate_xr(int64_t total, int s,
                                                        double *__restrict__ X,
@@ -800,7 +826,9 @@ __global__ __launch_bounds__(kBlock) void k_bdot32_rows(int64_t total, int s,
   if (threadIdx.x < (unsigned)s) cpart[(int64_t)blockIdx.x * s + threadIdx.x] = c;
 }
 
-// k_bupdate_xr's update, |r|^2 per workgroup
+// k_bupdate_xr's update, |r|^2 per workgroup; WITH_X false: r, r32 and
+// |r|^2 only (x's step deferred to k_bupdate_xp32)
+template <bool WITH_X>
 __global__ __launch_bounds__(kBlock) void k_bupdate_xr_rows(int64_t total, int s,
                                                             double *__restrict__ X,
                                                             double *__restrict__ R,
@@ -816,7 +844,7 @@ __global__ __launch_bounds__(kBlock) void k_bupdate_xr_rows(int64_t total, int s
   for (int e = 0; e < kBE; ++e) {
     const int64_t t = t0 + (int64_t)e * kBlock;
     if (t < total) {
-      X[t] = X[t] + alpha * Pv[t];
+      if (WITH_X) X[t] = X[t] + alpha * Pv[t];
       const double r = R[t] + (-alpha) * AP[t];
       R[t] = r;
       R32[t] = (float)r;
@@ -1273,6 +1301,25 @@ void block_residual(Problem &P, int s, const double *Bv, const double *X, hipStr
   HPCGX_LAUNCHED();
 }
 
+// Whether the block iteration's vector kernels take the kBE-per-thread
+// form (s divides 64 and cpart holds their column sums), and whether x's
+// step is deferred to the next p update (defer_x, with that form).
+bool block_rows_vec(const Problem &P, int s) {
+  const auto &B = P.blk;
+  const int64_t total = P.L[0].n * s;
+  const int64_t nb = (total + kBlock * kBE - 1) / (kBlock * kBE);
+  return P.xr_rows && B.cpart && 64 % s == 0 && nb * s <= B.ncpart;
+}
+bool block_defer_x(const Problem &P, int s) { return P.defer_x && block_rows_vec(P, s); }
+
+// x's deferred step now (before a residual recompute, or after the loop)
+void block_step_x(Problem &P, int s, double *X, hipStream_t st) {
+  auto &B = P.blk;
+  const int64_t total = P.L[0].n * s;
+  k_bstep_x<<<grid_for(total), kBlock, 0, st>>>(total, s, X, B.Pv, B.scalars);
+  HPCGX_LAUNCHED();
+}
+
 // One block iteration without the recompute (CG.cpp:102-124 per column).
 void block_iteration(Problem &P, int ordering, int s, double *X, bool first, hipStream_t st) {
   auto &B = P.blk;
@@ -1281,7 +1328,8 @@ void block_iteration(Problem &P, int ordering, int s, double *X, bool first, hip
   // the kBE-per-thread vector kernels where s divides 64 and cpart holds
   // their column sums
   const int64_t nb = (total + kBlock * kBE - 1) / (kBlock * kBE);
-  const bool rows = P.xr_rows && B.cpart && 64 % s == 0 && nb * s <= B.ncpart;
+  const bool rows = block_rows_vec(P, s);
+  const bool defer = block_defer_x(P, s);
   const unsigned R = (unsigned)std::min<int64_t>(256, nb);
   mg32_block(P, 0, ordering, s, B.R32, B.Z32, st);
   if (rows) {
@@ -1292,12 +1340,20 @@ void block_iteration(Problem &P, int ordering, int s, double *X, bool first, hip
     k_bdot32<<<gr, kBlock, 0, st>>>(total, s, B.R, B.Z32, B.partials, B.ticket, B.scalars);
   }
   HPCGX_LAUNCHED();
-  k_bupdate_p32<<<grid_for(total), kBlock, 0, st>>>(total, s, B.Z32, B.Pv, B.scalars, first);
+  if (defer && !first)
+    k_bupdate_xp32<<<grid_for(total), kBlock, 0, st>>>(total, s, X, B.Z32, B.Pv, B.scalars);
+  else
+    k_bupdate_p32<<<grid_for(total), kBlock, 0, st>>>(total, s, B.Z32, B.Pv, B.scalars, first);
   HPCGX_LAUNCHED();
   block_spmv_dot(P, s, B.Pv, B.AP, true, st);
-  if (rows) {
-    k_bupdate_xr_rows<<<(unsigned)nb, kBlock, 0, st>>>(total, s, X, B.R, B.Pv, B.AP, B.R32,
-                                                       B.scalars, B.cpart);
+  if (rows && defer) {
+    k_bupdate_xr_rows<false><<<(unsigned)nb, kBlock, 0, st>>>(total, s, X, B.R, B.Pv, B.AP,
+                                                              B.R32, B.scalars, B.cpart);
+    HPCGX_LAUNCHED();
+    k_chunk_bdot<<<R, kBlock, 0, st>>>(nb, s, B.cpart, B.partials, B.ticket, B.scalars, kBStore);
+  } else if (rows) {
+    k_bupdate_xr_rows<true><<<(unsigned)nb, kBlock, 0, st>>>(total, s, X, B.R, B.Pv, B.AP, B.R32,
+                                                             B.scalars, B.cpart);
     HPCGX_LAUNCHED();
     k_chunk_bdot<<<R, kBlock, 0, st>>>(nb, s, B.cpart, B.partials, B.ticket, B.scalars, kBStore);
   } else {
@@ -1358,13 +1414,25 @@ void cg_block_mixed(Problem &P, int ordering, int s, const double *Bv, double *X
       HPCGX_HIP(hipGraphLaunch(block_graph(P, ordering, s, X), st));
     else
       block_iteration(P, ordering, s, X, it == 1, st);
-    if (recompute > 0 && it % recompute == 0) block_residual(P, s, Bv, X, st);
+    if (recompute > 0 && it % recompute == 0) {
+      if (block_defer_x(P, s)) {
+        // the pending x step now; alpha then reads 0 in the next p update,
+        // whose x step (x + 0 p) leaves x as it is
+        block_step_x(P, s, X, st);
+        launch_fill(P.blk.scalars + 2 * s, s, 0.0, st);
+      }
+      block_residual(P, s, Bv, X, st);
+    }
     *niters = it;
     if (check_each || it == max_iter) {
       read_block_scalars(P, s, n2, st);
       for (int k = 0; k < s; ++k) normr[k] = std::sqrt(n2[k]);
     }
   }
+  // the last iteration's deferred x step (none after a recompute, which took
+  // it, or before the first iteration)
+  if (block_defer_x(P, s) && *niters > 0 && !(recompute > 0 && *niters % recompute == 0))
+    block_step_x(P, s, X, st);
   if (!check_each) {
     read_block_scalars(P, s, n2, st);
     for (int k = 0; k < s; ++k) normr[k] = std::sqrt(n2[k]);

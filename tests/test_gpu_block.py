@@ -259,3 +259,33 @@ def test_block_xr_rows(hp, s):
     assert float((X1 - X0).abs().max()) <= 1e-7 * float(X0.abs().max())
     assert rg["normr"] == r1["normr"] and torch.equal(Xg, X1)
     del P
+
+
+@pytest.mark.parametrize("iters,tol", [(12, 0.0), (10, 0.0), (40, 1e-6)])
+def test_block_defer_x(hp, iters, tol):
+    """x's step taken with the next p update (defer_x) gives the bits of the
+    step in the r update, eager and captured: runs that end between residual
+    recomputes (12 iterations, recompute every 5), right after one (10), and
+    at a tolerance."""
+    import torch
+    dims, s = (32, 48, 64), 8
+    say("block defer_x", iters, tol)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    B = P.to_device_order_multi(rhs(P.nrow(0), s, 42), 0)
+    out = {}
+    try:
+        for defer in (0, 1):
+            P.tune("defer_x", defer)
+            for graph in (0, 1):
+                P.tune("mixed_graph", graph)
+                X = torch.zeros_like(B)
+                r = P.cg_block_mixed(B, X, max_iter=iters, tolerance=tol, recompute=5)
+                out[(defer, graph)] = (r["niters"], r["normr"], X)
+    finally:
+        P.tune("defer_x", 1)
+        P.tune("mixed_graph", 1)
+    ref = out[(0, 0)]
+    for k, v in out.items():
+        assert v[0] == ref[0] and v[1] == ref[1], k
+        assert torch.equal(v[2], ref[2]), k
+    del P
