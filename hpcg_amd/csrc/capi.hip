@@ -449,7 +449,8 @@ int hpcgx_tune(hpcgx_problem *p, const char *key, double value) {
       return;
     }
     if (k == "fleet_blocks") {
-      if (value < 1 || value > 65536) throw Error(HPCGX_EINVAL, "fleet_blocks must be in [1, 65536]");
+      if (value < 0 || value > 65536 || value != (int)value)
+        throw Error(HPCGX_EINVAL, "fleet_blocks must be an integer in [0, 65536]");
       P.fleet.grid = (int)value;
       return;
     }
@@ -546,7 +547,7 @@ int hpcgx_autotune(hpcgx_problem *p, int ordering, int iterations, char *report,
     std::vector<Knob> knobs;
     if (any_c16) knobs.push_back({"col8", {1, 0}});
     if (P.fleet.count) {
-      knobs.push_back({"fleet_blocks", {512, 1024, 2048, 4096}});
+      knobs.push_back({"fleet_blocks", {0, 512, 1024, 2048, 4096}});
     } else {
       knobs.push_back({"spmv_dot_chunks", {0, 1}});
       knobs.push_back({"spmv_dot_blocks", {1024, 2048, 4096, 8192}});

@@ -59,8 +59,12 @@ FleetView view_of(Problem &P, double tol) {
                    F.S,      F.live,    F.niters, F.nlive, tol};
 }
 
+// fleet_blocks 0: one workgroup per piece, each reduction finished by a
+// launch of its own; else a persistent grid of that many workgroups whose
+// last one finishes
 unsigned piece_grid(const Problem &P) {
-  const int64_t g = std::min<int64_t>(P.fleet.npieces, P.fleet.grid);
+  const int64_t g = P.fleet.grid == 0 ? P.fleet.npieces
+                                      : std::min<int64_t>(P.fleet.npieces, P.fleet.grid);
   return (unsigned)std::max<int64_t>(1, g);
 }
 
@@ -91,16 +95,11 @@ __device__ inline double fblock_sum(double v) {
 //   kFNorm       |r|^2 after the updates; one more iteration; still live?
 // Frozen problems (not live) keep their scalars.  The ticket is reset for
 // the next launch and nlive counts the live problems.
-__device__ void fleet_finish(const FleetView &F, int finish) {
-  __shared__ int is_last;
+// The finishing step itself, by one workgroup once every partial is
+// visible: the last workgroup of a piece kernel (fleet_finish) or a launch
+// of its own (k_ffinish, after a piece kernel with one workgroup per piece).
+__device__ void fleet_finish_sums(const FleetView &F, int finish) {
   __shared__ int live_s[kMaxFleet];
-  if (threadIdx.x == 0) {
-    __threadfence();
-    is_last = atomicAdd(F.ticket, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
   const int lane = threadIdx.x & 63;
   for (int q = threadIdx.x >> 6; q < F.count; q += kBlock / 64) {
     double v = 0.0;
@@ -141,8 +140,27 @@ __device__ void fleet_finish(const FleetView &F, int finish) {
     int c = 0;
     for (int q = 0; q < F.count; ++q) c += live_s[q];
     *F.nlive = c;
-    *F.ticket = 0u;
   }
+}
+
+__device__ void fleet_finish(const FleetView &F, int finish) {
+  __shared__ int is_last;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    is_last = atomicAdd(F.ticket, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  fleet_finish_sums(F, finish);
+  if (threadIdx.x == 0) *F.ticket = 0u;
+}
+
+// The finishing step in a launch of its own (one workgroup): the launch
+// boundary orders the piece kernel's partial stores before these loads, so
+// the piece kernel needs no fence or ticket.
+__global__ __launch_bounds__(kBlock) void k_ffinish(FleetView F, int finish) {
+  fleet_finish_sums(F, finish);
 }
 
 // The piece kernels: each workgroup takes pieces blockIdx.x, blockIdx.x +
@@ -158,6 +176,7 @@ __device__ inline Piece piece_at(const FleetView &F, int64_t k) {
 }
 
 // r = b - Ax (x's product is in Ap) and |r|^2 per problem (CG.cpp:88-89)
+template <bool SPLIT>
 __global__ __launch_bounds__(kBlock) void k_fresidual0(FleetView F, const double *__restrict__ b,
                                                        const double *__restrict__ Ap,
                                                        double *__restrict__ r) {
@@ -187,10 +206,11 @@ __global__ __launch_bounds__(kBlock) void k_fresidual0(FleetView F, const double
     s = fblock_sum(s);
     if (threadIdx.x == 0) F.partials[k] = s;
   }
-  fleet_finish(F, kFResidual0);
+  if (!SPLIT) fleet_finish(F, kFResidual0);
 }
 
 // x.y per live problem, finished as `finish`
+template <bool SPLIT>
 __global__ __launch_bounds__(kBlock) void k_fdot(FleetView F, const double *__restrict__ x,
                                                  const double *__restrict__ y, int finish) {
   for (int64_t k = blockIdx.x; k < F.npieces; k += gridDim.x) {
@@ -213,7 +233,7 @@ __global__ __launch_bounds__(kBlock) void k_fdot(FleetView F, const double *__re
     s = fblock_sum(s);
     if (threadIdx.x == 0) F.partials[k] = s;
   }
-  fleet_finish(F, finish);
+  if (!SPLIT) fleet_finish(F, finish);
 }
 
 // p = z (first iteration, CG.cpp:109) or p = z + beta_q p (CG.cpp:115)
@@ -229,6 +249,7 @@ __global__ __launch_bounds__(kBlock) void k_fupdate_p(FleetView F, const double 
 }
 
 // x += alpha_q p; r -= alpha_q Ap; |r|^2 (CG.cpp:121-124)
+template <bool SPLIT>
 __global__ __launch_bounds__(kBlock) void k_fupdate_xr(FleetView F, double *__restrict__ x,
                                                        double *__restrict__ r,
                                                        const double *__restrict__ p,
@@ -269,7 +290,7 @@ __global__ __launch_bounds__(kBlock) void k_fupdate_xr(FleetView F, double *__re
     s = fblock_sum(s);
     if (threadIdx.x == 0) F.partials[k] = s;
   }
-  fleet_finish(F, kFNorm);
+  if (!SPLIT) fleet_finish(F, kFNorm);
 }
 
 // out[c*(Q+1) + q] = first device row of colour c whose natural row is >=
@@ -297,14 +318,30 @@ void fleet_iteration(Problem &P, const FleetView &V, int ordering, bool precond,
     mg(P, 0, ordering, P.r, P.z, s);                          // CG.cpp:103
   else
     launch_copy(P.r, P.z, A.n, s);                            // :105
-  k_fdot<<<G, kBlock, 0, s>>>(V, P.r, P.z, kFBeta);           // :110, :113-114
+  const bool split = P.fleet.grid == 0;
+  if (split) {
+    k_fdot<true><<<G, kBlock, 0, s>>>(V, P.r, P.z, kFBeta);   // :110, :113-114
+    k_ffinish<<<1, kBlock, 0, s>>>(V, kFBeta);
+  } else {
+    k_fdot<false><<<G, kBlock, 0, s>>>(V, P.r, P.z, kFBeta);
+  }
   HPCGX_LAUNCHED();
   k_fupdate_p<<<G, kBlock, 0, s>>>(V, P.z, P.p, first ? 1 : 0);  // :109, :115
   HPCGX_LAUNCHED();
   launch_spmv(P, A, P.p, P.Ap, s);                            // :118
-  k_fdot<<<G, kBlock, 0, s>>>(V, P.p, P.Ap, kFAlpha);         // :119-120
+  if (split) {
+    k_fdot<true><<<G, kBlock, 0, s>>>(V, P.p, P.Ap, kFAlpha); // :119-120
+    k_ffinish<<<1, kBlock, 0, s>>>(V, kFAlpha);
+  } else {
+    k_fdot<false><<<G, kBlock, 0, s>>>(V, P.p, P.Ap, kFAlpha);
+  }
   HPCGX_LAUNCHED();
-  k_fupdate_xr<<<G, kBlock, 0, s>>>(V, x, P.r, P.p, P.Ap);    // :121-124
+  if (split) {
+    k_fupdate_xr<true><<<G, kBlock, 0, s>>>(V, x, P.r, P.p, P.Ap);  // :121-124
+    k_ffinish<<<1, kBlock, 0, s>>>(V, kFNorm);
+  } else {
+    k_fupdate_xr<false><<<G, kBlock, 0, s>>>(V, x, P.r, P.p, P.Ap);
+  }
   HPCGX_LAUNCHED();
 }
 
@@ -432,7 +469,12 @@ void cg_fleet(Problem &P, int ordering, const double *b, double *x, int max_iter
   const unsigned G = piece_grid(P);
   launch_copy(x, P.p, A.n, s);                                 // CG.cpp:86
   launch_spmv(P, A, P.p, P.Ap, s);                             // :87
-  k_fresidual0<<<G, kBlock, 0, s>>>(V, b, P.Ap, P.r);          // :88-89
+  if (P.fleet.grid == 0) {
+    k_fresidual0<true><<<G, kBlock, 0, s>>>(V, b, P.Ap, P.r);  // :88-89
+    k_ffinish<<<1, kBlock, 0, s>>>(V, kFResidual0);
+  } else {
+    k_fresidual0<false><<<G, kBlock, 0, s>>>(V, b, P.Ap, P.r);
+  }
   HPCGX_LAUNCHED();
   int nlive = read_nlive(P, s);
   const bool graphed = (flags & HPCGX_CG_GRAPH) && !P.klog.on && !P.use_wave;

@@ -325,7 +325,7 @@ struct Problem {
     int *live = nullptr;         // problem still iterating
     int *niters = nullptr;
     int *nlive = nullptr;
-    int grid = 2048;             // persistent grid of the piece kernels
+    int grid = 0;                // piece kernels' grid: 0 = one workgroup per piece
     IterGraph graph;             // captured fleet iteration
     double graph_tol = -1.0;
   } fleet;

@@ -452,7 +452,9 @@ int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, i
  *                      the colour blocks (default); 1: colour block by
  *                      colour block.  Changes the dots' summation order.
  *   "fleet_blocks"     workgroups of a fleet's persistent CG vector kernels
- *                      (1..65536, default 2048; fleets only)
+ *                      (1..65536), or 0: one workgroup per piece, each
+ *                      reduction finished by a one-workgroup launch of its
+ *                      own (default 0; fleets only; the same bits)
  * Synchronous.  Unknown keys return HPCGX_EINVAL. */
 int hpcgx_tune(hpcgx_problem *p, const char *key, double value);
 
