@@ -883,6 +883,16 @@ inline unsigned reduce_grid(int64_t n) {
   return (unsigned)(g < kReduceBlocks ? (g < 1 ? 1 : g) : kReduceBlocks);
 }
 
+// Workgroups of k_chunk_dot over n partials: about 8 per thread, at most
+// 256 workgroups.  Each workgroup of the last-workgroup finish passes a
+// device-scope fence; with reduce_grid's up to 4096 workgroups the fences,
+// not the few megabytes of partials, set the time (55-93 us per launch at
+// 512x512x640).
+inline unsigned chunk_grid(int64_t n) {
+  const int64_t g = (n + 8 * kBlock - 1) / (8 * kBlock);
+  return (unsigned)(g < 256 ? (g < 1 ? 1 : g) : 256);
+}
+
 // The grid-stride loops below visit i = gtid(), gtid() + stride, ... in that
 // order and add into one running sum in that order; the main loop loads
 // four of those rows before it uses any, so four rows' loads are in flight
@@ -1498,7 +1508,7 @@ void launch_dot_raw(int64_t n, const double *x, const double *y, double *parts,
     else if (np > 0)
       k_dot_rows<false><<<(unsigned)np, kBlock, 0, s>>>(n, x, y, parts);
     if (np > 0) HPCGX_LAUNCHED();
-    k_chunk_dot<<<reduce_grid(np), kBlock, 0, s>>>(np, parts, partials, ticket, scalars, finish,
+    k_chunk_dot<<<chunk_grid(np), kBlock, 0, s>>>(np, parts, partials, ticket, scalars, finish,
                                                    out);
   } else if (stream_vec(n)) {
     k_dot<true><<<reduce_grid(n), kBlock, 0, s>>>(n, x, y, partials, ticket, scalars, out,
@@ -1523,7 +1533,7 @@ void launch_dot(Problem &P, int64_t n, const double *x, const double *y,
 
 void launch_partials_dot(Problem &P, const double *part, int64_t count, int finish,
                          hipStream_t s) {
-  k_chunk_dot<<<reduce_grid(count), kBlock, 0, s>>>(count, part, P.partials, P.ticket,
+  k_chunk_dot<<<chunk_grid(count), kBlock, 0, s>>>(count, part, P.partials, P.ticket,
                                                     P.scalars, finish);
   HPCGX_LAUNCHED();
 }
@@ -1533,7 +1543,7 @@ bool chunked(const Problem &P, const Level &L) {
   return P.spmv_dot_chunked && P.chunk_partials && L.nchunks == P.nchunk_partials;
 }
 void chunk_dot(Problem &P, const Level &L, hipStream_t s) {
-  k_chunk_dot<<<reduce_grid(L.nchunks), kBlock, 0, s>>>(L.nchunks, P.chunk_partials, P.partials,
+  k_chunk_dot<<<chunk_grid(L.nchunks), kBlock, 0, s>>>(L.nchunks, P.chunk_partials, P.partials,
                                                         P.ticket, P.scalars, kFinishAlpha);
   HPCGX_LAUNCHED();
 }
@@ -1633,7 +1643,7 @@ void launch_update_xr(Problem &P, int64_t n, double *x, double *r,
     }
 #undef HPCGX_UXRR
     HPCGX_LAUNCHED();
-    k_chunk_dot<<<reduce_grid(g), kBlock, 0, s>>>(g, P.chunk_partials, P.partials, P.ticket,
+    k_chunk_dot<<<chunk_grid(g), kBlock, 0, s>>>(g, P.chunk_partials, P.partials, P.ticket,
                                                   P.scalars, kFinishStore, &P.scalars[kNormr2]);
     HPCGX_LAUNCHED();
     return;
