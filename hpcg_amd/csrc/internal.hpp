@@ -244,6 +244,8 @@ struct Problem {
   // bit 0 the FP64 finest level, bit 1 the FP64 coarser levels, bit 2 the
   // FP32 V-cycle, bit 3 the block V-cycle
   int zero_live = 11;
+  // the mixed CG's FP64 residual recompute in one pass ("resid_fused")
+  bool resid_fused = true;
   // block PCG finest colour steps: slot 3m + 2's x rows from slot 3m's by
   // W-lane moves on affine tiles ("block_pairs")
   bool block_pairs = false;
@@ -463,6 +465,11 @@ void launch_update_p32(int64_t n, const float *z, double *p, const double *scala
                        hipStream_t s, bool stream);
 void launch_update_p(int64_t n, const double *z, double *p,
                      const double *scalars, hipStream_t s, bool stream);
+// r = b - A x on the finest level L with r32 = (float)r and r.r (into the
+// scalars) in one pass over the row chunks; false (nothing launched) where
+// the chunk partials do not cover L
+bool launch_residual_fused(Problem &P, const Level &L, const double *b, const double *x,
+                           double *r, float *r32, hipStream_t s);
 void launch_residual0(Problem &P, int64_t n, const double *b, const double *Ap,
                       double *r, hipStream_t s);
 void launch_replace_diag(Level &L, const double *d, hipStream_t s);

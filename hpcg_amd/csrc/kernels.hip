@@ -1248,6 +1248,29 @@ __global__ __launch_bounds__(kBlock) void k_residual0(
   grid_finish(s, partials, ticket, scalars, &scalars[kNormr2], kFinishStore);
 }
 
+// r = b - A x, r32 = (float)r and one r.r partial per row chunk in one pass
+// (CG.cpp:86-89's residual, as the mixed CG recomputes it in FP64): k_spmv's
+// row sums, k_residual0's r and the FP32 rounding, without writing and
+// reading A x and r again; k_chunk_dot sums the partials.
+template <bool STREAM, bool C16>
+__global__ __launch_bounds__(kBlock) void k_spmv_resid(
+    const int64_t *__restrict__ chunks, int64_t n, Col16 C, const int32_t *__restrict__ col,
+    const double *__restrict__ val, const double *__restrict__ x, const double *__restrict__ b,
+    double *__restrict__ r, float *__restrict__ r32, double *__restrict__ cpart, bool xcd) {
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x, xcd);
+  const int64_t i = chunks[2 * blk] + threadIdx.x;
+  double s = 0.0;
+  if (i < chunks[2 * blk + 1]) {
+    const double y = C16 ? row_dot16<STREAM>(C, col, val, x, n, i) : row_dot<STREAM>(col, val, x, n, i);
+    const double ri = mload<STREAM>(b + i) + -1.0 * y;
+    mstore<STREAM>(r + i, ri);
+    mstore<STREAM>(r32 + i, (float)ri);
+    s = ri * ri;
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) cpart[blk] = s;
+}
+
 // p = z + beta p (CG.cpp:115)
 template <bool STREAM>
 __global__ __launch_bounds__(kBlock) void k_update_p(
@@ -1721,6 +1744,28 @@ void launch_update_p(int64_t n, const double *z, double *p,
   else
     k_update_p<false><<<grid_for(n), kBlock, 0, s>>>(n, z, p, scalars);
   HPCGX_LAUNCHED();
+}
+
+bool launch_residual_fused(Problem &P, const Level &L, const double *b, const double *x,
+                           double *r, float *r32, hipStream_t s) {
+  if (!P.resid_fused || !P.chunk_partials || L.nchunks != P.nchunk_partials) return false;
+  const unsigned g = (unsigned)L.nchunks;
+  const Col16 C = col16_of(P, L);
+  const bool c16 = use_col16(P, L, 0);
+#define HPCGX_SR(ST, CC)                                                                       \
+  k_spmv_resid<ST, CC><<<g, kBlock, 0, s>>>(L.chunks, L.n, C, L.col, L.val, x, b, r, r32,      \
+                                            P.chunk_partials, L.xcd)
+  if (L.stream && c16) HPCGX_SR(true, true);
+  else if (L.stream) HPCGX_SR(true, false);
+  else if (c16) HPCGX_SR(false, true);
+  else HPCGX_SR(false, false);
+#undef HPCGX_SR
+  HPCGX_LAUNCHED();
+  k_chunk_dot<<<chunk_grid(L.nchunks), kBlock, 0, s>>>(L.nchunks, P.chunk_partials, P.partials,
+                                                       P.ticket, P.scalars, kFinishStore,
+                                                       &P.scalars[kNormr2]);
+  HPCGX_LAUNCHED();
+  return true;
 }
 
 void launch_residual0(Problem &P, int64_t n, const double *b, const double *Ap,

@@ -1019,9 +1019,11 @@ void cg_mixed(Problem &P, int ordering, const double *b, double *x,
         launch_fill(P.scalars + kAlpha, 1, 0.0, s);
       }
       // r = b - A x in FP64 (replaces the recursively updated r)
-      launch_spmv(P, A, x, P.Ap, s);
-      launch_residual0(P, n, b, P.Ap, P.r, s);
-      to_f32(n, P.r, P.r32, s);
+      if (!launch_residual_fused(P, A, b, x, P.r, P.r32, s)) {
+        launch_spmv(P, A, x, P.Ap, s);
+        launch_residual0(P, n, b, P.Ap, P.r, s);
+        to_f32(n, P.r, P.r32, s);
+      }
     }
     *niters = k;
     if (check_each || k == max_iter) nr = std::sqrt(read_scalar(P, kNormr2, s));

@@ -394,6 +394,11 @@ int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, i
  *                      and x values of colours already updated in that
  *                      sweep (default 1; the same x as a zero fill followed
  *                      by the full sweep)
+ *   "resid_fused"      1/0: hpcgx_cg_mixed's FP64 residual recompute
+ *                      (r = b - A x, its FP32 copy and r.r) in one pass over
+ *                      the row chunks, or as SpMV, residual and rounding
+ *                      passes (default 1; the same r, r.r summed in another
+ *                      order)
  *   "zero_live"        0..15: the zero-x colour steps (see "zero_guess")
  *                      issue loads for the live slots only, eight per chunk,
  *                      instead of every slot through empty views; bit 0 the

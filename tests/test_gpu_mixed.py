@@ -267,6 +267,29 @@ def test_cg_mixed_defer_x(hp, iters, tol):
     del P
 
 
+def test_cg_mixed_resid_fused(hp):
+    """The FP64 residual recompute in one pass (resid_fused) gives the x of
+    the SpMV, residual and rounding passes, and r.r to rounding (summed in
+    another order)."""
+    import torch
+    dims = (32, 48, 64)
+    say("cg mixed resid_fused", dims)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    b, x, _ = P.vectors()
+    out = {}
+    try:
+        for v in (0, 1):
+            P.tune("resid_fused", v)
+            x.zero_()
+            r = P.cg_mixed(b, x, max_iter=12, tolerance=0.0, recompute=5)
+            out[v] = (r["normr"], x.clone())
+    finally:
+        P.tune("resid_fused", 1)
+    assert torch.equal(out[0][1], out[1][1])
+    assert out[1][0] == pytest.approx(out[0][0], rel=1e-12)
+    del P
+
+
 def test_cg_mixed_rz_fused(hp):
     """r.z from the partials of the FP32 V-cycle's last sweep (r in FP64, z
     widened) against the dot kernel over r and z: the same sum in another
