@@ -649,6 +649,38 @@ __global__ __launch_bounds__(kBlock) void k_bspmv_chunks(
   if (threadIdx.x < (unsigned)S) cpart[q * S + threadIdx.x] = t;
 }
 
+// R = B - A X, R32 = (float)R and the column sums of R.R over the rows of
+// one chunk per W workgroups (k_bspmv_chunks's rows; CG.cpp:86-89 per
+// column, as the periodic recompute needs it): one pass, where the SpMM and
+// k_bresidual0 wrote and read A X again.  Column sums to cpart[q * S + k].
+template <bool STREAM, int S, bool H>
+__global__ __launch_bounds__(kBlock) void k_bspmv_resid_chunks(
+    const int64_t *__restrict__ chunks, int64_t n, Col16 C, const uint32_t *__restrict__ vc,
+    const uint8_t *__restrict__ dslot, const double *__restrict__ diag,
+    const int32_t *__restrict__ col, const double *__restrict__ val,
+    const double *__restrict__ X, const double *__restrict__ Bv, double *__restrict__ R,
+    float *__restrict__ R32, double *__restrict__ cpart) {
+  constexpr int V = 2, W = S / V;
+  const int64_t q = blockIdx.x, b = q / W;
+  const int64_t i = chunks[2 * b] + (q % W) * (kBlock / W) + threadIdx.x / W;
+  const int o = (int)(threadIdx.x % W) * V;
+  double acc[V] = {0.0, 0.0};
+  if (i < chunks[2 * b + 1]) {
+    double y[V], bv[V];
+    bspmv_row<STREAM, S, H>(i, o, n, C, vc, dslot, diag, col, val, X, y);
+    ldv<V>(Bv + i * S + o, bv);
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      const double r = bv[k] - y[k];
+      R[i * S + o + k] = r;
+      R32[i * S + o + k] = (float)r;
+      acc[k] = r * r;
+    }
+  }
+  const double t = bcolumn_sum<S>(acc);
+  if (threadIdx.x < (unsigned)S) cpart[q * S + threadIdx.x] = t;
+}
+
 // p_k . (A p)_k from k_bspmv_chunks's partials (a launch of its own, so
 // their stores are ordered before these loads), in fixed order: thread t of
 // workgroup w sums column t % s of parts lo + t / s, + kBlock / s, ...;
@@ -1292,9 +1324,40 @@ void block_spmv_dot(Problem &P, int s, const double *x, double *y, bool dot, hip
 }
 
 // r = b - A x, r32, |r|^2 (CG.cpp:86-89, and the periodic recompute)
+template <int S>
+void block_residual_fused_s(Problem &P, const double *Bv, const double *X, hipStream_t st) {
+  auto &B = P.blk;
+  const Level &L = P.L[0];
+  const Col16 C = col16_of(P, L);
+  const bool h = L.vc && P.half_exact && use_col16(P, L, 0);
+  constexpr int W = S / 2;
+  const unsigned G = (unsigned)(L.nchunks * W);
+#define HPCGX_BRC(ST, HH)                                                                    \
+  k_bspmv_resid_chunks<ST, S, HH><<<G, kBlock, 0, st>>>(L.chunks, L.n, C, L.vc, L.dslot,      \
+                                                        L.diag, L.col, L.val, X, Bv, B.R,     \
+                                                        B.R32, B.cpart)
+  if (L.stream && h) HPCGX_BRC(true, true);
+  else if (L.stream) HPCGX_BRC(true, false);
+  else if (h) HPCGX_BRC(false, true);
+  else HPCGX_BRC(false, false);
+#undef HPCGX_BRC
+  HPCGX_LAUNCHED();
+  const int64_t nparts = L.nchunks * W;
+  const unsigned R = (unsigned)std::min<int64_t>(256, nparts);
+  k_chunk_bdot<<<R, kBlock, 0, st>>>(nparts, S, B.cpart, B.partials, B.ticket, B.scalars, kBStore);
+  HPCGX_LAUNCHED();
+}
+
 void block_residual(Problem &P, int s, const double *Bv, const double *X, hipStream_t st) {
   auto &B = P.blk;
   const int64_t total = P.L[0].n * s;
+  if (P.resid_fused && rows_kernels(s) && P.spmv_dot_chunked && B.cpart &&
+      B.ncpart >= P.L[0].nchunks * (s / 2) * s) {
+    if (s == 4) block_residual_fused_s<4>(P, Bv, X, st);
+    else if (s == 8) block_residual_fused_s<8>(P, Bv, X, st);
+    else block_residual_fused_s<16>(P, Bv, X, st);
+    return;
+  }
   block_spmv_dot(P, s, X, B.AP, false, st);
   k_bresidual0<<<block_reduce_grid(total, s), kBlock, 0, st>>>(total, s, Bv, B.AP, B.R, B.R32,
                                                                B.partials, B.ticket, B.scalars);

@@ -394,7 +394,8 @@ int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, i
  *                      and x values of colours already updated in that
  *                      sweep (default 1; the same x as a zero fill followed
  *                      by the full sweep)
- *   "resid_fused"      1/0: hpcgx_cg_mixed's FP64 residual recompute
+ *   "resid_fused"      1/0: hpcgx_cg_mixed's (and, for s = 4, 8, 16,
+ *                      hpcgx_cg_block_mixed's) FP64 residual recompute
  *                      (r = b - A x, its FP32 copy and r.r) in one pass over
  *                      the row chunks, or as SpMV, residual and rounding
  *                      passes (default 1; the same r, r.r summed in another

@@ -289,3 +289,28 @@ def test_block_defer_x(hp, iters, tol):
         assert v[0] == ref[0] and v[1] == ref[1], k
         assert torch.equal(v[2], ref[2]), k
     del P
+
+
+def test_block_resid_fused(hp):
+    """The block residual (start and recomputes) in one pass (resid_fused)
+    gives the X of the SpMM and residual passes, and the norms to rounding
+    (column sums in another order)."""
+    import torch
+    dims, s = (32, 48, 64), 8
+    say("block resid_fused", dims, s)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    B = P.to_device_order_multi(rhs(P.nrow(0), s, 42), 0)
+    out = {}
+    try:
+        for v in (0, 1):
+            P.tune("resid_fused", v)
+            X = torch.zeros_like(B)
+            r = P.cg_block_mixed(B, X, max_iter=12, tolerance=0.0, recompute=5)
+            out[v] = (r, X)
+    finally:
+        P.tune("resid_fused", 1)
+    assert torch.equal(out[0][1], out[1][1])
+    for k in range(s):
+        assert out[1][0]["normr0"][k] == pytest.approx(out[0][0]["normr0"][k], rel=1e-13), k
+        assert out[1][0]["normr"][k] == pytest.approx(out[0][0]["normr"][k], rel=1e-12), k
+    del P
