@@ -767,20 +767,56 @@ __global__ __launch_bounds__(kBlock) void k_bupdate_p32(int64_t total, int s,
   Pv[t] = first ? (double)Z32[t] : (double)Z32[t] + sc[3 * s + k] * Pv[t];
 }
 
+// elements per thread of the block vector kernels that take the workgroup-
+// per-kBlock*kBE layout (k_bupdate_xp32, k_bdot32_rows, k_bupdate_xr_rows)
+constexpr int kBE = 4;
+
+// Multivector loads and stores of those kernels; NT: non-temporal (the
+// multivectors are far larger than the last-level cache)
+template <bool NT, typename T>
+__device__ inline T vld(const T *p) {
+  if (NT) return __builtin_nontemporal_load(p);
+  return *p;
+}
+template <bool NT, typename T>
+__device__ inline void vst(T *p, T v) {
+  if (NT)
+    __builtin_nontemporal_store(v, p);
+  else
+    *p = v;
+}
+
 // p = z + beta_k p with the previous iteration's x += alpha_k p taken in
 // the same pass (CG.cpp:115, :121; "defer_x"): alpha is still the previous
 // iteration's when this runs, before the SpMV's p.Ap sets the new one.
+// Taken with the kBE-per-thread layout of k_bupdate_xr_rows (s divides 64,
+// so the column is threadIdx.x % s).
+template <bool NT>
 __global__ __launch_bounds__(kBlock) void k_bupdate_xp32(int64_t total, int s,
                                                          double *__restrict__ X,
                                                          const float *__restrict__ Z32,
                                                          double *__restrict__ Pv,
                                                          const double *__restrict__ sc) {
-  const int64_t t = gtid();
-  if (t >= total) return;
-  const int k = (int)(t % s);
-  const double pv = Pv[t];
-  X[t] = X[t] + sc[2 * s + k] * pv;
-  Pv[t] = (double)Z32[t] + sc[3 * s + k] * pv;
+  const int64_t t0 = (int64_t)blockIdx.x * (kBlock * kBE) + threadIdx.x;
+  const int k = (int)(threadIdx.x % s);
+  const double alpha = sc[2 * s + k], beta = sc[3 * s + k];
+  double xv[kBE], pv[kBE];
+  float zv[kBE];
+#pragma unroll
+  for (int e = 0; e < kBE; ++e) {
+    const int64_t t = t0 + (int64_t)e * kBlock;
+    xv[e] = t < total ? vld<NT>(X + t) : 0.0;
+    pv[e] = t < total ? vld<NT>(Pv + t) : 0.0;
+    zv[e] = t < total ? vld<NT>(Z32 + t) : 0.0f;
+  }
+#pragma unroll
+  for (int e = 0; e < kBE; ++e) {
+    const int64_t t = t0 + (int64_t)e * kBlock;
+    if (t < total) {
+      vst<NT>(X + t, xv[e] + alpha * pv[e]);
+      vst<NT>(Pv + t, (double)zv[e] + beta * pv[e]);
+    }
+  }
 }
 
 // x += alpha_k p alone: a deferred step before the residual recompute and
@@ -823,7 +859,6 @@ __global__ __launch_bounds__(kBlock) void k_bupdate_xr(int64_t total, int s,
 // own, sums them in a fixed order and finishes.  The grid-stride forms above
 // keep a small grid and the last workgroup's finish; these launch a
 // workgroup per kBlock * kBE elements, so more loads are in flight.
-constexpr int kBE = 4;
 
 // Column sums of a workgroup in a fixed order: within each wavefront a
 // shuffle tree over the lanes of a column (lanes k, k + s, ...), then the
@@ -843,6 +878,7 @@ __device__ inline double bwave_column_sum(double v, int s) {
 }
 
 // r_k . z_k per workgroup (k_bdot32's sums)
+template <bool NT>
 __global__ __launch_bounds__(kBlock) void k_bdot32_rows(int64_t total, int s,
                                                         const double *__restrict__ R,
                                                         const float *__restrict__ Z32,
@@ -852,7 +888,7 @@ __global__ __launch_bounds__(kBlock) void k_bdot32_rows(int64_t total, int s,
 #pragma unroll
   for (int e = 0; e < kBE; ++e) {
     const int64_t t = t0 + (int64_t)e * kBlock;
-    if (t < total) acc += R[t] * (double)Z32[t];
+    if (t < total) acc += vld<NT>(R + t) * (double)vld<NT>(Z32 + t);
   }
   const double c = bwave_column_sum(acc, s);
   if (threadIdx.x < (unsigned)s) cpart[(int64_t)blockIdx.x * s + threadIdx.x] = c;
@@ -860,7 +896,7 @@ __global__ __launch_bounds__(kBlock) void k_bdot32_rows(int64_t total, int s,
 
 // k_bupdate_xr's update, |r|^2 per workgroup; WITH_X false: r, r32 and
 // |r|^2 only (x's step deferred to k_bupdate_xp32)
-template <bool WITH_X>
+template <bool WITH_X, bool NT>
 __global__ __launch_bounds__(kBlock) void k_bupdate_xr_rows(int64_t total, int s,
                                                             double *__restrict__ X,
                                                             double *__restrict__ R,
@@ -876,10 +912,10 @@ __global__ __launch_bounds__(kBlock) void k_bupdate_xr_rows(int64_t total, int s
   for (int e = 0; e < kBE; ++e) {
     const int64_t t = t0 + (int64_t)e * kBlock;
     if (t < total) {
-      if (WITH_X) X[t] = X[t] + alpha * Pv[t];
-      const double r = R[t] + (-alpha) * AP[t];
-      R[t] = r;
-      R32[t] = (float)r;
+      if (WITH_X) vst<NT>(X + t, vld<NT>(X + t) + alpha * vld<NT>(Pv + t));
+      const double r = vld<NT>(R + t) + (-alpha) * vld<NT>(AP + t);
+      vst<NT>(R + t, r);
+      vst<NT>(R32 + t, (float)r);
       acc += r * r;
     }
   }
@@ -1395,28 +1431,46 @@ void block_iteration(Problem &P, int ordering, int s, double *X, bool first, hip
   const bool defer = block_defer_x(P, s);
   const unsigned R = (unsigned)std::min<int64_t>(256, nb);
   mg32_block(P, 0, ordering, s, B.R32, B.Z32, st);
+  const bool nt = P.block_nt && total >= ((int64_t)1 << 24);
   if (rows) {
-    k_bdot32_rows<<<(unsigned)nb, kBlock, 0, st>>>(total, s, B.R, B.Z32, B.cpart);
+    if (nt)
+      k_bdot32_rows<true><<<(unsigned)nb, kBlock, 0, st>>>(total, s, B.R, B.Z32, B.cpart);
+    else
+      k_bdot32_rows<false><<<(unsigned)nb, kBlock, 0, st>>>(total, s, B.R, B.Z32, B.cpart);
     HPCGX_LAUNCHED();
     k_chunk_bdot<<<R, kBlock, 0, st>>>(nb, s, B.cpart, B.partials, B.ticket, B.scalars, kBBeta);
   } else {
     k_bdot32<<<gr, kBlock, 0, st>>>(total, s, B.R, B.Z32, B.partials, B.ticket, B.scalars);
   }
   HPCGX_LAUNCHED();
-  if (defer && !first)
-    k_bupdate_xp32<<<grid_for(total), kBlock, 0, st>>>(total, s, X, B.Z32, B.Pv, B.scalars);
+  if (defer && !first && nt)
+    k_bupdate_xp32<true><<<(unsigned)nb, kBlock, 0, st>>>(total, s, X, B.Z32, B.Pv, B.scalars);
+  else if (defer && !first)
+    k_bupdate_xp32<false><<<(unsigned)nb, kBlock, 0, st>>>(total, s, X, B.Z32, B.Pv, B.scalars);
   else
     k_bupdate_p32<<<grid_for(total), kBlock, 0, st>>>(total, s, B.Z32, B.Pv, B.scalars, first);
   HPCGX_LAUNCHED();
   block_spmv_dot(P, s, B.Pv, B.AP, true, st);
   if (rows && defer) {
-    k_bupdate_xr_rows<false><<<(unsigned)nb, kBlock, 0, st>>>(total, s, X, B.R, B.Pv, B.AP,
-                                                              B.R32, B.scalars, B.cpart);
+    if (nt)
+      k_bupdate_xr_rows<false, true><<<(unsigned)nb, kBlock, 0, st>>>(total, s, X, B.R, B.Pv,
+                                                                      B.AP, B.R32, B.scalars,
+                                                                      B.cpart);
+    else
+      k_bupdate_xr_rows<false, false><<<(unsigned)nb, kBlock, 0, st>>>(total, s, X, B.R, B.Pv,
+                                                                       B.AP, B.R32, B.scalars,
+                                                                       B.cpart);
     HPCGX_LAUNCHED();
     k_chunk_bdot<<<R, kBlock, 0, st>>>(nb, s, B.cpart, B.partials, B.ticket, B.scalars, kBStore);
   } else if (rows) {
-    k_bupdate_xr_rows<true><<<(unsigned)nb, kBlock, 0, st>>>(total, s, X, B.R, B.Pv, B.AP, B.R32,
-                                                             B.scalars, B.cpart);
+    if (nt)
+      k_bupdate_xr_rows<true, true><<<(unsigned)nb, kBlock, 0, st>>>(total, s, X, B.R, B.Pv,
+                                                                     B.AP, B.R32, B.scalars,
+                                                                     B.cpart);
+    else
+      k_bupdate_xr_rows<true, false><<<(unsigned)nb, kBlock, 0, st>>>(total, s, X, B.R, B.Pv,
+                                                                      B.AP, B.R32, B.scalars,
+                                                                      B.cpart);
     HPCGX_LAUNCHED();
     k_chunk_bdot<<<R, kBlock, 0, st>>>(nb, s, B.cpart, B.partials, B.ticket, B.scalars, kBStore);
   } else {

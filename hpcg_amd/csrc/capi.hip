@@ -420,6 +420,10 @@ int hpcgx_tune(hpcgx_problem *p, const char *key, double value) {
       P.gs_block = (int)value;
       return;
     }
+    if (k == "block_nt") {
+      P.block_nt = value != 0.0;
+      return;
+    }
     if (k == "resid_fused") {
       P.resid_fused = value != 0.0;
       return;

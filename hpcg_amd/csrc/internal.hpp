@@ -246,6 +246,9 @@ struct Problem {
   int zero_live = 11;
   // the mixed CG's FP64 residual recompute in one pass ("resid_fused")
   bool resid_fused = true;
+  // non-temporal multivector loads and stores in the block CG's vector
+  // kernels ("block_nt")
+  bool block_nt = true;
   // block PCG finest colour steps: slot 3m + 2's x rows from slot 3m's by
   // W-lane moves on affine tiles ("block_pairs")
   bool block_pairs = false;

@@ -394,6 +394,10 @@ int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, i
  *                      and x values of colours already updated in that
  *                      sweep (default 1; the same x as a zero fill followed
  *                      by the full sweep)
+ *   "block_nt"         1/0: the block PCG's vector kernels (r.z, the p and
+ *                      r updates) use non-temporal loads and stores on
+ *                      multivectors of 2^24 or more values (default 1; the
+ *                      same bits)
  *   "resid_fused"      1/0: hpcgx_cg_mixed's (and, for s = 4, 8, 16,
  *                      hpcgx_cg_block_mixed's) FP64 residual recompute
  *                      (r = b - A x, its FP32 copy and r.r) in one pass over

@@ -314,3 +314,26 @@ def test_block_resid_fused(hp):
         assert out[1][0]["normr0"][k] == pytest.approx(out[0][0]["normr0"][k], rel=1e-13), k
         assert out[1][0]["normr"][k] == pytest.approx(out[0][0]["normr"][k], rel=1e-12), k
     del P
+
+
+def test_block_nt_same_bits(hp):
+    """Non-temporal multivector loads and stores in the block vector kernels
+    (block_nt; multivectors of 2^24 or more values, here 128^3 x 8) change
+    no bit of X or the norms."""
+    import torch
+    dims, s = (128, 128, 128), 8
+    say("block nt", dims, s)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    B = P.to_device_order_multi(rhs(P.nrow(0), s, 42), 0)
+    out = {}
+    try:
+        for v in (0, 1):
+            P.tune("block_nt", v)
+            X = torch.zeros_like(B)
+            r = P.cg_block_mixed(B, X, max_iter=12, tolerance=0.0, recompute=5)
+            out[v] = (r["normr"], X)
+    finally:
+        P.tune("block_nt", 1)
+    assert out[0][0] == out[1][0]
+    assert torch.equal(out[0][1], out[1][1])
+    del P
