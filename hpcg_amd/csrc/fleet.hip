@@ -101,7 +101,7 @@ __device__ inline double fblock_sum(double v) {
 __device__ void fleet_finish_sums(const FleetView &F, int finish) {
   __shared__ int live_s[kMaxFleet];
   const int lane = threadIdx.x & 63;
-  for (int q = threadIdx.x >> 6; q < F.count; q += kBlock / 64) {
+  for (int q = threadIdx.x >> 6; q < F.count; q += (int)(blockDim.x >> 6)) {
     double v = 0.0;
     for (int k = F.pptr[q] + lane; k < F.pptr[q + 1]; k += 64)
       v += __hip_atomic_load(&F.partials[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -156,10 +156,12 @@ __device__ void fleet_finish(const FleetView &F, int finish) {
   if (threadIdx.x == 0) *F.ticket = 0u;
 }
 
-// The finishing step in a launch of its own (one workgroup): the launch
+// The finishing step in a launch of its own (one workgroup of kFinishThreads,
+// a wavefront per problem in two rounds for 32 problems): the launch
 // boundary orders the piece kernel's partial stores before these loads, so
 // the piece kernel needs no fence or ticket.
-__global__ __launch_bounds__(kBlock) void k_ffinish(FleetView F, int finish) {
+constexpr int kFinishThreads = 1024;
+__global__ __launch_bounds__(kFinishThreads) void k_ffinish(FleetView F, int finish) {
   fleet_finish_sums(F, finish);
 }
 
@@ -321,7 +323,7 @@ void fleet_iteration(Problem &P, const FleetView &V, int ordering, bool precond,
   const bool split = P.fleet.grid == 0;
   if (split) {
     k_fdot<true><<<G, kBlock, 0, s>>>(V, P.r, P.z, kFBeta);   // :110, :113-114
-    k_ffinish<<<1, kBlock, 0, s>>>(V, kFBeta);
+    k_ffinish<<<1, kFinishThreads, 0, s>>>(V, kFBeta);
   } else {
     k_fdot<false><<<G, kBlock, 0, s>>>(V, P.r, P.z, kFBeta);
   }
@@ -331,14 +333,14 @@ void fleet_iteration(Problem &P, const FleetView &V, int ordering, bool precond,
   launch_spmv(P, A, P.p, P.Ap, s);                            // :118
   if (split) {
     k_fdot<true><<<G, kBlock, 0, s>>>(V, P.p, P.Ap, kFAlpha); // :119-120
-    k_ffinish<<<1, kBlock, 0, s>>>(V, kFAlpha);
+    k_ffinish<<<1, kFinishThreads, 0, s>>>(V, kFAlpha);
   } else {
     k_fdot<false><<<G, kBlock, 0, s>>>(V, P.p, P.Ap, kFAlpha);
   }
   HPCGX_LAUNCHED();
   if (split) {
     k_fupdate_xr<true><<<G, kBlock, 0, s>>>(V, x, P.r, P.p, P.Ap);  // :121-124
-    k_ffinish<<<1, kBlock, 0, s>>>(V, kFNorm);
+    k_ffinish<<<1, kFinishThreads, 0, s>>>(V, kFNorm);
   } else {
     k_fupdate_xr<false><<<G, kBlock, 0, s>>>(V, x, P.r, P.p, P.Ap);
   }
@@ -471,7 +473,7 @@ void cg_fleet(Problem &P, int ordering, const double *b, double *x, int max_iter
   launch_spmv(P, A, P.p, P.Ap, s);                             // :87
   if (P.fleet.grid == 0) {
     k_fresidual0<true><<<G, kBlock, 0, s>>>(V, b, P.Ap, P.r);  // :88-89
-    k_ffinish<<<1, kBlock, 0, s>>>(V, kFResidual0);
+    k_ffinish<<<1, kFinishThreads, 0, s>>>(V, kFResidual0);
   } else {
     k_fresidual0<false><<<G, kBlock, 0, s>>>(V, b, P.Ap, P.r);
   }
