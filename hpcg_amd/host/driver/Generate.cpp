@@ -132,7 +132,9 @@ void GenerateProblem(SparseMatrix &A, Vector *b, Vector *x, Vector *xexact) {
   A.localNumberOfColumns = n;  // one rank: no halo columns
   A.localNumberOfNonzeros = (local_int_t)(nnz > 0x7fffffffLL ? 0x7fffffff : nnz);
   A.localToGlobalMap.resize(n);
-  for (local_int_t i = 0; i < n; ++i) A.localToGlobalMap[i] = i;
+  global_int_t *l2g = A.localToGlobalMap.data();
+#pragma omp parallel for schedule(static)
+  for (local_int_t i = 0; i < n; ++i) l2g[i] = i;
 }
 
 void GenerateCoarseProblem(const SparseMatrix &Af) {
