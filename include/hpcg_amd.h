@@ -410,8 +410,8 @@ int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, i
  *                      FP64 finest level, bit 1 the FP64 coarser levels,
  *                      bit 2 the FP32 V-cycle, bit 3 the block V-cycle
  *                      (default 11; the same x)
- *   "defer_x"          1/0: hpcgx_cg, hpcgx_cg_mixed and hpcgx_cg_block_mixed
- *                      (where "xr_rows" applies) take each
+ *   "defer_x"          1/0: hpcgx_cg, hpcgx_cg_mixed and (where "xr_rows"
+ *                      applies) hpcgx_cg_block_mixed take each
  *                      iteration's x += alpha p with the next iteration's p
  *                      update, in one pass over x and p, and the last one
  *                      after the loop (the mixed CG also before each FP64
