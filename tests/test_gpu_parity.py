@@ -416,9 +416,12 @@ def test_wavefront_bitexact(hp, dims, slab_factor):
     del P
 
 
-def test_waxpby_bitexact(hp, torch):
-    say("waxpby")
-    n = 100003
+@pytest.mark.parametrize("n", [100003, (1 << 24) + 5])
+def test_waxpby_bitexact(hp, torch, n):
+    """All three forms, and the in-place ones, at a length with a partial
+    last workgroup and at one past 2^24 elements (non-temporal loads and
+    stores)."""
+    say("waxpby", n)
     x, y = rnd(70, n), rnd(71, n)
     X = torch.tensor(x, device="cuda")
     Y = torch.tensor(y, device="cuda")
