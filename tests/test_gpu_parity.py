@@ -438,7 +438,7 @@ def test_waxpby_bitexact(hp, torch, n):
     assert np.array_equal(Wy.cpu().numpy(), O.waxpby(1.0, x, 0.25, y))
 
 
-@pytest.mark.parametrize("n", [1, 255, 256, 4097, 1 << 20, 3_000_017])
+@pytest.mark.parametrize("n", [1, 255, 256, 4097, 1 << 20, 3_000_017, (1 << 24) + 3])
 def test_dot(hp, torch, n):
     say("dot", n)
     P = problem(hp, (16, 16, 16))
