@@ -161,6 +161,28 @@ def test_fleet_graph_replays(fleet, tol):
         fleet.cg_fleet(b, x, max_iter=2, tolerance=0.0)
 
 
+@pytest.mark.parametrize("tol", [0.0, 1e-6])
+def test_fleet_blocks_same_bits(fleet, tol):
+    """The piece kernels with one workgroup per piece and a finishing launch
+    of their own (fleet_blocks 0) give the bits of a persistent grid whose
+    last workgroup finishes (fleet_blocks 3, several pieces per workgroup),
+    stopping counts included."""
+    import torch
+    say("fleet blocks", tol)
+    b, x, _ = fleet.vectors()
+    out = {}
+    try:
+        for g in (0, 3):
+            fleet.tune("fleet_blocks", g)
+            x.zero_()
+            r = fleet.cg_fleet(b, x, max_iter=40, tolerance=tol)
+            out[g] = (r["niters"], r["normr"], x.clone())
+    finally:
+        fleet.tune("fleet_blocks", 0)
+    assert out[0][0] == out[3][0] and out[0][1] == out[3][1]
+    assert torch.equal(out[0][2], out[3][2])
+
+
 def test_fleet_autotune(hp):
     say("fleet autotune")
     P = hp.Problem.generate_fleet(hp.fleet_dims(count=6, seed=7, sizes=(16, 32, 48)))
