@@ -162,6 +162,20 @@ def test_512_packed_words(hp):
         res[c16] = P.cg_mixed(b, xx, max_iter=3, tolerance=0.0, recompute=10)
     assert res[1]["normr"] == pytest.approx(res[0]["normr"], rel=1e-12)
     assert res[1]["normr"] / res[1]["normr0"] < 0.2
+    # the FP64 residual recompute in one pass at this size (int64 entry
+    # offsets past 2^32) against the SpMV, residual and rounding passes
+    P.tune("col16", 1)
+    fused = {}
+    try:
+        for v in (0, 1):
+            P.tune("resid_fused", v)
+            xx.zero_()
+            r = P.cg_mixed(b, xx, max_iter=4, tolerance=0.0, recompute=2)
+            fused[v] = (r["normr"], xx.clone())
+    finally:
+        P.tune("resid_fused", 1)
+    assert torch.equal(fused[0][1], fused[1][1])
+    assert fused[1][0] == pytest.approx(fused[0][0], rel=1e-12)
     del P
 
 
