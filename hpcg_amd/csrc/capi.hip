@@ -428,6 +428,10 @@ int hpcgx_tune(hpcgx_problem *p, const char *key, double value) {
       P.resid_fused = value != 0.0;
       return;
     }
+    if (k == "hc_pairs") {
+      P.hc_pairs = value != 0.0;
+      return;
+    }
     if (k == "zero_live") {
       if (value < 0 || value > 15 || value != (int)value)
         throw Error(HPCGX_EINVAL, "zero_live must be an integer from 0 to 15");

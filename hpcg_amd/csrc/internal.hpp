@@ -246,6 +246,9 @@ struct Problem {
   int zero_live = 11;
   // the mixed CG's FP64 residual recompute in one pass ("resid_fused")
   bool resid_fused = true;
+  // the mixed CG's FP64 A p reads 8-bit tiles as FP16 value pairs and 8-bit
+  // offsets, not 4-byte offset/value words ("hc_pairs")
+  bool hc_pairs = true;
   // non-temporal multivector loads and stores in the block CG's vector
   // kernels ("block_nt")
   bool block_nt = true;

@@ -1028,19 +1028,56 @@ __device__ inline double row_dot_h(const Col16 &C, const uint32_t *__restrict__ 
   return y;
 }
 
+// row_dot_h on an 8-bit tile from the FP16 value pairs (vp, kPairWords words)
+// and the 8-bit offsets (none loaded where the tile's amask bits say affine):
+// the same FP16 values, columns and slot order, 2 to 3 bytes per slot in
+// place of 4.  Other tiles take row_dot_h.
+template <bool STREAM>
+__device__ inline double row_dot_hp(const Col16 &C, const uint32_t *__restrict__ vp,
+                                    const uint32_t *__restrict__ vc,
+                                    const uint8_t *__restrict__ dslot,
+                                    const double *__restrict__ diag,
+                                    const int32_t *__restrict__ col,
+                                    const double *__restrict__ val,
+                                    const double *__restrict__ p, int64_t n, int64_t i) {
+  const int t = tile_of(i);
+  if (C.info[t] != kTile8 || !C.use8)
+    return row_dot_h<STREAM>(C, vc, dslot, diag, col, val, p, n, i);
+  const int32_t *__restrict__ base = C.tbase + (int64_t)t * kBaseStride;
+  const unsigned am = C.amask ? C.amask[t] : 0u;
+  uint32_t o8[kSlotWords], pv[kPairWords];
+#pragma unroll
+  for (int g = 0; g < kSlotWords; ++g) o8[g] = off8_word<STREAM>(C.off8, am, g, n, i);
+#pragma unroll
+  for (int w = 0; w < kPairWords; ++w) pv[w] = mload<STREAM>(vp + (int64_t)w * n + i);
+  const int ds = mload<STREAM>(dslot + i);
+  const double di = mload<STREAM>(diag + i);
+  double y = 0.0;
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int32_t c = base[j] + (int32_t)((o8[j / 4] >> (8 * (j % 4))) & 0xffu);
+    const uint16_t h = (uint16_t)(pv[j / 2] >> (16 * (j % 2)));
+    const double a = j == ds ? di : (double)__builtin_bit_cast(_Float16, h);
+    y += a * p[c];
+  }
+  return y;
+}
+
 // k_spmv_dot_h with one workgroup per row chunk; partials as k_spmv_dot_c.
+// vp non-null: 8-bit tiles take row_dot_hp.
 template <bool STREAM>
 __global__ __launch_bounds__(kBlock) void k_spmv_dot_hc(
     const int64_t *__restrict__ chunks, int64_t n, Col16 C, const uint32_t *__restrict__ vc,
-    const uint8_t *__restrict__ dslot, const double *__restrict__ diag,
-    const int32_t *__restrict__ col, const double *__restrict__ val,
-    const double *__restrict__ p, double *__restrict__ Ap, double *__restrict__ cpart,
-    bool xcd) {
+    const uint32_t *__restrict__ vp, const uint8_t *__restrict__ dslot,
+    const double *__restrict__ diag, const int32_t *__restrict__ col,
+    const double *__restrict__ val, const double *__restrict__ p, double *__restrict__ Ap,
+    double *__restrict__ cpart, bool xcd) {
   const int64_t b = xcd_block(blockIdx.x, gridDim.x, xcd);
   const int64_t i = chunks[2 * b] + threadIdx.x;
   double s = 0.0;
   if (i < chunks[2 * b + 1]) {
-    const double y = row_dot_h<STREAM>(C, vc, dslot, diag, col, val, p, n, i);
+    const double y = vp ? row_dot_hp<STREAM>(C, vp, vc, dslot, diag, col, val, p, n, i)
+                        : row_dot_h<STREAM>(C, vc, dslot, diag, col, val, p, n, i);
     mstore<STREAM>(Ap + i, y);
     s = p[i] * y;
   }
@@ -1622,12 +1659,14 @@ void launch_spmv_dot_half(Problem &P, const Level &L, const double *p, double *A
   const Col16 C = col16_of(P, L);
   if (chunked(P, L)) {
     const unsigned G = (unsigned)L.nchunks;
+    const uint32_t *vp =
+        P.hc_pairs && L.v16p && P.use_c8 && L.byte_tiles > 0 ? L.v16p : nullptr;
     if (L.stream)
-      k_spmv_dot_hc<true><<<G, kBlock, 0, s>>>(L.chunks, L.n, C, L.vc, L.dslot, L.diag, L.col,
-                                               L.val, p, Ap, P.chunk_partials, L.xcd);
+      k_spmv_dot_hc<true><<<G, kBlock, 0, s>>>(L.chunks, L.n, C, L.vc, vp, L.dslot, L.diag,
+                                               L.col, L.val, p, Ap, P.chunk_partials, L.xcd);
     else
-      k_spmv_dot_hc<false><<<G, kBlock, 0, s>>>(L.chunks, L.n, C, L.vc, L.dslot, L.diag, L.col,
-                                                L.val, p, Ap, P.chunk_partials, L.xcd);
+      k_spmv_dot_hc<false><<<G, kBlock, 0, s>>>(L.chunks, L.n, C, L.vc, vp, L.dslot, L.diag,
+                                                L.col, L.val, p, Ap, P.chunk_partials, L.xcd);
     HPCGX_LAUNCHED();
     chunk_dot(P, L, s);
     return;

@@ -404,6 +404,10 @@ int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, i
  *                      the row chunks, or as SpMV, residual and rounding
  *                      passes (default 1; the same r, r.r summed in another
  *                      order)
+ *   "hc_pairs"         1/0: hpcgx_cg_mixed's FP64 A p reads the 8-bit
+ *                      column tiles as FP16 value pairs and 8-bit offsets
+ *                      (those of the FP32 V-cycle) rather than 4-byte
+ *                      offset/value words (default 1; the same bits)
  *   "zero_live"        0..15: the zero-x colour steps (see "zero_guess")
  *                      issue loads for the live slots only, eight per chunk,
  *                      instead of every slot through empty views; bit 0 the

@@ -304,6 +304,34 @@ def test_cg_mixed_resid_fused(hp):
     del P
 
 
+@pytest.mark.parametrize("dims", [(32, 48, 64), (64, 64, 64)])
+def test_cg_mixed_hc_pairs(hp, dims):
+    """The FP64 A p from the FP16 value pairs and 8-bit offsets (hc_pairs)
+    gives the bits of the 4-byte offset/value words: the same x and r.r,
+    with the affine tiles' offsets implied and with every offset loaded."""
+    import torch
+    say("cg mixed hc_pairs", dims)
+    P = hp.Problem.generate(*dims, nlevels=4)
+    assert P.info(0)["byte_tiles"] > 0
+    b, x, _ = P.vectors()
+    out = {}
+    try:
+        for aff in (1, 0):
+            P.tune("affine8", aff)
+            for v in (0, 1):
+                P.tune("hc_pairs", v)
+                x.zero_()
+                r = P.cg_mixed(b, x, max_iter=12, tolerance=0.0, recompute=5)
+                out[aff, v] = (r["normr"], x.clone())
+    finally:
+        P.tune("hc_pairs", 1)
+        P.tune("affine8", 1)
+    for aff in (1, 0):
+        assert torch.equal(out[aff, 0][1], out[aff, 1][1]), aff
+        assert out[aff, 1][0] == out[aff, 0][0], aff
+    del P
+
+
 def test_cg_mixed_rz_fused(hp):
     """r.z from the partials of the FP32 V-cycle's last sweep (r in FP64, z
     widened) against the dot kernel over r and z: the same sum in another
