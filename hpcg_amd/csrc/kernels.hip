@@ -1308,6 +1308,32 @@ __global__ __launch_bounds__(kBlock) void k_spmv_resid(
   if (threadIdx.x == 0) cpart[blk] = s;
 }
 
+// k_spmv_resid from the FP16 values and the FP64 diagonal of a matrix whose
+// off-diagonal values FP16 holds exactly (row_dot_h, or row_dot_hp on 8-bit
+// tiles where vp is non-null): the same products in the same order, so the
+// same r.
+template <bool STREAM>
+__global__ __launch_bounds__(kBlock) void k_spmv_resid_h(
+    const int64_t *__restrict__ chunks, int64_t n, Col16 C, const uint32_t *__restrict__ vc,
+    const uint32_t *__restrict__ vp, const uint8_t *__restrict__ dslot,
+    const double *__restrict__ diag, const int32_t *__restrict__ col,
+    const double *__restrict__ val, const double *__restrict__ x, const double *__restrict__ b,
+    double *__restrict__ r, float *__restrict__ r32, double *__restrict__ cpart, bool xcd) {
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x, xcd);
+  const int64_t i = chunks[2 * blk] + threadIdx.x;
+  double s = 0.0;
+  if (i < chunks[2 * blk + 1]) {
+    const double y = vp ? row_dot_hp<STREAM>(C, vp, vc, dslot, diag, col, val, x, n, i)
+                        : row_dot_h<STREAM>(C, vc, dslot, diag, col, val, x, n, i);
+    const double ri = mload<STREAM>(b + i) + -1.0 * y;
+    mstore<STREAM>(r + i, ri);
+    mstore<STREAM>(r32 + i, (float)ri);
+    s = ri * ri;
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) cpart[blk] = s;
+}
+
 // p = z + beta p (CG.cpp:115)
 template <bool STREAM>
 __global__ __launch_bounds__(kBlock) void k_update_p(
@@ -1791,14 +1817,22 @@ bool launch_residual_fused(Problem &P, const Level &L, const double *b, const do
   const unsigned g = (unsigned)L.nchunks;
   const Col16 C = col16_of(P, L);
   const bool c16 = use_col16(P, L, 0);
+  const bool h = P.hc_pairs && L.vc && P.half_exact && c16;
+  const uint32_t *vp = h && L.v16p && P.use_c8 && L.byte_tiles > 0 ? L.v16p : nullptr;
 #define HPCGX_SR(ST, CC)                                                                       \
   k_spmv_resid<ST, CC><<<g, kBlock, 0, s>>>(L.chunks, L.n, C, L.col, L.val, x, b, r, r32,      \
                                             P.chunk_partials, L.xcd)
-  if (L.stream && c16) HPCGX_SR(true, true);
+#define HPCGX_SRH(ST)                                                                          \
+  k_spmv_resid_h<ST><<<g, kBlock, 0, s>>>(L.chunks, L.n, C, L.vc, vp, L.dslot, L.diag, L.col,  \
+                                          L.val, x, b, r, r32, P.chunk_partials, L.xcd)
+  if (h && L.stream) HPCGX_SRH(true);
+  else if (h) HPCGX_SRH(false);
+  else if (L.stream && c16) HPCGX_SR(true, true);
   else if (L.stream) HPCGX_SR(true, false);
   else if (c16) HPCGX_SR(false, true);
   else HPCGX_SR(false, false);
 #undef HPCGX_SR
+#undef HPCGX_SRH
   HPCGX_LAUNCHED();
   k_chunk_dot<<<chunk_grid(L.nchunks), kBlock, 0, s>>>(L.nchunks, P.chunk_partials, P.partials,
                                                        P.ticket, P.scalars, kFinishStore,

@@ -407,7 +407,11 @@ int hpcgx_cg_fleet(hpcgx_problem *p, int ordering, const double *b, double *x, i
  *   "hc_pairs"         1/0: hpcgx_cg_mixed's FP64 A p reads the 8-bit
  *                      column tiles as FP16 value pairs and 8-bit offsets
  *                      (those of the FP32 V-cycle) rather than 4-byte
- *                      offset/value words (default 1; the same bits)
+ *                      offset/value words, and its one-pass residual
+ *                      recompute ("resid_fused") reads the FP16 values and
+ *                      the FP64 diagonal rather than the FP64 matrix, where
+ *                      FP16 holds every off-diagonal value exactly
+ *                      (default 1; the same bits)
  *   "zero_live"        0..15: the zero-x colour steps (see "zero_guess")
  *                      issue loads for the live slots only, eight per chunk,
  *                      instead of every slot through empty views; bit 0 the
