@@ -332,6 +332,70 @@ def test_cg_mixed_hc_pairs(hp, dims):
     del P
 
 
+def test_cg_mixed_hc_pairs_wide_tiles(hp):
+    """A finest level with 8-bit and wider tiles: boundary rows of the 27-
+    point problem get symmetric -1 couplings to distant rows of another
+    colour (the greedy colours stay as they were, so the colour blocks stay
+    on 64-row tiles, and the tiles holding the far columns leave the 8-bit
+    format).  The FP64 SpMV equals the oracle's; the mixed CG with hc_pairs
+    gives the bits of the packed-word path (8-bit tiles from the FP16 pairs,
+    the others from the packed words) and the oracle's trajectory."""
+    import torch
+    dims = (16, 16, 24)
+    say("hc_pairs wide tiles", dims)
+    h = O.Hierarchy(*dims)
+    m = h.level(0)  # live views: the oracle sees the added couplings too
+    n = h.nrow(0)
+    nnz, cols, vals = m["nnz_row"], m["cols"].reshape(n, 27), m["vals"].reshape(n, 27)
+    color, _ = O.colors(m)
+    spare = np.flatnonzero(nnz < 27)
+    lo, hi = spare[spare < n // 4], spare[spare >= 3 * n // 4][::-1]
+    added = 0
+    for i, j in zip(lo[::7], hi[::7]):
+        if color[i] == color[j] or nnz[i] >= 27 or nnz[j] >= 27:
+            continue
+        cols[i, nnz[i]], vals[i, nnz[i]] = j, -1.0
+        cols[j, nnz[j]], vals[j, nnz[j]] = i, -1.0
+        nnz[i] += 1
+        nnz[j] += 1
+        added += 1
+    assert added >= 10
+    c2, nc2 = O.colors(m)
+    assert nc2 == 8 and np.array_equal(c2, color)
+    levels = []
+    for l in range(4):
+        ml = h.level(l)
+        d = dict(nnz_row=ml["nnz_row"], cols=ml["cols"], vals=ml["vals"],
+                 diag_slot=ml["diag_slot"])
+        if l < 3:
+            d["f2c"] = ml["f2c"]
+        levels.append(d)
+    h.use_color_order()
+    P = hp.Problem.from_reference_levels(levels)
+    P.prepare_mixed()
+    inf = P.info(0)
+    assert inf["col16"] and 0 < inf["byte_tiles"] < n // 64, inf
+    xv = np.random.default_rng(3).uniform(-1, 1, n)
+    y = P.empty(0)
+    P.spmv(P.to_device_order(xv), y)
+    assert np.array_equal(P.to_numpy(y), O.spmv(m, xv))
+    want = h.cg_mixed(max_iter=12, tol=0.0, recompute=5)["traj"]
+    b, x = P.to_device_order(h.b), P.zeros()
+    out = {}
+    try:
+        for v in (0, 1):
+            P.tune("hc_pairs", v)
+            x.zero_()
+            r = P.cg_mixed(b, x, max_iter=12, tolerance=0.0, recompute=5)
+            out[v] = (r["normr"], x.clone())
+            assert r["normr"] / r["normr0"] == pytest.approx(want[12], rel=1e-9), v
+    finally:
+        P.tune("hc_pairs", 1)
+    assert torch.equal(out[0][1], out[1][1])
+    assert out[0][0] == out[1][0]
+    del P
+
+
 def test_cg_mixed_rz_fused(hp):
     """r.z from the partials of the FP32 V-cycle's last sweep (r in FP64, z
     widened) against the dot kernel over r and z: the same sum in another
