@@ -379,8 +379,21 @@ def test_cg_mixed_hc_pairs_wide_tiles(hp):
     y = P.empty(0)
     P.spmv(P.to_device_order(xv), y)
     assert np.array_equal(P.to_numpy(y), O.spmv(m, xv))
-    want = h.cg_mixed(max_iter=12, tol=0.0, recompute=5)["traj"]
+    # the V-cycles (zero-x first sweeps over both tile kinds) give the
+    # oracle's bits; the FP64 CG its trajectory
+    rr = np.random.default_rng(4).uniform(-1, 1, n)
+    z = P.zeros()
+    P.mg(P.to_device_order(rr), z, ordering="multicolor")
+    assert np.array_equal(P.to_numpy(z), h.mg(rr))
+    z32 = P.zeros32()
+    P.mg32(P.to_device_order(rr).float(), z32)
+    assert np.array_equal(P.to_numpy(z32.double()).astype(np.float32),
+                          h.mg32(rr.astype(np.float32)))
     b, x = P.to_device_order(h.b), P.zeros()
+    want64 = h.cg(max_iter=12, tol=0.0)["traj"]
+    r = P.cg(b, x, max_iter=12, tolerance=0.0, ordering="multicolor")
+    assert r["normr"] / r["normr0"] == pytest.approx(want64[12], rel=1e-9)
+    want = h.cg_mixed(max_iter=12, tol=0.0, recompute=5)["traj"]
     out = {}
     try:
         for v in (0, 1):
